@@ -1,0 +1,261 @@
+/* sundog_api.h — public C ABI of libsundog.so (MI355X / gfx950 build).
+ *
+ * Part 1 declares the reference ABI with identical names, argument types and
+ * struct layouts, so a ctypes call program written against the reference
+ * (scenes/scenelib.py:724-880) drives this library unchanged. Each group
+ * cites the reference declaration it replaces (src/sundog_api.h:<line>).
+ *
+ * Conventions (reference src/sundog_api.h:1-15, src/capi_internal.h:26-56):
+ *   - doubles are narrowed to float exactly once at the boundary;
+ *   - "unset" is NaN for doubles, NULL for vectors, -1 for tri-state ints;
+ *   - set_* / validate / probe / render return 0 (ok) or 1 (error);
+ *     add_* return an id >= 0, or -1; create returns NULL on error;
+ *   - the message of the last failure is in sundog_last_error()
+ *     (thread-local);
+ *   - call order is config/registries -> objects -> flames -> explicit
+ *     lights -> render, which freezes the light-list layout
+ *     [auto NEE area lights in object order][2 per flame][explicit].
+ *
+ * Part 2 holds additive entry points with no counterpart in the reference:
+ * resident render sessions (scene data and film kept in HBM across passes),
+ * ray-query and sampling probes used by parity tests, and the flat scene
+ * view that the CPU checker consumes. They change no Part-1 symbol.
+ */
+#ifndef SUNDOG_API_H
+#define SUNDOG_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define SUNDOG_API extern "C" __attribute__((visibility("default")))
+#else
+#define SUNDOG_API __attribute__((visibility("default")))
+#endif
+
+/* ======================================================================== */
+/* Part 1 — reference ABI                                                   */
+/* ======================================================================== */
+
+/* error model: reference src/sundog_api.h:27-30 */
+#define SUNDOG_OK 0
+#define SUNDOG_ERROR 1
+SUNDOG_API const char* sundog_last_error(void);
+
+/* lifetime: reference src/sundog_api.h:32-35 */
+typedef struct sundog_scene sundog_scene;
+SUNDOG_API sundog_scene* sundog_scene_create(const char* base_dir);
+SUNDOG_API void sundog_scene_destroy(sundog_scene*);
+
+/* enums: reference src/sundog_api.h:37-54 */
+enum {
+  SUNDOG_GK_SPHERE = 0,
+  SUNDOG_GK_RECT = 1,
+  SUNDOG_GK_DISK = 2,
+  SUNDOG_GK_CYLINDER = 3,
+  SUNDOG_GK_PARABOLA = 4,
+  SUNDOG_GK_MESH = 5,
+};
+enum {
+  SUNDOG_XF_SCALE = 0,
+  SUNDOG_XF_TRANSLATE = 1,
+  SUNDOG_XF_ROTATE_X = 2,
+  SUNDOG_XF_ROTATE_Y = 3,
+  SUNDOG_XF_ROTATE_Z = 4,
+};
+#define SUNDOG_MAT_NONE (-1)
+#define SUNDOG_MAT_DEFAULT (-2)
+
+/* config blocks: reference src/sundog_api.h:56-76 */
+SUNDOG_API int sundog_set_render(sundog_scene*, int width, int height, int spp,
+                                 int max_depth, double clamp, int64_t seed,
+                                 double exposure, int transparent_shadows);
+SUNDOG_API int sundog_set_camera(sundog_scene*, const double lookfrom[3],
+                                 const double lookat[3], const double up[3],
+                                 double vfov, double aperture, double focus_dist);
+SUNDOG_API int sundog_set_background_solid(sundog_scene*, const double color[3]);
+SUNDOG_API int sundog_set_background_gradient(sundog_scene*, const double horizon[3],
+                                              const double zenith[3]);
+SUNDOG_API int sundog_set_background_envmap(sundog_scene*, const char* file,
+                                            double rotate_deg, double intensity,
+                                            int importance);
+SUNDOG_API int sundog_set_physics(sundog_scene*, const double gravity[3],
+                                  double timestep, double max_time,
+                                  double friction, double restitution,
+                                  int pos_iters, int vel_iters,
+                                  double sleep_threshold, double stop_time);
+
+/* registries: reference src/sundog_api.h:78-111 */
+SUNDOG_API int sundog_add_texture_solid(sundog_scene*, const double color[3]);
+SUNDOG_API int sundog_add_texture_checker(sundog_scene*, const double a[3],
+                                          const double b[3], const double scale[2]);
+SUNDOG_API int sundog_add_texture_grid(sundog_scene*, const double a[3],
+                                       const double b[3], const double scale[2],
+                                       double width);
+SUNDOG_API int sundog_add_texture_image(sundog_scene*, const char* file, int srgb);
+SUNDOG_API int sundog_add_material_lambert(sundog_scene*, const double color[3],
+                                           int tex_id);
+SUNDOG_API int sundog_add_material_metal(sundog_scene*, const double color[3],
+                                         int tex_id, double roughness);
+SUNDOG_API int sundog_add_material_dielectric(sundog_scene*, double ior,
+                                              const double absorb[3],
+                                              const double color[3], int tex_id,
+                                              double roughness);
+SUNDOG_API int sundog_add_material_emissive(sundog_scene*, const double color[3],
+                                            int tex_id, double intensity,
+                                            int two_sided);
+SUNDOG_API int sundog_add_material_water(sundog_scene*, double ior,
+                                         const double absorb[3], double wave_amp,
+                                         double wave_freq, const double color[3]);
+SUNDOG_API int sundog_add_material_plastic(sundog_scene*, const double color[3],
+                                           int tex_id, double roughness);
+SUNDOG_API int sundog_add_mesh(sundog_scene*, const char* obj_file,
+                               int smooth_normals, const char* usemtl);
+
+/* objects: reference src/sundog_api.h:113-138 */
+typedef struct {
+  int32_t kind;
+  double a, b, c;
+} sundog_xform_step;
+
+typedef struct {
+  int32_t dynamic;
+  double density;
+  int32_t has_velocity, has_angular_velocity;
+  double velocity[3], angular_velocity[3];
+  double friction, restitution;
+  double thickness;
+} sundog_physics_body;
+
+SUNDOG_API int sundog_add_object(sundog_scene*, int geom_kind, int mesh_id,
+                                 int mat_front, int mat_back, int cutout_tex_id,
+                                 const sundog_xform_step* steps, int num_steps,
+                                 int nee, const sundog_physics_body* physics);
+
+/* flames and explicit lights: reference src/sundog_api.h:140-152 */
+SUNDOG_API int sundog_add_flame(sundog_scene*, const double base[3], double height,
+                                double radius, double intensity, double sigma,
+                                double noise_scale, int64_t seed,
+                                double light_intensity);
+SUNDOG_API int sundog_add_point_light(sundog_scene*, const double position[3],
+                                      const double intensity[3], double radius);
+SUNDOG_API int sundog_add_distant_light(sundog_scene*, const double direction[3],
+                                        const double radiance[3]);
+
+/* validation / probe / render: reference src/sundog_api.h:154-179 */
+SUNDOG_API int sundog_scene_validate(sundog_scene*);
+
+typedef struct {
+  char name[256];
+  int32_t cc_major, cc_minor;           /* gfx950 reports 9, 5 */
+  int32_t driver_version, runtime_version;
+  uint64_t total_mem_mb;
+  uint32_t optix_header_version;        /* always 0: no OptiX */
+  uint32_t rtcore_version;              /* always 0: no RT cores */
+} sundog_device_info;
+SUNDOG_API int sundog_probe(sundog_device_info* out);
+
+typedef struct {
+  const char* out_path;
+  const char* stats_path;
+  const char* aov_albedo_path, *aov_normal_path;
+  const char* scene_name;
+  int32_t spp, width, height;
+  int64_t seed;
+  int32_t denoise, transparent_shadows;
+  double clamp;
+  double physics_time;
+  int32_t quiet;
+} sundog_render_options;
+SUNDOG_API int sundog_render(sundog_scene*, const sundog_render_options*);
+
+/* ======================================================================== */
+/* Part 2 — additive entry points (no counterpart in the reference)         */
+/* ======================================================================== */
+
+/* Scene-level march/ripple quality knobs. The reference hard-codes a 32-step
+ * march over a bounding cylinder with 3-octave fbm for flames
+ * (device/volume.cuh:90-115) and 3-octave fbm for water ripples
+ * (device/noise.cuh:67-79); those stay the defaults. These calls select the
+ * step-length / box-bounds / octave variants. */
+#define SUNDOG_FLAME_BOUNDS_CYLINDER 0
+#define SUNDOG_FLAME_BOUNDS_BOX 1
+SUNDOG_API int sundog_set_flame_march(sundog_scene*, int flame_id, int octaves,
+                                      double step /* <=0: 32 fixed steps */,
+                                      int bounds /* SUNDOG_FLAME_BOUNDS_* */);
+SUNDOG_API int sundog_set_water_octaves(sundog_scene*, int material_id, int octaves);
+
+/* ---- resident render sessions ----------------------------------------------
+ * A session owns every device resource of one render (BVHs, instance table,
+ * materials, lights, env map + CDFs, film) and keeps them in HBM until it is
+ * destroyed. sundog_render is create + progressive passes + download + write.
+ * Render settings come from the scene after applying `opt` overrides exactly
+ * as sundog_render does (opt may be NULL). The scene handle is not retained. */
+typedef struct sundog_session sundog_session;
+
+#define SUNDOG_KERNEL_MEGA 0      /* one work-item per pixel, regenerated paths */
+#define SUNDOG_KERNEL_WAVEFRONT 1 /* staged queues, material-sorted shading */
+
+typedef struct {
+  uint64_t rays_traced;     /* radiance + traced shadow queries */
+  uint64_t paths;           /* pixels x samples rendered since reset */
+  double render_ms;         /* device time of render passes since reset */
+  double build_ms;          /* BVH build + uploads at create */
+  uint64_t device_bytes;    /* bytes of resident device allocations */
+  int32_t stack_depth;      /* traversal stack entries needed (checked) */
+  int32_t tlas_nodes, blas_nodes;
+  int32_t pad;
+} sundog_session_stats;
+
+SUNDOG_API sundog_session* sundog_session_create(sundog_scene*,
+                                                 const sundog_render_options* opt);
+SUNDOG_API void sundog_session_destroy(sundog_session*);
+/* Clears the film and counters. */
+SUNDOG_API int sundog_session_reset(sundog_session*);
+/* Renders samples [sample_offset, sample_offset+count) of every pixel into
+ * the film (one progressive pass). Asynchronous on the session stream. */
+SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int count);
+SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
+SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
+/* Waits for outstanding work. */
+SUNDOG_API int sundog_session_sync(sundog_session*);
+/* Copies width*height float4 (RGBA) into each non-NULL host buffer. */
+SUNDOG_API int sundog_session_download(sundog_session*, float* beauty,
+                                       float* albedo, float* normal);
+SUNDOG_API int sundog_session_get_stats(sundog_session*, sundog_session_stats*);
+SUNDOG_API int sundog_session_size(sundog_session*, int* width, int* height, int* spp);
+/* The HIP stream (hipStream_t) the session launches on, for event timing. */
+SUNDOG_API void* sundog_session_stream(sundog_session*);
+
+/* Closest-hit query of n world rays (rays: n x 8 floats ox,oy,oz,tmin,
+ * dx,dy,dz,tmax) through the session's two-level BVH. Outputs per ray: t
+ * (INFINITY on miss), instance id (-1 on miss), primitive id (triangle index
+ * in its mesh; 0 or 1 = root index for quadrics). brute != 0 tests every
+ * primitive of every instance instead of traversing. Radiance-ray masking
+ * applies (pass-through faces are skipped). Synchronous. */
+SUNDOG_API int sundog_session_trace(sundog_session*, const float* rays, int n,
+                                    int brute, float* t_out, int32_t* inst_out,
+                                    int32_t* prim_out);
+/* Shadow transmittance of n segments (same ray layout; tmax = segment end)
+ * including the transparent-shadow accumulation; out: n x 3 floats. */
+SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,
+                                     float* tr_out);
+/* Environment importance sampling on device: for n pairs (u0,u1) returns the
+ * marginal row and conditional column picked by the binary searches. */
+SUNDOG_API int sundog_session_env_bins(sundog_session*, const float* u, int n,
+                                       int32_t* row_out, int32_t* col_out);
+
+/* PCG32 on device: n outputs of the stream init(seed, seq), raw u32 draws. */
+SUNDOG_API int sundog_device_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out);
+
+/* ---- flat scene view -------------------------------------------------------
+ * A read-only view of the derived scene (after render overrides), with
+ * meshes loaded and the environment image decoded. Pointers stay valid until
+ * sundog_flat_free. Layouts are plain arrays of floats/ints documented in
+ * sundog_flat.h. */
+struct sundog_flat_scene;
+SUNDOG_API struct sundog_flat_scene* sundog_scene_flatten(sundog_scene*,
+                                                          const sundog_render_options* opt);
+SUNDOG_API void sundog_flat_free(struct sundog_flat_scene*);
+
+#endif /* SUNDOG_API_H */

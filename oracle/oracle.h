@@ -1,0 +1,70 @@
+/* oracle.h — CPU checker for the sundog render path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (sundog_amd/, libsundog.so)
+ * includes, links or calls this code; only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, as the checker.
+ *
+ * The oracle is a scalar C++17 + OpenMP restatement of the reference render
+ * path (device/programs.cu:311-572 and the device helper headers), consuming
+ * the flat scene view (include/sundog_flat.h). It has its own BVH (binary,
+ * median split) and shares no code with the GPU implementation.
+ */
+#ifndef SUNDOG_ORACLE_H
+#define SUNDOG_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/sundog_flat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t sample_offset, sample_count; /* samples [offset, offset+count) */
+  int32_t tile_stride;  /* render every tile_stride-th 16x16 tile (1 = all) */
+  int32_t threads;      /* 0 = OpenMP default */
+  int32_t brute;        /* 1 = brute-force intersection instead of the BVH */
+} oracle_params;
+
+typedef struct {
+  uint64_t radiance_rays, shadow_rays, paths;
+  double seconds;
+  int32_t threads;
+} oracle_stats;
+
+/* Renders into full-frame float4 films (width*height*4 floats each); only the
+ * selected tiles are written. Films accumulate as running means, so passes
+ * can be chained like the GPU's progressive passes. */
+int oracle_render(const sundog_flat_scene* s, const oracle_params* p, float* accum,
+                  float* albedo, float* normal, oracle_stats* st);
+
+/* Closest hit of n rays (8 floats each: ox,oy,oz,tmin,dx,dy,dz,tmax). */
+int oracle_trace(const sundog_flat_scene* s, const float* rays, int n, int brute,
+                 float* t_out, int32_t* inst_out, int32_t* prim_out);
+/* Shadow transmittance (n x 3 floats). */
+int oracle_shadow(const sundog_flat_scene* s, const float* rays, int n, float* tr_out);
+/* Environment CDF bins for n (u0, u1) pairs. */
+int oracle_env_bins(const sundog_flat_scene* s, const float* u, int n, int32_t* row, int32_t* col);
+/* Environment CDFs as built by the oracle (sizes h+1 and h*(w+1)). */
+int oracle_env_cdf(const sundog_flat_scene* s, float* marg, float* cond);
+/* Raw PCG32 draws of stream init(seed, seq). */
+void oracle_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out);
+/* BSDF probe: sample / eval / pdf for a unit-frame configuration. */
+int oracle_bsdf(const sundog_flat_material* m, const float albedo[3], const float wo[3],
+                const float wi[3], float eta, float f_out[3], float* pdf_out);
+int oracle_bsdf_sample(const sundog_flat_material* m, const float albedo[3], const float raydir[3],
+                       int frontface, float eta_ext, uint64_t seed, uint64_t seq,
+                       float wi_out[3], float weight_out[3], float* pdf_out, int* delta_out,
+                       int* valid_out, int* draws_out);
+/* Flame transmittance probe on the flat scene's flames. */
+float oracle_flame_transmittance(const sundog_flat_scene* s, const float o[3], const float d[3],
+                                 float tend, uint64_t seed, uint64_t seq, int skip, int* draws_out);
+/* Last error message. */
+const char* oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

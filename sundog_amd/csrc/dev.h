@@ -1,0 +1,141 @@
+// Device-resident scene tables and launch parameters for the gfx950 kernels.
+//
+// The information content follows the reference's descriptors
+// (device/params.h:20-145, HitRecordData :148-159); the layout is this
+// library's own: fixed 16-byte-aligned records, all scene arrays in HBM
+// pools, and an instance table that carries the world->object transform
+// (the reference leaves that inverse to OptiX).
+#pragma once
+
+#include <stdint.h>
+
+#include "bvh.h"
+#include "kinds.h"
+#include "sd_math.h"
+
+namespace sdk {
+
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kRefMarker = 0xFFFFFFFEu;  // "return to the top level"
+constexpr int kMedMax = 4;
+
+struct alignas(16) DevTexture {
+  int32_t kind;
+  float a[3];
+  float b[3];
+  float scale[2];
+  float width;
+  int32_t imgOffset;  // texel index into the float4 texel pool
+  int32_t imgW, imgH;
+  int32_t pad[2];
+};
+
+struct alignas(16) DevMaterial {
+  int32_t kind, texId;
+  float color[3];
+  float roughness, ior, intensity;
+  int32_t twoSided;
+  float absorb[3];
+  float waveAmp, waveFreq;
+  int32_t waveOctaves;
+  int32_t pad;
+};
+
+struct alignas(16) DevLight {
+  int32_t kind;
+  float p[3], u[3], v[3], n[3], L[3];
+  float radius, area;
+  float dir[3];
+  int32_t twoSided, texId, flameId;
+  // mesh lights
+  int32_t mPosOffset;   // world-space vertex pool offset (vec3)
+  int32_t mIdxOffset;   // index pool offset (uint32 triplets, mesh-local vertex ids)
+  int32_t mUvOffset;    // uv pool offset or -1
+  int32_t mCdfOffset;   // cdf pool offset (numTris + 1 floats)
+  int32_t mNumTris;
+  float mNgSign;
+  int32_t pad[2];
+};
+
+struct alignas(16) DevFlame {
+  float base[3];
+  float height, radius, intensity, sigma, noiseScale;
+  uint32_t seed;
+  int32_t octaves;
+  float step;
+  int32_t bounds;
+};
+
+enum : int32_t { INST_MASKED = 1, INST_TRANSMISSIVE = 2 };
+
+struct alignas(16) DevInstance {
+  float inv[12];        // world -> object, row-major 3x4
+  int32_t geomKind;
+  uint32_t blasRoot;    // packed node reference of the mesh's BLAS root
+  int32_t meshId;
+  uint32_t mats;        // front | back << 16 (0xFFFF = pass-through)
+  int32_t lightId;
+  int32_t cutoutTex;
+  int32_t flags;        // INST_*
+  int32_t pad;
+};
+static_assert(sizeof(DevInstance) == 80, "instance layout");
+
+struct alignas(16) DevMesh {
+  int32_t posOffset, idxOffset, nrmOffset, uvOffset;  // pools; -1 = absent
+  int32_t numTris, numVerts;
+  int32_t pad[2];
+};
+
+struct CameraDev {
+  float origin[3], lowerLeft[3], horizontal[3], vertical[3], u[3], v[3], w[3];
+  float lensRadius;
+};
+
+struct LaunchParams {
+  int32_t width, height, sppTotal, sampleOffset, sppThisLaunch, maxDepth;
+  float clampVal;
+  uint32_t seed;
+  int32_t countRays, transparentShadows;
+  int32_t numLights, numFlames;
+
+  float4* accum;
+  float4* albedo;
+  float4* normal;
+  unsigned long long* rayCounter;
+
+  CameraDev cam;
+  int32_t bgKind;
+  float bgA[3], bgB[3];
+
+  const float4* envTex;
+  const float* margCdf;
+  const float* condCdf;
+  int32_t envW, envH;
+  float envRotate, envIntensity;
+  int32_t envImportance;
+  int32_t hasEnv;
+
+  const DevTexture* textures;
+  const float4* texels;
+  const DevMaterial* materials;
+  const DevLight* lights;
+  const DevFlame* flames;
+  const DevInstance* instances;
+  const BvhNode4* nodes;
+  const float4* tris;        // 3 float4 per leaf-ordered triangle; v0.w = triangle id bits
+  const DevMesh* meshes;
+  const float* positions;    // vec3 pool
+  const uint32_t* indices;   // uint32 pool
+  const float* normals;      // vec3 pool
+  const float* uvs;          // vec2 pool
+  const float* lightPos;     // world-space mesh-light vertices (vec3 pool)
+  const float* lightCdf;
+  uint32_t* spill;           // traversal-stack overflow, [entry][thread]
+  int32_t spillStride;       // threads covered by one spill row
+  int32_t spillRows;
+  uint32_t tlasRoot;         // packed reference of the top-level root
+};
+
+}  // namespace sdk

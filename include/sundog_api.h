@@ -185,6 +185,12 @@ SUNDOG_API int sundog_set_flame_march(sundog_scene*, int flame_id, int octaves,
                                       int bounds /* SUNDOG_FLAME_BOUNDS_* */);
 SUNDOG_API int sundog_set_water_octaves(sundog_scene*, int material_id, int octaves);
 
+/* JSON description of the derived scene state (render settings, camera and
+ * its derived frame, background, registries, objects with composed
+ * transforms and light back-links, the light list, flames). GPU-free.
+ * Writes at most cap bytes (NUL-terminated) and reports the size needed. */
+SUNDOG_API int sundog_scene_describe(sundog_scene*, char* buf, size_t cap, size_t* needed);
+
 /* ---- resident render sessions ----------------------------------------------
  * A session owns every device resource of one render (BVHs, instance table,
  * materials, lights, env map + CDFs, film) and keeps them in HBM until it is
@@ -205,6 +211,10 @@ typedef struct {
   int32_t stack_depth;      /* traversal stack entries needed (checked) */
   int32_t tlas_nodes, blas_nodes;
   int32_t pad;
+  /* traversal work since reset, counted only while count_work is on:
+   * 4-wide node visits (4 box tests each), triangle tests, quadric tests,
+   * world->object instance transforms */
+  uint64_t node_visits, tri_tests, quad_tests, inst_transforms;
 } sundog_session_stats;
 
 SUNDOG_API sundog_session* sundog_session_create(sundog_scene*,
@@ -217,6 +227,8 @@ SUNDOG_API int sundog_session_reset(sundog_session*);
 SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int count);
 SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
+/* Runs the instrumented kernel variant that counts traversal work. */
+SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);
 /* Waits for outstanding work. */
 SUNDOG_API int sundog_session_sync(sundog_session*);
 /* Copies width*height float4 (RGBA) into each non-NULL host buffer. */

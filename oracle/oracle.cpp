@@ -186,7 +186,11 @@ int wrap(int i, int n) {
   return r < 0 ? r + n : r;
 }
 void bilinear(const float* img, int w, int h, float u, float v, bool wrapV, float out[4]) {
+  u = u - floorf(u);
+  if (wrapV) v = v - floorf(v);
   float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+  x = fminf(fmaxf(x, -1.0f), (float)w);
+  y = fminf(fmaxf(y, -1.0f), (float)h);
   float fx = floorf(x), fy = floorf(y);
   float ax = x - fx, ay = y - fy;
   int x0 = wrap((int)fx, w), x1 = wrap((int)fx + 1, w);
@@ -476,11 +480,9 @@ struct LightSample {
   bool valid = false;
 };
 
-LightSample sampleEnv(const Scene& S, Rng& rng) {
+LightSample sampleEnvU(const Scene& S, float r0, float r1) {
   LightSample s;
   s.dist = 1e16f;
-  float r0 = rng.rnd();
-  float r1 = rng.rnd();
   if (!S.envImportance) {
     s.wi = uniformOnSphere(r0, r1);
     s.pdf = 0.25f * INV_PI;
@@ -508,15 +510,21 @@ LightSample sampleEnv(const Scene& S, Rng& rng) {
   s.valid = s.pdf > 0.0f;
   return s;
 }
+// Same draw count in both modes: two rnd() (device/env_light.cuh:60).
+LightSample sampleEnv(const Scene& S, Rng& rng) {
+  float r0 = rng.rnd();
+  float r1 = rng.rnd();
+  return sampleEnvU(S, r0, r1);
+}
 float envPdfSolidAngle(const Scene& S, V dir) {
   if (!S.envImportance) return 0.25f * INV_PI;
   float sinT = sqrtf(fmaxf(0.0f, 1.0f - dir.y * dir.y));
-  if (sinT < 1e-6f) return 0.0f;
+  if (!(sinT >= 1e-6f)) return 0.0f;  // also rejects NaN directions
   float u, v;
   envDirToUv(S, dir, u, v);
   int W = S.f->env_w, H = S.f->env_h;
-  int col = std::min((int)(u * W), W - 1);
-  int row = std::min((int)(v * H), H - 1);
+  int col = (int)fminf(fmaxf(u * W, 0.0f), (float)(W - 1));
+  int row = (int)fminf(fmaxf(v * H, 0.0f), (float)(H - 1));
   float dm = S.marg[row + 1] - S.marg[row];
   const float* cdf = S.cond.data() + (size_t)row * (W + 1);
   float dc = cdf[col + 1] - cdf[col];
@@ -1874,4 +1882,42 @@ extern "C" float oracle_flame_transmittance(const sundog_flat_scene* f, const fl
   float tr = flameTransmittance(S, ld(o), ld(d), tend, rng, skip);
   if (draws_out) *draws_out = (int)rng.draws;
   return tr;
+}
+
+// ---------------------------------------------------------------------------
+// probes used by the checker's own tests
+// ---------------------------------------------------------------------------
+
+extern "C" int oracle_env_sample(const sundog_flat_scene* f, float u0, float u1, float wi_out[3],
+                                 float* pdf_out, float* pdf_back_out) {
+  return guard([&] {
+    if (f->bg_kind != 2 || !f->env_rgba) throw std::runtime_error("scene has no environment map");
+    Scene S;
+    S.f = f;
+    S.envImportance = buildEnvCdf(f, S.marg, S.cond) && f->env_importance;
+    LightSample s = sampleEnvU(S, u0, u1);
+    wi_out[0] = s.wi.x;
+    wi_out[1] = s.wi.y;
+    wi_out[2] = s.wi.z;
+    *pdf_out = s.valid ? s.pdf : 0.0f;
+    *pdf_back_out = s.valid ? envPdfSolidAngle(S, s.wi) : 0.0f;
+  });
+}
+
+extern "C" float oracle_march_flames(const sundog_flat_scene* f, const float o[3], const float d[3], float tend,
+                                     uint64_t seed, uint64_t seq, float trans_out[3], float L_out[3],
+                                     int* draws_out) {
+  Scene S;
+  S.f = f;
+  Rng rng = Rng::init(seed, seq);
+  V trans = spl(1.0f);
+  V L = marchFlames(S, ld(o), ld(d), tend, rng, trans);
+  trans_out[0] = trans.x;
+  trans_out[1] = trans.y;
+  trans_out[2] = trans.z;
+  L_out[0] = L.x;
+  L_out[1] = L.y;
+  L_out[2] = L.z;
+  if (draws_out) *draws_out = (int)rng.draws;
+  return trans.x;
 }

@@ -60,6 +60,14 @@ int oracle_bsdf_sample(const sundog_flat_material* m, const float albedo[3], con
 /* Flame transmittance probe on the flat scene's flames. */
 float oracle_flame_transmittance(const sundog_flat_scene* s, const float o[3], const float d[3],
                                  float tend, uint64_t seed, uint64_t seq, int skip, int* draws_out);
+/* Environment sample for fixed uniforms (u0, u1): direction, its pdf, and
+ * the pdf the MIS side recomputes for that direction. */
+int oracle_env_sample(const sundog_flat_scene* s, float u0, float u1, float wi_out[3],
+                      float* pdf_out, float* pdf_back_out);
+/* Emission + transmittance march over the flat scene's flames. */
+float oracle_march_flames(const sundog_flat_scene* s, const float o[3], const float d[3], float tend,
+                          uint64_t seed, uint64_t seq, float trans_out[3], float L_out[3],
+                          int* draws_out);
 /* Last error message. */
 const char* oracle_last_error(void);
 

@@ -9,6 +9,7 @@
 //                                   src/scene_build.cpp:13-181
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -648,6 +649,217 @@ extern "C" int sundog_set_water_octaves(sundog_scene* h, int material_id, int oc
       sceneFail("water octaves: material is not water");
     if (octaves < 1 || octaves > 8) sceneFail("water octaves: octaves must be in [1, 8]");
     h->scene.materials[material_id].waveOctaves = octaves;
+    return SUNDOG_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// sundog_scene_describe: JSON dump of the derived scene state (GPU-free)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Json {
+  std::string s;
+  void key(const char* k) { s += "\""; s += k; s += "\":"; }
+  void num(double v) {
+    char b[64];
+    if (std::isnan(v)) snprintf(b, sizeof b, "null");
+    else snprintf(b, sizeof b, "%.9g", v);
+    s += b;
+  }
+  void v3(const vec3& v) {
+    s += "[";
+    num(v.x); s += ","; num(v.y); s += ","; num(v.z);
+    s += "]";
+  }
+  void str(const std::string& v) {
+    s += "\"";
+    for (char c : v) {
+      if (c == '"' || c == '\\') { s += '\\'; s += c; }
+      else if ((unsigned char)c < 0x20) s += ' ';
+      else s += c;
+    }
+    s += "\"";
+  }
+};
+
+}  // namespace
+
+extern "C" int sundog_scene_describe(sundog_scene* h, char* buf, size_t cap, size_t* needed) {
+  return guardInt(SUNDOG_ERROR, [&] {
+    if (!h) sceneFail("null scene handle");
+    const Scene& S = h->scene;
+    Json j;
+    j.s += "{";
+    j.key("phase"); j.num(h->phase); j.s += ",";
+    j.key("has_camera"); j.s += h->hasCamera ? "true" : "false"; j.s += ",";
+    j.key("base_dir"); j.str(S.baseDir); j.s += ",";
+    const RenderSettings& r = S.render;
+    j.key("render"); j.s += "{";
+    j.key("width"); j.num(r.width); j.s += ",";
+    j.key("height"); j.num(r.height); j.s += ",";
+    j.key("spp"); j.num(r.spp); j.s += ",";
+    j.key("max_depth"); j.num(r.maxDepth); j.s += ",";
+    j.key("clamp"); j.num(r.clampVal); j.s += ",";
+    j.key("seed"); j.num(r.seed); j.s += ",";
+    j.key("exposure"); j.num(r.exposure); j.s += ",";
+    j.key("transparent_shadows"); j.s += r.transparentShadows ? "true" : "false"; j.s += ",";
+    j.key("chunk"); j.num(r.chunk); j.s += ",";
+    j.key("denoise"); j.s += r.denoise ? "true" : "false";
+    j.s += "},";
+    const CameraSettings& c = S.camera;
+    j.key("camera"); j.s += "{";
+    j.key("lookfrom"); j.v3(c.lookfrom); j.s += ",";
+    j.key("lookat"); j.v3(c.lookat); j.s += ",";
+    j.key("up"); j.v3(c.up); j.s += ",";
+    j.key("vfov"); j.num(c.vfov); j.s += ",";
+    j.key("aperture"); j.num(c.aperture); j.s += ",";
+    j.key("focus_dist"); j.num(c.focusDist);
+    if (h->hasCamera) {
+      Camera d = makeCamera(c, r.width, r.height);
+      j.s += ",";
+      j.key("origin"); j.v3(d.origin); j.s += ",";
+      j.key("lower_left"); j.v3(d.lowerLeft); j.s += ",";
+      j.key("horizontal"); j.v3(d.horizontal); j.s += ",";
+      j.key("vertical"); j.v3(d.vertical); j.s += ",";
+      j.key("u"); j.v3(d.u); j.s += ",";
+      j.key("v"); j.v3(d.v); j.s += ",";
+      j.key("w"); j.v3(d.w); j.s += ",";
+      j.key("lens_radius"); j.num(d.lensRadius);
+    }
+    j.s += "},";
+    j.key("background"); j.s += "{";
+    j.key("kind"); j.num(S.bgKind); j.s += ",";
+    j.key("a"); j.v3(S.bgA); j.s += ",";
+    j.key("b"); j.v3(S.bgB); j.s += ",";
+    j.key("env_file"); j.str(S.env.file); j.s += ",";
+    j.key("env_rotate_deg"); j.num(S.env.rotateDeg); j.s += ",";
+    j.key("env_intensity"); j.num(S.env.intensity); j.s += ",";
+    j.key("env_importance"); j.s += S.env.importance ? "true" : "false";
+    j.s += "},";
+    j.key("physics"); j.s += "{";
+    j.key("enabled"); j.s += S.physics.enabled ? "true" : "false"; j.s += ",";
+    j.key("gravity"); j.v3(S.physics.gravity); j.s += ",";
+    j.key("timestep"); j.num(S.physics.timestep); j.s += ",";
+    j.key("max_time"); j.num(S.physics.maxTime); j.s += ",";
+    j.key("stop_time"); j.num(S.physics.stopTime); j.s += ",";
+    j.key("friction"); j.num(S.physics.friction); j.s += ",";
+    j.key("restitution"); j.num(S.physics.restitution); j.s += ",";
+    j.key("pos_iters"); j.num(S.physics.posIters); j.s += ",";
+    j.key("vel_iters"); j.num(S.physics.velIters);
+    j.s += "},";
+    j.key("textures"); j.s += "[";
+    for (size_t i = 0; i < S.textures.size(); i++) {
+      const Texture& t = S.textures[i];
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("kind"); j.num(t.kind); j.s += ",";
+      j.key("a"); j.v3(t.a); j.s += ",";
+      j.key("b"); j.v3(t.b); j.s += ",";
+      j.key("scale"); j.s += "["; j.num(t.scale.x); j.s += ","; j.num(t.scale.y); j.s += "],";
+      j.key("width"); j.num(t.width); j.s += ",";
+      j.key("file"); j.str(t.file); j.s += ",";
+      j.key("srgb"); j.s += t.srgb ? "true" : "false";
+      j.s += "}";
+    }
+    j.s += "],";
+    j.key("materials"); j.s += "[";
+    for (size_t i = 0; i < S.materials.size(); i++) {
+      const Material& m = S.materials[i];
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("kind"); j.num(m.kind); j.s += ",";
+      j.key("tex_id"); j.num(m.texId); j.s += ",";
+      j.key("color"); j.v3(m.color); j.s += ",";
+      j.key("roughness"); j.num(m.roughness); j.s += ",";
+      j.key("ior"); j.num(m.ior); j.s += ",";
+      j.key("intensity"); j.num(m.intensity); j.s += ",";
+      j.key("two_sided"); j.num(m.twoSided); j.s += ",";
+      j.key("absorb"); j.v3(m.absorb); j.s += ",";
+      j.key("wave_amp"); j.num(m.waveAmp); j.s += ",";
+      j.key("wave_freq"); j.num(m.waveFreq); j.s += ",";
+      j.key("wave_octaves"); j.num(m.waveOctaves);
+      j.s += "}";
+    }
+    j.s += "],";
+    j.key("meshes"); j.s += "[";
+    for (size_t i = 0; i < S.meshes.size(); i++) {
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("obj"); j.str(S.meshes[i].objFile); j.s += ",";
+      j.key("usemtl"); j.str(S.meshes[i].usemtl); j.s += ",";
+      j.key("smooth"); j.s += S.meshes[i].smoothNormals ? "true" : "false";
+      j.s += "}";
+    }
+    j.s += "],";
+    j.key("objects"); j.s += "[";
+    for (size_t i = 0; i < S.objects.size(); i++) {
+      const Object& o = S.objects[i];
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("geom_kind"); j.num(o.geomKind); j.s += ",";
+      j.key("mesh_id"); j.num(o.meshId); j.s += ",";
+      j.key("mat_front"); j.num(o.matFront); j.s += ",";
+      j.key("mat_back"); j.num(o.matBack); j.s += ",";
+      j.key("cutout_tex"); j.num(o.cutoutTexId); j.s += ",";
+      j.key("light_id"); j.num(o.lightId); j.s += ",";
+      j.key("nee"); j.s += o.nee ? "true" : "false"; j.s += ",";
+      j.key("physics"); j.s += o.physics.enabled ? (o.physics.dynamic ? "\"dynamic\"" : "\"static\"") : "null";
+      j.s += ",";
+      j.key("xform"); j.s += "[";
+      for (int k = 0; k < 12; k++) {
+        if (k) j.s += ",";
+        j.num(o.xform.m[k]);
+      }
+      j.s += "]}";
+    }
+    j.s += "],";
+    j.key("lights"); j.s += "[";
+    for (size_t i = 0; i < S.lights.size(); i++) {
+      const Light& l = S.lights[i];
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("kind"); j.num(l.kind); j.s += ",";
+      j.key("p"); j.v3(l.p); j.s += ",";
+      j.key("u"); j.v3(l.u); j.s += ",";
+      j.key("v"); j.v3(l.v); j.s += ",";
+      j.key("n"); j.v3(l.n); j.s += ",";
+      j.key("L"); j.v3(l.L); j.s += ",";
+      j.key("radius"); j.num(l.radius); j.s += ",";
+      j.key("area"); j.num(l.area); j.s += ",";
+      j.key("dir"); j.v3(l.dir); j.s += ",";
+      j.key("two_sided"); j.num(l.twoSided); j.s += ",";
+      j.key("tex_id"); j.num(l.texId); j.s += ",";
+      j.key("flame_id"); j.num(l.flameId); j.s += ",";
+      j.key("object_id"); j.num(l.objectId);
+      j.s += "}";
+    }
+    j.s += "],";
+    j.key("flames"); j.s += "[";
+    for (size_t i = 0; i < S.flames.size(); i++) {
+      const Flame& f = S.flames[i];
+      if (i) j.s += ",";
+      j.s += "{";
+      j.key("base"); j.v3(f.base); j.s += ",";
+      j.key("height"); j.num(f.height); j.s += ",";
+      j.key("radius"); j.num(f.radius); j.s += ",";
+      j.key("intensity"); j.num(f.intensity); j.s += ",";
+      j.key("sigma"); j.num(f.sigma); j.s += ",";
+      j.key("noise_scale"); j.num(f.noiseScale); j.s += ",";
+      j.key("seed"); j.num(f.seed); j.s += ",";
+      j.key("octaves"); j.num(f.octaves); j.s += ",";
+      j.key("step"); j.num(f.step); j.s += ",";
+      j.key("bounds"); j.num(f.bounds);
+      j.s += "}";
+    }
+    j.s += "]}";
+    if (needed) *needed = j.s.size() + 1;
+    if (buf && cap > 0) {
+      size_t n = std::min(cap - 1, j.s.size());
+      memcpy(buf, j.s.data(), n);
+      buf[n] = '\0';
+    }
     return SUNDOG_OK;
   });
 }

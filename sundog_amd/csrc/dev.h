@@ -104,6 +104,7 @@ struct LaunchParams {
   float4* albedo;
   float4* normal;
   unsigned long long* rayCounter;
+  unsigned long long* workCounters;  // node visits, triangle tests, quadric tests, instance transforms
 
   CameraDev cam;
   int32_t bgKind;

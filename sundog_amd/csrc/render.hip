@@ -97,7 +97,12 @@ DEV int wrapi(int i, int n) {
 }
 
 DEV float4 bilinear(const float4* texels, int w, int h, float u, float v, bool wrapV) {
+  u = u - floorf(u);  // u always wraps; v wraps for textures, clamps for the sky
+  if (wrapV) v = v - floorf(v);
   float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+  // keep float->int conversions in range (NaN maps to -1 via fmaxf)
+  x = fminf(fmaxf(x, -1.0f), (float)w);
+  y = fminf(fmaxf(y, -1.0f), (float)h);
   float fx = floorf(x), fy = floorf(y);
   float ax = x - fx, ay = y - fy;
   int x0 = wrapi((int)fx, w), x1 = wrapi((int)fx + 1, w);
@@ -411,11 +416,12 @@ DEV LightSample sampleEnv(const LaunchParams& P, Pcg& rng) {
 DEV float envPdf(const LaunchParams& P, vec3 dir) {
   if (!P.envImportance) return 0.25f * kInvPi;
   float sinT = sqrtf(fmaxf(0.0f, 1.0f - dir.y * dir.y));
-  if (sinT < 1e-6f) return 0.0f;
+  if (!(sinT >= 1e-6f)) return 0.0f;  // also rejects NaN directions
   vec2 uv = envDirToUv(P, dir);
   int W = P.envW, H = P.envH;
-  int col = min((int)(uv.x * W), W - 1);
-  int row = min((int)(uv.y * H), H - 1);
+  // fmaxf drops NaN, so the texel index stays inside the image
+  int col = (int)fminf(fmaxf(uv.x * W, 0.0f), (float)(W - 1));
+  int row = (int)fminf(fmaxf(uv.y * H, 0.0f), (float)(H - 1));
   float dm = P.margCdf[row + 1] - P.margCdf[row];
   const float* cdf = P.condCdf + (size_t)row * (W + 1);
   float dc = cdf[col + 1] - cdf[col];
@@ -1130,6 +1136,11 @@ struct Stack {
   }
 };
 
+// Per-query traversal work, for the algorithmic-FLOP count of the roofline.
+struct WorkCount {
+  uint32_t nodes = 0, tris = 0, quads = 0, insts = 0;
+};
+
 struct Hit {
   float t;
   int inst;
@@ -1225,7 +1236,7 @@ DEV RayState makeRay(vec3 o, vec3 d) {
 // Closest hit with masking. Ties in t resolve to the smaller (instance,
 // primitive) so the answer does not depend on traversal order.
 DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
-                      vec3& oObjOut, vec3& dObjOut, int& nodeVisits, int& primTests) {
+                      vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
   hit.t = tmax;
   hit.inst = 0x7fffffff;
   hit.prim = 0x7fffffff;
@@ -1241,7 +1252,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
   while (true) {
     if (!(cur & kRefLeaf)) {
       const RayState& r = inBlas ? obj : world;
-      nodeVisits++;
+      wc.nodes++;
       float dist[4];
       boxes4(P.nodes[cur], r.o, r.inv, tmin, hit.t, dist);
       const int32_t* refs = &P.nodes[cur].child[0];
@@ -1267,6 +1278,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
       int ii = (int)(cur & 0x07FFFFFFu);
       const DevInstance& in = P.instances[ii];
       vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
+      wc.insts++;
       if (in.geomKind == GK_MESH) {
         st.push(kRefMarker);
         obj = makeRay(oo, dd);
@@ -1276,7 +1288,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
         cur = in.blasRoot;
         continue;
       }
-      primTests++;
+      wc.quads++;
       float ts[2];
       int n = quadricRoots(in.geomKind, oo, dd, tmin, hit.t * 1.0000004f + 1e-30f, ts);
       for (int k = 0; k < n; k++) {
@@ -1295,7 +1307,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
       int cnt = (int)((cur >> 27) & 0xFu) + 1;
       const DevInstance& in = P.instances[curInst];
       for (int k = 0; k < cnt; k++) {
-        primTests++;
+        wc.tris++;
         const float4* tv = P.tris + 3 * (size_t)(first + k);
         float4 a = tv[0], b = tv[1], c = tv[2];
         float t, b1, b2;
@@ -1328,8 +1340,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
 // :247-292): opaque => 0; with transparent shadows, each glass / water
 // crossing multiplies (1 - F) and adds its signed sigma * t; every primitive
 // is visited once, so each crossing counts once.
-DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmax, int& nodeVisits,
-                     int& primTests) {
+DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmax, WorkCount& wc) {
   const float tmin = 0.0f;
   float trF = 1.0f;
   vec3 tau = v3(0.0f);
@@ -1344,7 +1355,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
   while (!occluded) {
     if (!(cur & kRefLeaf)) {
       const RayState& r = inBlas ? obj : world;
-      nodeVisits++;
+      wc.nodes++;
       float dist[4];
       boxes4(P.nodes[cur], r.o, r.inv, tmin, tmax, dist);
       int4 rc = *reinterpret_cast<const int4*>(&P.nodes[cur].child[0]);
@@ -1369,6 +1380,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
       if (!inBlas) {
         const DevInstance& in = P.instances[first];
         vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
+        wc.insts++;
         if (in.geomKind == GK_MESH) {
           st.push(kRefMarker);
           obj = makeRay(oo, dd);
@@ -1388,13 +1400,13 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
         int prim;
         bool h;
         if (inBlas) {
-          primTests++;
+          wc.tris++;
           const float4* tv = P.tris + 3 * (size_t)(first + k);
           float4 a = tv[0], b = tv[1], c = tv[2];
           h = triHit(rt, obj.o, f4xyz(a), f4xyz(b), f4xyz(c), tmin, tmax, t, b1, b2);
           prim = __float_as_int(a.w);
         } else {
-          if (k == 0) primTests++;
+          if (k == 0) wc.quads++;
           float ts[2];
           int n = quadricRoots(in.geomKind, obj.o, obj.d, tmin, tmax, ts);
           h = k < n;
@@ -1480,12 +1492,14 @@ DEV void pixelOf(int& px, int& py) {
   py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
+template <bool kCountWork>
 __global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
   __shared__ uint32_t s_stack[kStackLds * kBlock];
   int px, py;
   pixelOf(px, py);
   const bool active = px < P.width && py < P.height;
   uint32_t rays = 0;
+  uint32_t work[4] = {0, 0, 0, 0};
   if (active) {
     const int tid = threadIdx.x;
     const int gtid = (blockIdx.y * gridDim.x + blockIdx.x) * kBlock + tid;
@@ -1494,7 +1508,7 @@ __global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
     st.spill = P.spill + gtid;
     st.stride = P.spillStride;
     st.sp = 0;
-    int dummyN = 0, dummyT = 0;
+    WorkCount wc;
 
     const int pixel = py * P.width + px;
     int nStrata = (int)floorf(sqrtf((float)P.sppTotal) + 0.5f);
@@ -1541,7 +1555,7 @@ __global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
       for (int depth = 0; depth < P.maxDepth; depth++) {
         Hit hit;
         vec3 oObj = v3(0.0f), dObj = v3(0.0f);
-        bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oObj, dObj, dummyN, dummyT);
+        bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oObj, dObj, wc);
         rays++;
 
         if (medDepth > 0) {
@@ -1628,7 +1642,7 @@ __global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
             rays++;
             vec3 so = offsetRay(x, transmitNee ? -ns : ns);
             float tmaxS = ls.dist * 0.999f;
-            vec3 tr = traceShadow(P, st, so, ls.wi, tmaxS, dummyN, dummyT);
+            vec3 tr = traceShadow(P, st, so, ls.wi, tmaxS, wc);
             if (maxc(tr) > 0.0f) {
               if (P.numFlames > 0 && P.transparentShadows) {
                 int skip = (k < P.numLights) ? P.lights[k].flameId : -1;
@@ -1691,6 +1705,21 @@ __global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
     P.accum[pixel] = accum;
     P.albedo[pixel] = aovA;
     P.normal[pixel] = aovN;
+    if (kCountWork) {
+      work[0] = wc.nodes;
+      work[1] = wc.tris;
+      work[2] = wc.quads;
+      work[3] = wc.insts;
+    }
+  }
+  if (kCountWork) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      unsigned long long v = work[c];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if ((threadIdx.x & 63) == 0 && v) atomicAdd(P.workCounters + c, v);
+    }
   }
   if (P.countRays) {
     // wave-level sum, one 64-bit atomic per wave
@@ -1719,14 +1748,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(LaunchParams P, const float* r
   vec3 o = vec3{r[0], r[1], r[2]}, d = vec3{r[4], r[5], r[6]};
   Hit h;
   vec3 oo, dd;
-  int nv = 0, nt = 0;
-  bool hit = traceClosest(P, st, o, d, r[3], r[7], h, oo, dd, nv, nt);
+  WorkCount wc;
+  bool hit = traceClosest(P, st, o, d, r[3], r[7], h, oo, dd, wc);
   tOut[i] = hit ? h.t : __builtin_inff();
   instOut[i] = hit ? h.inst : -1;
   primOut[i] = hit ? h.prim : -1;
   if (counters) {
-    counters[2 * i] = nv;
-    counters[2 * i + 1] = nt;
+    counters[4 * i] = (int)wc.nodes;
+    counters[4 * i + 1] = (int)wc.tris;
+    counters[4 * i + 2] = (int)wc.quads;
+    counters[4 * i + 3] = (int)wc.insts;
   }
 }
 
@@ -1792,8 +1823,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(LaunchParams P, const float* 
   st.stride = P.spillStride;
   st.sp = 0;
   const float* r = rays + 8 * (size_t)i;
-  int nv = 0, nt = 0;
-  vec3 tr = traceShadow(P, st, vec3{r[0], r[1], r[2]}, vec3{r[4], r[5], r[6]}, r[7], nv, nt);
+  WorkCount wc;
+  vec3 tr = traceShadow(P, st, vec3{r[0], r[1], r[2]}, vec3{r[4], r[5], r[6]}, r[7], wc);
   trOut[3 * i] = tr.x;
   trOut[3 * i + 1] = tr.y;
   trOut[3 * i + 2] = tr.z;
@@ -1818,9 +1849,12 @@ __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
 // host-side launch helpers (called from session.hip)
 // ---------------------------------------------------------------------------
 
-hipError_t launchRender(const LaunchParams& P, hipStream_t s) {
+hipError_t launchRender(const LaunchParams& P, bool countWork, hipStream_t s) {
   dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-  hipLaunchKernelGGL(k_render, grid, dim3(kBlock), 0, s, P);
+  if (countWork)
+    hipLaunchKernelGGL(k_render<true>, grid, dim3(kBlock), 0, s, P);
+  else
+    hipLaunchKernelGGL(k_render<false>, grid, dim3(kBlock), 0, s, P);
   return hipGetLastError();
 }
 

@@ -25,7 +25,7 @@
 
 namespace sdk {
 
-hipError_t launchRender(const LaunchParams& P, hipStream_t s);
+hipError_t launchRender(const LaunchParams& P, bool countWork, hipStream_t s);
 hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool brute, int numInst, float* t,
                        int* inst, int* prim, int* counters, hipStream_t s);
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
@@ -170,13 +170,14 @@ struct sundog_session {
   Prepared prep;
   LaunchParams P{};
   int kernel = SUNDOG_KERNEL_MEGA;
+  bool countWork = false;
   int numInstances = 0;
   int stackNeed = 0;
   int spillRows = 0;
   DevBuf textures, texels, materials, lights, flames, instances, nodes, tris, meshes;
   DevBuf positions, indices, normals, uvs, lightPos, lightCdf;
   DevBuf envTex, margCdf, condCdf;
-  DevBuf accum, albedo, normal, rayCounter, spill;
+  DevBuf accum, albedo, normal, rayCounter, workCounters, spill;
   size_t spillThreads = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double renderMs = 0.0;
@@ -194,7 +195,7 @@ struct sundog_session {
   uint64_t deviceBytes() const {
     const DevBuf* all[] = {&textures, &texels, &materials, &lights, &flames, &instances, &nodes, &tris,
                            &meshes, &positions, &indices, &normals, &uvs, &lightPos, &lightCdf, &envTex,
-                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &spill};
+                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->bytes;
     return b;
@@ -473,6 +474,7 @@ struct sundog_session {
     albedo.alloc(npx * 16);
     normal.alloc(npx * 16);
     rayCounter.alloc(8);
+    workCounters.alloc(32);
 
     // launch parameters
     LaunchParams& p = P;
@@ -490,6 +492,7 @@ struct sundog_session {
     p.albedo = albedo.as<float4>();
     p.normal = normal.as<float4>();
     p.rayCounter = rayCounter.as<unsigned long long>();
+    p.workCounters = workCounters.as<unsigned long long>();
     Camera c = makeCamera(S.camera, S.render.width, S.render.height);
     auto st3 = [](float* d, vec3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
     st3(p.cam.origin, c.origin);
@@ -539,6 +542,7 @@ struct sundog_session {
     HIP_OK(hipMemsetAsync(albedo.p, 0, npx * 16, stream));
     HIP_OK(hipMemsetAsync(normal.p, 0, npx * 16, stream));
     HIP_OK(hipMemsetAsync(rayCounter.p, 0, 8, stream));
+    HIP_OK(hipMemsetAsync(workCounters.p, 0, 32, stream));
     renderMs = 0.0;
     paths = 0;
   }
@@ -551,7 +555,7 @@ struct sundog_session {
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
     HIP_OK(hipEventRecord(ev0, stream));
-    HIP_OK(launchRender(P, stream));
+    HIP_OK(launchRender(P, countWork, stream));
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0.0f;
@@ -662,6 +666,14 @@ extern "C" int sundog_session_set_count_rays(sundog_session* s, int on) {
   });
 }
 
+extern "C" int sundog_session_set_count_work(sundog_session* s, int on) {
+  return guarded([&] {
+    if (!s) throw std::runtime_error("session: null handle");
+    s->countWork = on != 0;
+    return SUNDOG_OK;
+  });
+}
+
 extern "C" int sundog_session_sync(sundog_session* s) {
   return guarded([&] {
     if (!s) throw std::runtime_error("session: null handle");
@@ -686,10 +698,15 @@ extern "C" int sundog_session_get_stats(sundog_session* s, sundog_session_stats*
   return guarded([&] {
     if (!s || !out) throw std::runtime_error("session: null argument");
     HIP_OK(hipStreamSynchronize(s->stream));
-    unsigned long long rays = 0;
+    unsigned long long rays = 0, work[4] = {0, 0, 0, 0};
     HIP_OK(hipMemcpy(&rays, s->rayCounter.p, 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(work, s->workCounters.p, 32, hipMemcpyDeviceToHost));
     *out = sundog_session_stats{};
     out->rays_traced = rays;
+    out->node_visits = work[0];
+    out->tri_tests = work[1];
+    out->quad_tests = work[2];
+    out->inst_transforms = work[3];
     out->paths = s->paths;
     out->render_ms = s->renderMs;
     out->build_ms = s->buildMs;

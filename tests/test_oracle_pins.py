@@ -1,0 +1,415 @@
+"""Pins of the CPU checker against fixed points that do not come from the
+checker itself: published generator vectors, closed forms, invariants and
+brute force. Each test names what would fail if a term were dropped, a sign
+flipped or an index transposed. CPU only.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import sundog_amd as sd
+from sundog_amd import configs
+from sundog_amd.scenelib import rotate_x, scale, translate
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+LAMBERT, METAL, DIELECTRIC, EMISSIVE, WATER, PLASTIC = range(6)
+
+
+# ---------------------------------------------------------------------------
+# PCG32 (device/rng.cuh:10-35)
+# ---------------------------------------------------------------------------
+
+def test_pcg32_published_demo_vectors(orc):
+    gold = np.load(os.path.join(GOLDEN, "pcg32_demo.npz"))["round1"]
+    assert np.array_equal(orc.pcg32(42, 54, 6), gold)
+
+
+def test_pcg32_matches_python_restatement(orc):
+    # an independent restatement of PCG-XSH-RR in Python integers
+    p = configs.Pcg32(0xDEADBEEF, seq=7)
+    ref = np.array([p.next() for _ in range(32)], np.uint32)
+    assert np.array_equal(orc.pcg32(0xDEADBEEF, 7, 32), ref)
+
+
+# ---------------------------------------------------------------------------
+# BSDFs (device/bsdf.cuh)
+# ---------------------------------------------------------------------------
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
+
+
+def _hemi(rng, n):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, 2] = np.abs(d[:, 2])
+    return d.astype(np.float32)
+
+
+def test_lambert_eval_pdf_closed_form_and_white_furnace(orc):
+    m = orc.material(LAMBERT, color=(1, 1, 1))
+    rng = np.random.default_rng(1)
+    for wi in _hemi(rng, 50):
+        f, pdf = orc.bsdf_eval(m, [1, 1, 1], [0, 0, 1], wi)
+        assert f == pytest.approx([1 / math.pi] * 3, rel=1e-6)
+        assert pdf == pytest.approx(wi[2] / math.pi, rel=1e-5)
+    for s in range(50):
+        r = orc.bsdf_sample(m, [1, 1, 1], _unit([0.3, 0.1, -1]), True, 1.0, s, 3)
+        assert r["draws"] == 2 and r["valid"] and not r["delta"]
+        assert r["weight"] == pytest.approx([1, 1, 1])        # f cos / pdf = albedo = 1
+
+
+@pytest.mark.parametrize("kind,rough", [(METAL, 0.3), (METAL, 0.05), (PLASTIC, 0.2), (DIELECTRIC, 0.4)])
+def test_sample_weight_equals_eval_cos_over_pdf(orc, kind, rough):
+    m = orc.material(kind, color=(0.9, 0.7, 0.5), roughness=rough, ior=1.5)
+    alb = [0.9, 0.7, 0.5]
+    checked = 0
+    for s in range(300):
+        raydir = _unit([0.4 * math.sin(s), 0.3 * math.cos(3 * s), -1.0])
+        r = orc.bsdf_sample(m, alb, raydir, True, 1.0, s, 11)
+        if not r["valid"]:
+            continue
+        wo = -raydir
+        eta = 1.0 / 1.5 if kind == DIELECTRIC else 1.0
+        f, pdf = orc.bsdf_eval(m, alb, wo, r["wi"], eta)
+        assert pdf == pytest.approx(r["pdf"], rel=2e-3, abs=1e-6)
+        # both lobes of the rough dielectric: the lobe probability cancels
+        # the Fresnel term and the eta^2 radiance factor sits on both sides
+        expect = f * abs(r["wi"][2]) / r["pdf"]
+        np.testing.assert_allclose(r["weight"], expect, rtol=3e-3, atol=1e-6)
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("kind,rough", [(LAMBERT, 0.0), (METAL, 0.25), (PLASTIC, 0.3), (DIELECTRIC, 0.3)])
+def test_reflection_lobes_are_reciprocal(orc, kind, rough):
+    m = orc.material(kind, color=(0.8, 0.6, 0.4), roughness=rough, ior=1.5)
+    rng = np.random.default_rng(2)
+    a, b = _hemi(rng, 40), _hemi(rng, 40)
+    for wo, wi in zip(a, b):
+        f1, _ = orc.bsdf_eval(m, [0.8, 0.6, 0.4], wo, wi, 1.0 / 1.5)
+        f2, _ = orc.bsdf_eval(m, [0.8, 0.6, 0.4], wi, wo, 1.0 / 1.5)
+        np.testing.assert_allclose(f1, f2, rtol=1e-4, atol=1e-7)
+
+
+@pytest.mark.parametrize("rough", [0.1, 0.3, 0.6])
+def test_ggx_pdf_integral_closed_form(orc, rough):
+    # The VNDF pdf over the sphere integrates to the fraction of facet
+    # reflections that stay above the surface. At normal incidence a facet
+    # reflects below the horizon iff it is steeper than 45 degrees, and GGX
+    # gives P(tan(theta_h) > 1) = a^2 / (1 + a^2), so the integral is
+    # exactly 1 / (1 + a^2) with a = roughness^2. A missing G1, D or the
+    # 1/4 Jacobian breaks it.
+    # At normal incidence the pdf depends on theta only: integrate
+    # 2 pi * pdf(theta) sin(theta) dtheta with the midpoint rule on a grid
+    # fine enough for the narrowest lobe.
+    m = orc.material(METAL, color=(1, 1, 1), roughness=rough)
+    n = 20000
+    dth = (math.pi / 2) / n
+    tot = 0.0
+    for k in range(n):
+        th = (k + 0.5) * dth
+        _, pdf = orc.bsdf_eval(m, [1, 1, 1], [0, 0, 1], [math.sin(th), 0.0, math.cos(th)])
+        tot += pdf * math.sin(th)
+    integral = 2 * math.pi * tot * dth
+    a2 = (rough * rough) ** 2
+    assert integral == pytest.approx(1.0 / (1.0 + a2), rel=3e-3)
+
+
+def test_ggx_white_furnace_energy_bounded(orc):
+    # E(wo) = integral of f cos over the hemisphere <= 1 for a white metal
+    m = orc.material(METAL, color=(1, 1, 1), roughness=0.5)
+    rng = np.random.default_rng(4)
+    for wo in (_unit([0, 0, 1]), _unit([0.7, 0, 0.7]), _unit([0.95, 0, 0.3])):
+        tot = 0.0
+        n = 20000
+        for wi in _hemi(rng, n):
+            f, _ = orc.bsdf_eval(m, [1, 1, 1], wo, wi)
+            tot += f[0] * wi[2]
+        E = tot * 2 * math.pi / n
+        assert 0.6 < E <= 1.02, E
+
+
+def test_smooth_dielectric_draw_contract(orc):
+    m = orc.material(DIELECTRIC, roughness=0.0, ior=1.5)
+    r = orc.bsdf_sample(m, [1, 1, 1], _unit([0.2, 0, -1]), True, 1.0, 5, 1)
+    assert r["delta"] and r["valid"] and r["draws"] == 1
+    # from inside at a grazing angle: total internal reflection, no draw
+    r = orc.bsdf_sample(m, [1, 1, 1], _unit([0.95, 0, -0.3]), False, 1.0, 5, 1)
+    assert r["delta"] and r["draws"] == 0
+    assert r["wi"] == pytest.approx(_unit([0.95, 0, 0.3]), abs=1e-6)    # mirror
+    # plastic: exactly three draws on every sample
+    for s in range(20):
+        r = orc.bsdf_sample(orc.material(PLASTIC, roughness=0.2), [0.5] * 3, _unit([0.1, 0.2, -1]),
+                            True, 1.0, s, 2)
+        assert r["draws"] == 3
+
+
+def test_rough_metal_below_1e3_is_delta_mirror(orc):
+    m = orc.material(METAL, color=(1, 1, 1), roughness=5e-4)
+    r = orc.bsdf_sample(m, [1, 1, 1], _unit([0.3, 0.0, -1.0]), True, 1.0, 1, 1)
+    assert r["delta"] and r["draws"] == 0
+    assert r["wi"] == pytest.approx(_unit([0.3, 0.0, 1.0]), abs=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# geometry and traversal
+# ---------------------------------------------------------------------------
+
+def _scene(objects, **render):
+    s = sd.Scene()
+    s.render(width=8, height=8, spp=1, **render)
+    s.camera(lookfrom=[0, 0, 5], lookat=[0, 0, 0])
+    s.background_solid(color=[0, 0, 0])
+    objects(s)
+    return s.build()
+
+
+def test_sphere_and_rect_hit_distances_closed_form(orc):
+    def objs(s):
+        s.lambert("a", color=[0.5, 0.5, 0.5])
+        s.add("sphere", "a", scale(2.0), translate(0, 0, -10))
+        s.add("rect", "a", scale(3.0), rotate_x(90), translate(0, 0, -20))
+    h = _scene(objs)
+    flat = h.flatten()
+    rays = np.zeros((3, 8), np.float32)
+    rays[:, 7] = 1e16
+    rays[0, 4:7] = [0, 0, -1]                                 # through the sphere centre
+    rays[1, 0:3] = [0, 2.5, 0]
+    rays[1, 4:7] = [0, 0, -1]                                 # misses the sphere, hits the rect
+    rays[2, 0:3] = [0, 4.0, 0]
+    rays[2, 4:7] = [0, 0, -1]                                 # misses both
+    t, inst, prim = orc.trace(flat, rays)
+    assert t[0] == pytest.approx(8.0, abs=1e-5) and inst[0] == 0 and prim[0] == 0
+    assert t[1] == pytest.approx(20.0, abs=1e-5) and inst[1] == 1
+    assert inst[2] == -1 and math.isinf(t[2])
+
+
+def test_bvh_closest_hit_equals_brute_force(orc, asset_dir):
+    sc = configs.instanced(asset_dir, width=16, height=9, spp=1, instances=40, subdiv=2).build()
+    flat = sc.flatten()
+    rng = np.random.default_rng(9)
+    n = 3000
+    o = rng.uniform([-60, 1, -60], [60, 30, 60], (n, 3))
+    t = rng.uniform([-50, 0, -50], [50, 3, 50], (n, 3))
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7], rays[:, 7] = o, d, 1e16
+    a = orc.trace(flat, rays, brute=False)
+    b = orc.trace(flat, rays, brute=True)
+    assert (a[1] >= 0).sum() > n // 3
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_watertight_rays_through_shared_edges_and_vertices(orc, asset_dir):
+    v, f = configs.icosphere(2)
+    path = os.path.join(asset_dir, "wt_ico2.obj")
+    configs.write_obj(path, v, f)
+
+    def objs(s):
+        s.mesh("ico", path, normals="flat")
+        s.lambert("a", color=[0.5] * 3)
+        s.add("mesh:ico", "a")
+    flat = _scene(objs).flatten()
+    targets = [v[i] for i in range(len(v))]                       # vertices
+    targets += [(v[a] + v[b]) * 0.5 for a, b, _ in f[:200]]       # edge midpoints
+    targets = np.array(targets, np.float64)
+    n = len(targets)
+    rng = np.random.default_rng(5)
+    o = targets * 3.0 + rng.normal(scale=0.2, size=(n, 3))
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7], rays[:, 7] = o, d, 1e16
+    t, inst, prim = orc.trace(flat, rays, brute=True)
+    assert (inst == 0).all(), "%d rays slipped through a shared edge/vertex" % int((inst < 0).sum())
+
+
+# ---------------------------------------------------------------------------
+# transparent shadows (device/programs.cu:247-292)
+# ---------------------------------------------------------------------------
+
+def test_glass_slab_transmittance_closed_form(orc):
+    sigma = np.array([0.5, 1.0, 2.0])
+    thick = 0.2
+
+    def objs(s):
+        s.dielectric("g", ior=1.5, absorb=list(sigma))
+        s.add("rect", "g", scale(2.0), rotate_x(90), translate(0, 0, thick / 2))     # front, faces +z
+        s.add("rect", "g", scale(2.0), rotate_x(-90), translate(0, 0, -thick / 2))   # back, faces -z
+    flat = _scene(objs).flatten()
+    rays = np.zeros((1, 8), np.float32)
+    rays[0, 0:3] = [0.1, 0.2, 3.0]
+    rays[0, 4:7] = [0, 0, -1]
+    rays[0, 7] = 6.0
+    tr = orc.shadow(flat, rays)[0]
+    f0 = ((1 - 1 / 1.5) / (1 + 1 / 1.5)) ** 2                  # = ((1-1.5)/(1+1.5))^2 = 0.04
+    expect = (1 - f0) ** 2 * np.exp(-sigma * thick)
+    np.testing.assert_allclose(tr, expect, rtol=2e-5)
+    # opaque shadows mode: the same slab blocks completely
+    flat2 = _scene(objs, transparent_shadows=False).flatten()
+    assert np.array_equal(orc.shadow(flat2, rays)[0], np.zeros(3, np.float32))
+
+
+# ---------------------------------------------------------------------------
+# environment light (src/env_light.cpp:27-59, device/env_light.cuh)
+# ---------------------------------------------------------------------------
+
+def _env_scene(asset_dir, img, name):
+    path = os.path.join(asset_dir, name)
+    configs.write_hdr(path, img)
+
+    def objs(s):
+        s.background_envmap(path, importance=True)
+        s.lambert("a", color=[0.5] * 3)
+        s.add("sphere", "a", scale(0.1), translate(0, 0, -50))
+    return _scene(objs).flatten()
+
+
+def test_env_bins_equal_linear_scan(orc, asset_dir):
+    rng = np.random.default_rng(6)
+    img = rng.lognormal(size=(16, 32, 3)).astype(np.float32)
+    img[3] = 0.0                      # an all-black row
+    img[5, 4:9] = 0.0                 # flat spots inside a row
+    flat = _env_scene(asset_dir, img, "rand.hdr")
+    marg, cond = orc.env_cdf(flat, 32, 16)
+    u = rng.random((4000, 2)).astype(np.float32)
+    row, col = orc.env_bins(flat, u)
+    for k in range(len(u)):
+        r = max(i for i in range(16) if marg[i] <= u[k, 0])
+        c = max(i for i in range(32) if cond[r, i] <= u[k, 1])
+        assert (row[k], col[k]) == (r, c)
+    assert marg[0] == 0 and marg[16] == 1 and (np.diff(marg) >= 0).all()
+    assert (cond[:, 32] == 1).all() and (np.diff(cond, axis=1) >= 0).all()
+
+
+def test_white_environment_pdf_is_uniform_sphere(orc, asset_dir):
+    H, W = 64, 128
+    flat = _env_scene(asset_dir, np.ones((H, W, 3), np.float32), "white.hdr")
+    rng = np.random.default_rng(7)
+    sin_c = np.sin(np.pi * (np.arange(H) + 0.5) / H)
+    checked = 0
+    for u0, u1 in rng.random((300, 2)):
+        wi, pdf, back = orc.env_sample(flat, float(u0), float(u1))
+        if pdf == 0.0:
+            continue
+        # white image: dm = sin(theta_row) / sum, dc = 1 / W, hence
+        # pdf = sin(theta_row) H / (sum * 2 pi^2 sin(theta)) -> 1 / (4 pi)
+        th = math.acos(max(-1.0, min(1.0, float(wi[1]))))
+        row = min(int(th / math.pi * H), H - 1)
+        exact = sin_c[row] * H / (sin_c.sum() * 2 * math.pi ** 2 * math.sin(th))
+        assert pdf == pytest.approx(exact, rel=2e-3)
+        assert back == pytest.approx(pdf, rel=2e-3)
+        if abs(wi[1]) < 0.5:  # away from the poles the row quantisation is small
+            assert pdf == pytest.approx(1 / (4 * math.pi), rel=0.03)
+            checked += 1
+        assert np.linalg.norm(wi) == pytest.approx(1.0, abs=1e-5)
+    assert checked > 60
+
+
+# ---------------------------------------------------------------------------
+# flames (restating tests/host/test_volume.cpp:30-104)
+# ---------------------------------------------------------------------------
+
+def _flame_scene(flames):
+    s = sd.Scene()
+    s.render(width=8, height=8, spp=1)
+    s.camera(lookfrom=[0, 0, 5], lookat=[0, 0, 0])
+    s.lambert("a", color=[0.5] * 3)
+    s.add("sphere", "a", scale(0.01), translate(0, -50, 0))
+    for fl in flames:
+        s.flame(**fl)
+    return s.build().flatten()
+
+
+def _fl(base, sigma, seed):
+    return dict(base=base, height=2.0, radius=0.5, intensity=20.0, sigma=sigma, noise_scale=3.0, seed=seed)
+
+
+def test_flame_transmittance_draw_contract(orc):
+    flat = _flame_scene([_fl([0, 0, 0], 8.0, 7)])
+    o, d = [-2.0, 1.0, 0.0], [1.0, 0.0, 0.0]
+    tr, draws = orc.flame_transmittance(flat, [-2, 1, 5], d, 4.0, 1, 2)
+    assert tr == 1.0 and draws == 0                      # cylinder never entered
+    tr, draws = orc.flame_transmittance(flat, o, d, 1.0, 1, 2)
+    assert tr == 1.0 and draws == 0                      # segment ends before it
+    tr, draws = orc.flame_transmittance(flat, o, d, 4.0, 1, 2)
+    assert 0.0 < tr < 0.99 and draws == 1                # one jitter draw
+    tr, draws = orc.flame_transmittance(flat, o, d, 4.0, 1, 2, skip=0)
+    assert tr == 1.0 and draws == 0                      # owner exempt
+
+
+def test_two_flames_and_skip(orc):
+    flat = _flame_scene([_fl([0, 0, 0], 8.0, 7), _fl([0, 0, 1.5], 8.0, 11)])
+    o, d = [0, 1, -1], [0, 0, 1]
+    both, db = orc.flame_transmittance(flat, o, d, 4.0, 3, 4)
+    skip, ds = orc.flame_transmittance(flat, o, d, 4.0, 3, 4, skip=1)
+    assert db == 2 and ds == 1 and skip >= both
+
+
+def test_flame_transmittance_equals_marcher_bitwise(orc):
+    flat = _flame_scene([_fl([0, 0, 0], 4.5, 3)])
+    o, d = [-2.0, 1.0, 0.0], [1.0, 0.0, 0.0]
+    trans, L, dm = orc.march_flames(flat, o, d, 4.0, 9, 5)
+    tr, dt = orc.flame_transmittance(flat, o, d, 4.0, 9, 5)
+    assert trans[0] == np.float32(tr) and trans[1] == np.float32(tr) and trans[2] == np.float32(tr)
+    assert dm == dt == 1
+    assert (L > 0).any()
+
+
+def test_flame_sigma_monotonic(orc):
+    lo = _flame_scene([_fl([0, 0, 0], 4.0, 7)])
+    hi = _flame_scene([_fl([0, 0, 0], 8.0, 7)])
+    o, d = [-2.0, 1.0, 0.0], [1.0, 0.0, 0.0]
+    assert orc.flame_transmittance(hi, o, d, 4.0, 1, 2)[0] <= orc.flame_transmittance(lo, o, d, 4.0, 1, 2)[0]
+
+
+# ---------------------------------------------------------------------------
+# the path loop: exact closed forms
+# ---------------------------------------------------------------------------
+
+def test_direct_emitter_equals_le_and_convex_albedo_times_background(orc):
+    # camera sees (left) an emissive rect and (right) a lambert sphere of
+    # albedo a under a constant background c with no NEE strategy: the path
+    # value is exactly Le and exactly a*c (one bounce off a convex body).
+    s = sd.Scene()
+    s.render(width=32, height=16, spp=4, max_depth=4)
+    s.camera(lookfrom=[0, 0, 6], lookat=[0, 0, 0], vfov=40)
+    s.background_solid(color=[0.5, 0.25, 0.125])
+    s.emissive("lamp", color=[3.0, 2.0, 1.0], intensity=1.0)
+    s.lambert("ball", color=[0.6, 0.4, 0.2])
+    s.add("rect", "lamp", scale(0.6), rotate_x(90), translate(-1.2, 0, 0), nee=False)
+    s.add("sphere", "ball", scale(0.8), translate(1.2, 0, 0))
+    flat = s.build().flatten()
+    b, alb, _, _ = orc.render(flat, 32, 16, 0, 4)
+    # the frame spans x in [-4.37, 4.37] at z = 0: the lamp centre (x = -1.2)
+    # falls in column 11, the ball centre (x = 1.2) in column 20
+    cL = b[8, 11, :3]
+    cR = b[8, 20, :3]
+    np.testing.assert_allclose(cL, [3.0, 2.0, 1.0], rtol=1e-6)
+    np.testing.assert_allclose(cR, [0.6 * 0.5, 0.4 * 0.25, 0.2 * 0.125], rtol=1e-6)
+    np.testing.assert_allclose(alb[8, 20, :3], [0.6, 0.4, 0.2], rtol=1e-6)
+    np.testing.assert_allclose(b[0, 0, :3], [0.5, 0.25, 0.125], rtol=1e-6)   # background
+
+
+def test_distant_light_on_lambert_plane_closed_form(orc):
+    # a lambert plane (albedo a) under a distant light of radiance E at
+    # angle theta, black background: L = a / pi * E * cos(theta) exactly.
+    theta = math.radians(30.0)
+    s = sd.Scene()
+    s.render(width=16, height=16, spp=2, max_depth=3)
+    s.camera(lookfrom=[0, 5, 0.001], lookat=[0, 0, 0], vfov=20)
+    s.background_solid(color=[0, 0, 0])
+    s.lambert("floor", color=[0.5, 0.5, 0.5])
+    s.add("rect", "floor", scale(20))
+    s.distant_light(direction=[math.sin(theta), -math.cos(theta), 0.0], radiance=[2.0, 2.0, 2.0])
+    flat = s.build().flatten()
+    b, _, _, _ = orc.render(flat, 16, 16, 0, 2)
+    expect = 0.5 / math.pi * 2.0 * math.cos(theta)
+    np.testing.assert_allclose(b[..., :3], expect, rtol=2e-5)

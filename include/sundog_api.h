@@ -227,6 +227,9 @@ SUNDOG_API int sundog_session_reset(sundog_session*);
 SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int count);
 SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
+/* Launch-shape knobs picked by the autotuner. Keys: "waves" (2|3|4: the
+ * minimum waves per SIMD the render kernel's register budget allows). */
+SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);
 /* Waits for outstanding work. */
@@ -256,6 +259,66 @@ SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,
  * marginal row and conditional column picked by the binary searches. */
 SUNDOG_API int sundog_session_env_bins(sundog_session*, const float* u, int n,
                                        int32_t* row_out, int32_t* col_out);
+
+/* ---- serving: resident scenes + continuous batching ------------------------
+ * A server keeps every added scene resident (BVHs, tables, textures) and
+ * runs a worker thread that owns one GPU stream. Requests (scene, resolution,
+ * spp, seed) are admitted FIFO while their film fits the HBM budget
+ * (48 bytes per pixel: beauty + albedo + normal float4). Each batch renders
+ * the next pass_spp samples of every admitted request in one launch of the
+ * persistent tile-queue kernel; requests join and leave between batches.
+ * A request's film is bit-identical to rendering it alone. Times are
+ * milliseconds since the server was created. Requests larger than the whole
+ * budget complete immediately with status REJECTED. */
+typedef struct sundog_server sundog_server;
+
+#define SUNDOG_REQ_QUEUED 0
+#define SUNDOG_REQ_RUNNING 1
+#define SUNDOG_REQ_DONE 2
+#define SUNDOG_REQ_REJECTED 3
+#define SUNDOG_REQ_FAILED 4
+
+typedef struct {
+  int64_t id;
+  int32_t status;
+  int32_t scene;
+  int32_t width, height, spp;
+  int32_t pad;
+  double t_submit_ms, t_admit_ms, t_done_ms;
+  uint64_t paths;
+} sundog_request_info;
+
+typedef struct {
+  uint64_t paths, rays_traced, batches, work_items;
+  double kernel_ms, busy_ms;
+  uint64_t used_bytes, budget_bytes;
+  int32_t active, waiting, tile_size, waves, pass_spp, blocks_per_cu;
+} sundog_server_stats;
+
+SUNDOG_API sundog_server* sundog_server_create(uint64_t hbm_budget_bytes, int pass_spp,
+                                               int tile_size /* 8 | 16 | 32 */);
+SUNDOG_API void sundog_server_destroy(sundog_server*);
+/* Before the first submit. Returns the scene id, or -1. */
+SUNDOG_API int sundog_server_add_scene(sundog_server*, sundog_scene*, const sundog_render_options* opt);
+/* Keys: "tile" (8|16|32), "waves" (2|3|4), "pass_spp" (> 0). */
+SUNDOG_API int sundog_server_set_tuning(sundog_server*, const char* key, int value);
+/* Returns the request id (>= 0) or -1; seed < 0 uses the scene's seed. */
+SUNDOG_API int64_t sundog_server_submit(sundog_server*, int scene_id, int width, int height, int spp,
+                                        int64_t seed);
+/* Drains up to max completion records (done, rejected or failed); returns the count. */
+SUNDOG_API int sundog_server_poll(sundog_server*, sundog_request_info* out, int max);
+/* Copies a finished request's films (width*height float4 each; NULL skips). */
+SUNDOG_API int sundog_server_fetch(sundog_server*, int64_t id, float* beauty, float* albedo, float* normal);
+/* Frees a finished request's host-side record. */
+SUNDOG_API int sundog_server_release(sundog_server*, int64_t id);
+/* Blocks until no request is queued or running (timeout_s <= 0: no limit). */
+SUNDOG_API int sundog_server_wait_idle(sundog_server*, double timeout_s);
+SUNDOG_API int sundog_server_get_stats(sundog_server*, sundog_server_stats*);
+/* Baseline: one request rendered alone on a resident scene with the
+ * megakernel in 16-spp chunks, synchronising after each chunk. Only while
+ * the batching loop is idle. */
+SUNDOG_API int sundog_server_render_serial(sundog_server*, int scene_id, int width, int height, int spp,
+                                           int64_t seed, float* beauty, double* ms_out);
 
 /* PCG32 on device: n outputs of the stream init(seed, seq), raw u32 draws. */
 SUNDOG_API int sundog_device_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out);

@@ -59,6 +59,19 @@ class SessionStats(ctypes.Structure):
                 ("inst_transforms", c_u64)]
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", c_i64), ("status", c_i32), ("scene", c_i32), ("width", c_i32), ("height", c_i32),
+                ("spp", c_i32), ("pad", c_i32), ("t_submit_ms", c_d), ("t_admit_ms", c_d),
+                ("t_done_ms", c_d), ("paths", c_u64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("paths", c_u64), ("rays_traced", c_u64), ("batches", c_u64), ("work_items", c_u64),
+                ("kernel_ms", c_d), ("busy_ms", c_d), ("used_bytes", c_u64), ("budget_bytes", c_u64),
+                ("active", c_i32), ("waiting", c_i32), ("tile_size", c_i32), ("waves", c_i32),
+                ("pass_spp", c_i32), ("blocks_per_cu", c_i32)]
+
+
 def default_options(**kw):
     o = RenderOptions()
     o.out_path = None
@@ -121,6 +134,7 @@ SIGNATURES = {
     "sundog_session_set_kernel": (c_i, [c_p, c_i]),
     "sundog_session_set_count_rays": (c_i, [c_p, c_i]),
     "sundog_session_set_count_work": (c_i, [c_p, c_i]),
+    "sundog_session_set_tuning": (c_i, [c_p, c_s, c_i]),
     "sundog_session_sync": (c_i, [c_p]),
     "sundog_session_download": (c_i, [c_p, P_f, P_f, P_f]),
     "sundog_session_get_stats": (c_i, [c_p, ctypes.POINTER(SessionStats)]),
@@ -130,6 +144,17 @@ SIGNATURES = {
     "sundog_session_shadow": (c_i, [c_p, P_f, c_i, P_f]),
     "sundog_session_env_bins": (c_i, [c_p, P_f, c_i, P_i32, P_i32]),
     "sundog_device_pcg32": (c_i, [c_u64, c_u64, c_i, P_u32]),
+    "sundog_server_create": (c_p, [c_u64, c_i, c_i]),
+    "sundog_server_destroy": (None, [c_p]),
+    "sundog_server_add_scene": (c_i, [c_p, c_p, ctypes.POINTER(RenderOptions)]),
+    "sundog_server_set_tuning": (c_i, [c_p, c_s, c_i]),
+    "sundog_server_submit": (c_i64, [c_p, c_i, c_i, c_i, c_i, c_i64]),
+    "sundog_server_poll": (c_i, [c_p, ctypes.POINTER(RequestInfo), c_i]),
+    "sundog_server_fetch": (c_i, [c_p, c_i64, P_f, P_f, P_f]),
+    "sundog_server_release": (c_i, [c_p, c_i64]),
+    "sundog_server_wait_idle": (c_i, [c_p, c_d]),
+    "sundog_server_get_stats": (c_i, [c_p, ctypes.POINTER(ServerStats)]),
+    "sundog_server_render_serial": (c_i, [c_p, c_i, c_i, c_i, c_i, c_i64, P_f, ctypes.POINTER(c_d)]),
     "sundog_scene_flatten": (c_p, [c_p, ctypes.POINTER(RenderOptions)]),
     "sundog_flat_free": (None, [c_p]),
 }

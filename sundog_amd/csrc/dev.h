@@ -139,4 +139,16 @@ struct LaunchParams {
   uint32_t tlasRoot;         // packed reference of the top-level root
 };
 
+// Persistent tile-queue work item: one tile of one request for one pass.
+struct WorkItem {
+  int32_t request;
+  int32_t tile;  // tile index in row-major order over the request's frame
+};
+
+struct QueueState {
+  uint32_t next;  // work-item counter
+  uint32_t pass;  // pass counter advanced by the queue-prep kernel
+  uint32_t pad[2];
+};
+
 }  // namespace sdk

@@ -1481,8 +1481,22 @@ DEV vec3 evalBackground(const LaunchParams& P, vec3 dir) {
   return ld3(P.bgA);
 }
 
-DEV float sel4(const float a[4], int i) {
-  return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
+// Medium stack: four 16-bit material ids packed in two words. The
+// reference stores (sigma, ior) per level (device/programs.cu:364-367,
+// :531-542); both are per-material constants, so the id carries the same
+// information in 2 registers instead of 16.
+DEV uint32_t medGet(uint32_t lo, uint32_t hi, int level) {
+  uint32_t w = level < 2 ? lo : hi;
+  return (level & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+DEV void medSet(uint32_t& lo, uint32_t& hi, int level, uint32_t id) {
+  uint32_t& w = level < 2 ? lo : hi;
+  w = (level & 1) ? ((w & 0xFFFFu) | (id << 16)) : ((w & 0xFFFF0000u) | id);
+}
+
+DEV float4 runningMean(float4 m, vec3 x, int s) {
+  return make_float4(m.x + (x.x - m.x) / (s + 1), m.y + (x.y - m.y) / (s + 1), m.z + (x.z - m.z) / (s + 1),
+                     1.0f);
 }
 
 // 16x16-pixel tiles; each wave64 covers an 8x8 quarter.
@@ -1492,242 +1506,307 @@ DEV void pixelOf(int& px, int& py) {
   py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
-template <bool kCountWork>
-__global__ __launch_bounds__(kBlock) void k_render(LaunchParams P) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
-  int px, py;
-  pixelOf(px, py);
-  const bool active = px < P.width && py < P.height;
-  uint32_t rays = 0;
-  uint32_t work[4] = {0, 0, 0, 0};
-  if (active) {
-    const int tid = threadIdx.x;
-    const int gtid = (blockIdx.y * gridDim.x + blockIdx.x) * kBlock + tid;
-    Stack st;
-    st.lds = s_stack + tid;
-    st.spill = P.spill + gtid;
-    st.stride = P.spillStride;
-    st.sp = 0;
-    WorkCount wc;
-
+// One work-item renders samples [sampleOffset, +sppThisLaunch) of one pixel
+// in order. The sample loop and the bounce loop are merged: a lane whose
+// path ends starts its next sample at once instead of idling until the
+// slowest path of its wave finishes; per-pixel sample order (and so the
+// running mean) is unchanged.
+DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount& wc, uint32_t& rays) {
+  {
     const int pixel = py * P.width + px;
     int nStrata = (int)floorf(sqrtf((float)P.sppTotal) + 0.5f);
     if (nStrata * nStrata > P.sppTotal) nStrata--;
     if (nStrata < 1) nStrata = 1;
-
-    float4 accum = P.accum[pixel];
-    float4 aovA = P.albedo[pixel];
-    float4 aovN = P.normal[pixel];
-    const vec3 camU = ld3(P.cam.u), camV = ld3(P.cam.v), camW = ld3(P.cam.w);
     const bool hasEnv = P.hasEnv != 0;
     const int nStrat = P.numLights + (hasEnv ? 1 : 0);
 
-    for (int si = 0; si < P.sppThisLaunch; si++) {
-      const int s = P.sampleOffset + si;
-      Pcg rng = Pcg::make(((uint64_t)pixel << 32) ^ (uint64_t)s, (uint64_t)P.seed);
-      float jx, jy;
-      if (s < nStrata * nStrata) {
-        jx = ((s % nStrata) + rng.rnd()) / nStrata;
-        jy = ((s / nStrata) + rng.rnd()) / nStrata;
-      } else {
-        jx = rng.rnd();
-        jy = rng.rnd();
-      }
-      float u = (px + jx) / P.width;
-      float v = ((P.height - 1 - py) + jy) / P.height;
-      vec3 org = ld3(P.cam.origin);
-      if (P.cam.lensRadius > 0.0f) {
-        vec2 rd = concentricDisk(rng.rnd2()) * P.cam.lensRadius;
-        org = org + camU * rd.x + camV * rd.y;
-      }
-      vec3 dir = normalize(ld3(P.cam.lowerLeft) + u * ld3(P.cam.horizontal) + v * ld3(P.cam.vertical) - org);
+    Pcg rng;
+    vec3 L = v3(0.0f), beta = v3(1.0f), prevX = v3(0.0f), o = v3(0.0f), d = v3(0.0f);
+    float prevPdf = 0.0f;
+    bool specular = true, aovDone = false, alive = false;
+    uint32_t medLo = 0, medHi = 0;
+    int medDepth = 0, depth = 0, s = 0, si = 0;
 
-      vec3 L = v3(0.0f), beta = v3(1.0f);
-      float prevPdf = 0.0f;
-      bool specular = true;
-      vec3 prevX = org, o = org, d = dir;
-      bool aovDone = false;
-      float medSx[kMedMax], medSy[kMedMax], medSz[kMedMax], medIor[kMedMax];
-      int medDepth = 0;
-      const float invS = 1.0f / (float)(s + 1);
-      (void)invS;
-
-      for (int depth = 0; depth < P.maxDepth; depth++) {
-        Hit hit;
-        vec3 oObj = v3(0.0f), dObj = v3(0.0f);
-        bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oObj, dObj, wc);
-        rays++;
-
-        if (medDepth > 0) {
-          int top = min(medDepth, kMedMax) - 1;
-          vec3 sigma = vec3{sel4(medSx, top), sel4(medSy, top), sel4(medSz, top)};
-          if (maxc(sigma) > 0.0f) beta *= vexp(-sigma * (isHit ? hit.t : 1e4f));
+    while (true) {
+      if (!alive) {
+        if (si > 0) {  // the previous sample finished: fold it into the film
+          L = sanitize(L);
+          P.accum[pixel] = runningMean(P.accum[pixel], L, s);
         }
-        if (P.numFlames > 0) {
-          vec3 trans = v3(1.0f);
-          vec3 Lv = marchFlames(P, o, d, isHit ? hit.t : 1e16f, rng, trans);
-          vec3 cv = beta * Lv;
-          if (depth >= 1 && P.clampVal > 0.0f) cv = vmin(cv, v3(P.clampVal));
-          L += cv;
-          beta *= trans;
+        if (si == P.sppThisLaunch) break;
+        s = P.sampleOffset + si;
+        si++;
+        rng = Pcg::make(((uint64_t)pixel << 32) ^ (uint64_t)s, (uint64_t)P.seed);
+        float jx, jy;
+        if (s < nStrata * nStrata) {
+          jx = ((s % nStrata) + rng.rnd()) / nStrata;
+          jy = ((s / nStrata) + rng.rnd()) / nStrata;
+        } else {
+          jx = rng.rnd();
+          jy = rng.rnd();
         }
-        if (!isHit) {
-          vec3 bg = evalBackground(P, d);
+        float u = (px + jx) / P.width;
+        float v = ((P.height - 1 - py) + jy) / P.height;
+        vec3 org = ld3(P.cam.origin);
+        if (P.cam.lensRadius > 0.0f) {
+          vec2 rd = concentricDisk(rng.rnd2()) * P.cam.lensRadius;
+          org = org + ld3(P.cam.u) * rd.x + ld3(P.cam.v) * rd.y;
+        }
+        d = normalize(ld3(P.cam.lowerLeft) + u * ld3(P.cam.horizontal) + v * ld3(P.cam.vertical) - org);
+        o = org;
+        prevX = org;
+        L = v3(0.0f);
+        beta = v3(1.0f);
+        prevPdf = 0.0f;
+        specular = true;
+        aovDone = false;
+        medDepth = 0;
+        depth = 0;
+        alive = true;
+      }
+
+      // ---- one bounce (device/programs.cu:374-558) ----
+      Hit hit;
+      vec3 oObj = v3(0.0f), dObj = v3(0.0f);
+      bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oObj, dObj, wc);
+      rays++;
+      alive = false;  // set again at the bottom when the path continues
+
+      if (medDepth > 0) {
+        const DevMaterial& mm = P.materials[medGet(medLo, medHi, min(medDepth, kMedMax) - 1)];
+        vec3 sigma = ld3(mm.absorb);
+        if (maxc(sigma) > 0.0f) beta *= vexp(-sigma * (isHit ? hit.t : 1e4f));
+      }
+      if (P.numFlames > 0) {
+        vec3 trans = v3(1.0f);
+        vec3 Lv = marchFlames(P, o, d, isHit ? hit.t : 1e16f, rng, trans);
+        vec3 cv = beta * Lv;
+        if (depth >= 1 && P.clampVal > 0.0f) cv = vmin(cv, v3(P.clampVal));
+        L += cv;
+        beta *= trans;
+      }
+      if (!isHit) {
+        vec3 bg = evalBackground(P, d);
+        float w = 1.0f;
+        if (hasEnv && !specular) {
+          float pdfE = envPdf(P, d) / nStrat;
+          w = prevPdf / (prevPdf + pdfE);
+        }
+        vec3 c = beta * bg * w;
+        if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
+        L += c;
+        if (!aovDone) {
+          vec3 b = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
+          P.albedo[pixel] = runningMean(P.albedo[pixel], b, s);
+          P.normal[pixel] = runningMean(P.normal[pixel], v3(0.0f), s);
+        }
+        continue;
+      }
+
+      const DevInstance& in = P.instances[hit.inst];
+      ShadePoint sp = shadePoint(P, in, hit.prim, hit.b1, hit.b2, oObj, dObj, hit.t, true);
+      uint32_t matId = sp.frontface ? (in.mats & 0xFFFFu) : (in.mats >> 16);
+      vec3 ns = sp.frontface ? sp.nFront : -sp.nFront;
+      vec3 x = o + hit.t * d;
+      const DevMaterial& mat = P.materials[matId];
+      vec3 albedo = materialAlbedo(P, mat, sp.u, sp.v);
+      if (mat.kind == MT_WATER && mat.waveAmp > 0.0f)
+        ns = waterNormal(x, ns, -d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
+
+      if (!aovDone) {
+        vec3 nc = vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))};
+        P.albedo[pixel] = runningMean(P.albedo[pixel], albedo, s);
+        P.normal[pixel] = runningMean(P.normal[pixel], nc, s);
+        aovDone = true;
+      }
+
+      if (mat.kind == MT_EMISSIVE) {
+        if (sp.frontface || mat.twoSided) {
+          vec3 Le = albedo * mat.intensity;
           float w = 1.0f;
-          if (hasEnv && !specular) {
-            float pdfE = envPdf(P, d) / nStrat;
-            w = prevPdf / (prevPdf + pdfE);
+          if (!specular && in.lightId >= 0 && P.numLights > 0) {
+            float pdfL = lightPdfSolidAngle(P.lights[in.lightId], prevX, x, ns) / nStrat;
+            w = prevPdf / (prevPdf + pdfL);
           }
-          vec3 c = beta * bg * w;
+          vec3 c = beta * Le * w;
           if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
           L += c;
-          if (!aovDone) {
-            vec3 b = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
-            aovA = make_float4(aovA.x + (b.x - aovA.x) / (s + 1), aovA.y + (b.y - aovA.y) / (s + 1),
-                               aovA.z + (b.z - aovA.z) / (s + 1), 1.0f);
-            aovN = make_float4(aovN.x - aovN.x / (s + 1), aovN.y - aovN.y / (s + 1),
-                               aovN.z - aovN.z / (s + 1), 1.0f);
-            aovDone = true;
-          }
-          break;
         }
-
-        const DevInstance& in = P.instances[hit.inst];
-        ShadePoint sp = shadePoint(P, in, hit.prim, hit.b1, hit.b2, oObj, dObj, hit.t, true);
-        uint32_t matId = sp.frontface ? (in.mats & 0xFFFFu) : (in.mats >> 16);
-        vec3 ns = sp.frontface ? sp.nFront : -sp.nFront;
-        vec3 x = o + hit.t * d;
-        const DevMaterial& mat = P.materials[matId];
-        vec3 albedo = materialAlbedo(P, mat, sp.u, sp.v);
-        if (mat.kind == MT_WATER && mat.waveAmp > 0.0f)
-          ns = waterNormal(x, ns, -d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
-
-        if (!aovDone) {
-          vec3 nc = vec3{dot(ns, camU), dot(ns, camV), dot(ns, camW)};
-          aovA = make_float4(aovA.x + (albedo.x - aovA.x) / (s + 1), aovA.y + (albedo.y - aovA.y) / (s + 1),
-                             aovA.z + (albedo.z - aovA.z) / (s + 1), 1.0f);
-          aovN = make_float4(aovN.x + (nc.x - aovN.x) / (s + 1), aovN.y + (nc.y - aovN.y) / (s + 1),
-                             aovN.z + (nc.z - aovN.z) / (s + 1), 1.0f);
-          aovDone = true;
-        }
-
-        if (mat.kind == MT_EMISSIVE) {
-          if (sp.frontface || mat.twoSided) {
-            vec3 Le = albedo * mat.intensity;
-            float w = 1.0f;
-            if (!specular && in.lightId >= 0 && P.numLights > 0) {
-              float pdfL = lightPdfSolidAngle(P.lights[in.lightId], prevX, x, ns) / nStrat;
-              w = prevPdf / (prevPdf + pdfL);
-            }
-            vec3 c = beta * Le * w;
-            if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
-            L += c;
-          }
-          break;
-        }
-
-        float etaExt = 1.0f;
-        if (mat.kind == MT_DIELECTRIC || mat.kind == MT_WATER) {
-          int outer = min(sp.frontface ? medDepth : medDepth - 1, kMedMax);
-          if (outer > 0) etaExt = sel4(medIor, outer - 1);
-        }
-        float etaRel = sp.frontface ? etaExt / mat.ior : mat.ior / etaExt;
-
-        if (nStrat > 0 && !bsdfIsDelta(mat)) {
-          int k = min((int)(rng.rnd() * nStrat), nStrat - 1);
-          LightSample ls = (k < P.numLights) ? sampleLight(P, P.lights[k], x, rng) : sampleEnv(P, rng);
-          float cosS = ls.valid ? dot(ls.wi, ns) : 0.0f;
-          bool transmitNee = cosS < 0.0f && mat.kind == MT_DIELECTRIC;
-          if (ls.valid && (cosS > 0.0f || transmitNee)) {
-            rays++;
-            vec3 so = offsetRay(x, transmitNee ? -ns : ns);
-            float tmaxS = ls.dist * 0.999f;
-            vec3 tr = traceShadow(P, st, so, ls.wi, tmaxS, wc);
-            if (maxc(tr) > 0.0f) {
-              if (P.numFlames > 0 && P.transparentShadows) {
-                int skip = (k < P.numLights) ? P.lights[k].flameId : -1;
-                tr = tr * flameTransmittance(P, so, ls.wi, tmaxS, rng, skip);
-              }
-              float pdfLe = (ls.isDelta ? 1.0f : ls.pdf) / nStrat;
-              vec3 f = bsdfEval(mat, albedo, -d, ls.wi, ns, etaRel);
-              float w = 1.0f;
-              if (!ls.isDelta) {
-                float pdfB = bsdfPdf(mat, -d, ls.wi, ns, etaRel);
-                w = pdfLe / (pdfLe + pdfB);
-              }
-              vec3 c = beta * f * fabsf(cosS) * ls.Li * w / pdfLe;
-              c = c * tr;
-              if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
-              L += sanitize(c);
-            }
-          }
-        }
-
-        BsdfSample bs = bsdfSample(mat, albedo, d, ns, sp.frontface, etaExt, rng);
-        if (!bs.valid) break;
-        if ((mat.kind == MT_WATER || mat.kind == MT_DIELECTRIC) && dot(bs.wi, ns) < 0.0f) {
-          if (sp.frontface) {
-            if (medDepth < kMedMax) {
-              // register-resident stack write at a dynamic level
-#pragma unroll
-              for (int q = 0; q < kMedMax; q++) {
-                if (q == medDepth) {
-                  medSx[q] = mat.absorb[0];
-                  medSy[q] = mat.absorb[1];
-                  medSz[q] = mat.absorb[2];
-                  medIor[q] = mat.ior;
-                }
-              }
-            }
-            medDepth++;
-          } else if (medDepth > 0) {
-            medDepth--;
-          }
-        }
-        beta *= bs.weight;
-        specular = bs.isDelta;
-        prevPdf = bs.isDelta ? 0.0f : bs.pdf;
-        prevX = x;
-        vec3 offN = dot(bs.wi, ns) >= 0.0f ? ns : -ns;
-        o = offsetRay(x, offN);
-        d = bs.wi;
-        if (depth >= 4) {
-          float q = clampf(maxc(beta), 0.05f, 0.95f);
-          if (rng.rnd() >= q) break;
-          beta *= 1.0f / q;
-        }
-        if (maxc(beta) <= 0.0f) break;
+        continue;
       }
-      L = sanitize(L);
-      accum = make_float4(accum.x + (L.x - accum.x) / (s + 1), accum.y + (L.y - accum.y) / (s + 1),
-                          accum.z + (L.z - accum.z) / (s + 1), 1.0f);
-    }
-    P.accum[pixel] = accum;
-    P.albedo[pixel] = aovA;
-    P.normal[pixel] = aovN;
-    if (kCountWork) {
-      work[0] = wc.nodes;
-      work[1] = wc.tris;
-      work[2] = wc.quads;
-      work[3] = wc.insts;
+
+      float etaExt = 1.0f;
+      if (mat.kind == MT_DIELECTRIC || mat.kind == MT_WATER) {
+        int outer = min(sp.frontface ? medDepth : medDepth - 1, kMedMax);
+        if (outer > 0) etaExt = P.materials[medGet(medLo, medHi, outer - 1)].ior;
+      }
+      float etaRel = sp.frontface ? etaExt / mat.ior : mat.ior / etaExt;
+
+      if (nStrat > 0 && !bsdfIsDelta(mat)) {
+        int k = min((int)(rng.rnd() * nStrat), nStrat - 1);
+        LightSample ls = (k < P.numLights) ? sampleLight(P, P.lights[k], x, rng) : sampleEnv(P, rng);
+        float cosS = ls.valid ? dot(ls.wi, ns) : 0.0f;
+        bool transmitNee = cosS < 0.0f && mat.kind == MT_DIELECTRIC;
+        if (ls.valid && (cosS > 0.0f || transmitNee)) {
+          rays++;
+          vec3 so = offsetRay(x, transmitNee ? -ns : ns);
+          float tmaxS = ls.dist * 0.999f;
+          vec3 tr = traceShadow(P, st, so, ls.wi, tmaxS, wc);
+          if (maxc(tr) > 0.0f) {
+            if (P.numFlames > 0 && P.transparentShadows) {
+              int skip = (k < P.numLights) ? P.lights[k].flameId : -1;
+              tr = tr * flameTransmittance(P, so, ls.wi, tmaxS, rng, skip);
+            }
+            float pdfLe = (ls.isDelta ? 1.0f : ls.pdf) / nStrat;
+            vec3 f = bsdfEval(mat, albedo, -d, ls.wi, ns, etaRel);
+            float w = 1.0f;
+            if (!ls.isDelta) {
+              float pdfB = bsdfPdf(mat, -d, ls.wi, ns, etaRel);
+              w = pdfLe / (pdfLe + pdfB);
+            }
+            vec3 c = beta * f * fabsf(cosS) * ls.Li * w / pdfLe;
+            c = c * tr;
+            if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
+            L += sanitize(c);
+          }
+        }
+      }
+
+      BsdfSample bs = bsdfSample(mat, albedo, d, ns, sp.frontface, etaExt, rng);
+      if (!bs.valid) continue;
+      if ((mat.kind == MT_WATER || mat.kind == MT_DIELECTRIC) && dot(bs.wi, ns) < 0.0f) {
+        if (sp.frontface) {
+          if (medDepth < kMedMax) medSet(medLo, medHi, medDepth, matId);
+          medDepth++;
+        } else if (medDepth > 0) {
+          medDepth--;
+        }
+      }
+      beta *= bs.weight;
+      specular = bs.isDelta;
+      prevPdf = bs.isDelta ? 0.0f : bs.pdf;
+      prevX = x;
+      vec3 offN = dot(bs.wi, ns) >= 0.0f ? ns : -ns;
+      o = offsetRay(x, offN);
+      d = bs.wi;
+      if (depth >= 4) {
+        float q = clampf(maxc(beta), 0.05f, 0.95f);
+        if (rng.rnd() >= q) continue;
+        beta *= 1.0f / q;
+      }
+      if (maxc(beta) <= 0.0f) continue;
+      depth++;
+      alive = depth < P.maxDepth;
     }
   }
+}
+
+// Wave-level sums of the ray / work counters, one 64-bit atomic per wave.
+// Every lane of the wave must reach this call.
+template <bool kCountWork>
+DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned long long* workCounters,
+                       uint32_t rays, const WorkCount& wc) {
   if (kCountWork) {
+    uint32_t work[4] = {wc.nodes, wc.tris, wc.quads, wc.insts};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       unsigned long long v = work[c];
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-      if ((threadIdx.x & 63) == 0 && v) atomicAdd(P.workCounters + c, v);
+      if ((threadIdx.x & 63) == 0 && v) atomicAdd(workCounters + c, v);
     }
   }
-  if (P.countRays) {
-    // wave-level sum, one 64-bit atomic per wave
+  if (countRays) {
     uint32_t r = rays;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) r += __shfl_xor(r, off, 64);
-    if ((threadIdx.x & 63) == 0 && r) atomicAdd(P.rayCounter, (unsigned long long)r);
+    if ((threadIdx.x & 63) == 0 && r) atomicAdd(rayCounter, (unsigned long long)r);
   }
+}
+
+// kWaves: minimum waves per SIMD the register allocation must allow (2 fits
+// the kernel without spills; 3 and 4 trade scratch spills for occupancy).
+template <bool kCountWork, int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
+  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  int px, py;
+  pixelOf(px, py);
+  uint32_t rays = 0;
+  WorkCount wc;
+  if (px < P.width && py < P.height) {
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.spill = P.spill + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+    st.stride = P.spillStride;
+    st.sp = 0;
+    renderPixel(P, px, py, st, wc, rays);
+  }
+  flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
+}
+
+// ---------------------------------------------------------------------------
+// persistent tile-queue kernel: one launch drains (request, tile) work items
+// of many requests. Each wave takes the next item from a global counter and
+// renders that tile's pixels for the request's current pass; a pixel is only
+// ever touched by the wave holding its tile, so films stay bit-identical to a
+// render of the request alone.
+// ---------------------------------------------------------------------------
+
+// Prepares one graph-replayable pass: resets the work counter and writes
+// each request's sample offset for this pass (offset = first + pass * count,
+// clipped to the request's total). Runs as a kernel, never a memset, so it
+// can sit inside a captured graph.
+__global__ void k_queue_prep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq) {
+  __shared__ uint32_t s_pass;
+  if (threadIdx.x == 0) s_pass = q->pass;
+  __syncthreads();
+  for (int r = threadIdx.x; r < numReq; r += blockDim.x) {
+    LaunchParams& P = reqs[r];
+    int chunk = P.sppThisLaunch > 0 ? P.sppThisLaunch : 1;
+    int off = firstSample[r] + (int)s_pass * chunk;
+    P.sampleOffset = off;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    q->next = 0;
+    q->pass = s_pass + 1;
+  }
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void k_persistent(const LaunchParams* reqs, const WorkItem* items,
+                                                               int numItems, QueueState* q, int tileSize,
+                                                               uint32_t* spill, int spillStride,
+                                                               unsigned long long* rayCounter) {
+  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  const int lane = threadIdx.x & 63;
+  Stack st;
+  st.lds = s_stack + threadIdx.x;
+  st.spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.stride = spillStride;
+  st.sp = 0;
+  WorkCount wc;
+  uint32_t rays = 0;
+  const int pxPerTile = tileSize * tileSize;
+  while (true) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(&q->next, 1u);
+    item = __shfl(item, 0, 64);
+    if (item >= (uint32_t)numItems) break;
+    const WorkItem w = items[item];
+    const LaunchParams& P = reqs[w.request];
+    if (P.sampleOffset >= P.sppTotal) continue;  // request already complete
+    LaunchParams Pl = P;
+    Pl.sppThisLaunch = min(P.sppThisLaunch, P.sppTotal - P.sampleOffset);
+    const int tilesX = (P.width + tileSize - 1) / tileSize;
+    const int tx = (w.tile % tilesX) * tileSize, ty = (w.tile / tilesX) * tileSize;
+    for (int base = 0; base < pxPerTile; base += 64) {
+      int k = base + lane;
+      int px = tx + (k % tileSize), py = ty + (k / tileSize);
+      if (k < pxPerTile && px < P.width && py < P.height) renderPixel(Pl, px, py, st, wc, rays);
+    }
+  }
+  flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1849,12 +1928,16 @@ __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
 // host-side launch helpers (called from session.hip)
 // ---------------------------------------------------------------------------
 
-hipError_t launchRender(const LaunchParams& P, bool countWork, hipStream_t s) {
+hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s) {
   dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
   if (countWork)
-    hipLaunchKernelGGL(k_render<true>, grid, dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL((k_render<true, 2>), grid, dim3(kBlock), 0, s, P);
+  else if (waves >= 4)
+    hipLaunchKernelGGL((k_render<false, 4>), grid, dim3(kBlock), 0, s, P);
+  else if (waves == 3)
+    hipLaunchKernelGGL((k_render<false, 3>), grid, dim3(kBlock), 0, s, P);
   else
-    hipLaunchKernelGGL(k_render<false>, grid, dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL((k_render<false, 2>), grid, dim3(kBlock), 0, s, P);
   return hipGetLastError();
 }
 
@@ -1882,6 +1965,40 @@ hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row,
 hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg, dim3(1), dim3(64), 0, s, seed, seq, n, out);
   return hipGetLastError();
+}
+
+hipError_t launchQueuePrep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_queue_prep, dim3(1), dim3(256), 0, s, q, reqs, firstSample, numReq);
+  return hipGetLastError();
+}
+
+hipError_t launchPersistent(const LaunchParams* reqs, const WorkItem* items, int numItems, QueueState* q,
+                            int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
+                            unsigned long long* rayCounter, hipStream_t s) {
+  if (waves >= 4)
+    hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+                       spill, spillStride, rayCounter);
+  else if (waves == 3)
+    hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+                       spill, spillStride, rayCounter);
+  else
+    hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+                       spill, spillStride, rayCounter);
+  return hipGetLastError();
+}
+
+// Resident blocks per CU of the persistent kernel at a given occupancy variant.
+int persistentBlocksPerCU(int waves) {
+  int n = 0;
+  hipError_t e;
+  if (waves >= 4)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<4>, kBlock, 0);
+  else if (waves == 3)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<3>, kBlock, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<2>, kBlock, 0);
+  return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 int stackLdsEntries() { return kStackLds; }

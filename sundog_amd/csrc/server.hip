@@ -1,0 +1,534 @@
+// Serving loop: resident scenes, admission by HBM footprint, and continuous
+// batching of render requests through the persistent tile-queue kernel.
+//
+// The reference renders one scene per call and frees every device resource
+// on the way out (src/capi_render.cpp:65, :304); it has no request loop.
+// Here a worker thread owns the GPU stream. Each batch renders the next pass
+// (pass_spp samples) of every active request in ONE launch of the persistent
+// kernel, whose waves pull (request, tile) items from a shared counter.
+// Requests that arrive while a batch runs are admitted into the next batch
+// as soon as their film fits the HBM budget; requests that finish leave the
+// batch and their film is copied to host memory and freed. Per-pixel sample
+// order is the same as a render of the request alone, so each result is
+// bit-identical alone and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "session.h"
+
+using namespace sdk;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+template <class F>
+int guardedS(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    setLastError(e.what());
+    return SUNDOG_ERROR;
+  }
+}
+
+}  // namespace
+
+enum { REQ_QUEUED = 0, REQ_RUNNING = 1, REQ_DONE = 2, REQ_REJECTED = 3, REQ_FAILED = 4 };
+
+struct ServerRequest {
+  int64_t id = 0;
+  int scene = 0;
+  int width = 0, height = 0, spp = 0;
+  uint32_t seed = 0;
+  int done = 0;  // samples completed
+  int status = REQ_QUEUED;
+  size_t bytes = 0;
+  double tSubmit = 0, tAdmit = 0, tDone = 0;
+  DevBuf accum, albedo, normal;
+  std::vector<float> beauty, alb, nrm;  // host copies once done
+  LaunchParams P{};
+};
+
+struct sundog_server {
+  std::vector<std::unique_ptr<sundog_session>> scenes;
+  size_t budget = 0;
+  size_t used = 0;
+  int passSpp = 16;
+  int tileSize = 16;
+  int waves = 2;
+  int blocksPerCU = 1;
+  int numCUs = 1;
+  int spillRows = 0;
+  hipStream_t stream = nullptr;
+  DevBuf dReqs, dItems, dQueue, spill, rayCounter;
+  size_t reqCap = 0, itemCap = 0;
+  Clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv, idleCv;
+  std::deque<ServerRequest*> waiting;
+  std::vector<ServerRequest*> active;
+  std::map<int64_t, std::unique_ptr<ServerRequest>> all;
+  std::deque<int64_t> completed;  // ids not yet reported by poll
+  int64_t nextId = 0;
+  bool stop = false;
+  bool started = false;
+  std::string workerError;
+  std::thread worker;
+  // stats
+  uint64_t paths = 0, batches = 0, items = 0;
+  double kernelMs = 0, busyMs = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  double now() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+
+  ~sundog_server() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    if (worker.joinable()) worker.join();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  LaunchParams requestParams(const ServerRequest& r) const {
+    const sundog_session& sc = *scenes[r.scene];
+    LaunchParams P = sc.P;
+    P.width = r.width;
+    P.height = r.height;
+    P.sppTotal = r.spp;
+    P.seed = r.seed;
+    P.countRays = 1;
+    P.accum = r.accum.as<float4>();
+    P.albedo = r.albedo.as<float4>();
+    P.normal = r.normal.as<float4>();
+    P.rayCounter = rayCounter.as<unsigned long long>();
+    P.workCounters = nullptr;
+    Camera c = makeCamera(sc.prep.scene.camera, r.width, r.height);
+    auto st3 = [](float* d, vec3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
+    st3(P.cam.origin, c.origin);
+    st3(P.cam.lowerLeft, c.lowerLeft);
+    st3(P.cam.horizontal, c.horizontal);
+    st3(P.cam.vertical, c.vertical);
+    st3(P.cam.u, c.u);
+    st3(P.cam.v, c.v);
+    st3(P.cam.w, c.w);
+    P.cam.lensRadius = c.lensRadius;
+    return P;
+  }
+
+  static size_t filmBytes(int w, int h) { return (size_t)w * h * 48; }
+
+  void ensureDevice() {
+    if (stream) return;
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&ev0));
+    HIP_OK(hipEventCreate(&ev1));
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    numCUs = prop.multiProcessorCount;
+    dQueue.alloc(sizeof(QueueState));
+    rayCounter.alloc(8);
+    HIP_OK(hipMemsetAsync(rayCounter.p, 0, 8, stream));
+    applyTuning();
+  }
+
+  void applyTuning() {
+    blocksPerCU = persistentBlocksPerCU(waves);
+    size_t threads = (size_t)numCUs * blocksPerCU * renderBlockThreads();
+    int rows = std::max(0, spillRows);
+    spill.alloc(std::max<size_t>(16, (size_t)rows * threads * sizeof(uint32_t)));
+  }
+
+  // One batch: the next pass of every active request. Called by the worker
+  // with the lock released; only the worker touches `active`'s films.
+  void runBatch(std::vector<ServerRequest*>& batch) {
+    std::vector<LaunchParams> reqs(batch.size());
+    std::vector<WorkItem> work;
+    for (size_t i = 0; i < batch.size(); i++) {
+      ServerRequest& r = *batch[i];
+      reqs[i] = requestParams(r);
+      reqs[i].sampleOffset = r.done;
+      reqs[i].sppThisLaunch = std::min(passSpp, r.spp - r.done);
+      int tx = (r.width + tileSize - 1) / tileSize, ty = (r.height + tileSize - 1) / tileSize;
+      for (int t = 0; t < tx * ty; t++) work.push_back(WorkItem{(int32_t)i, t});
+    }
+    if (reqs.size() > reqCap) {
+      reqCap = reqs.size() * 2;
+      dReqs.alloc(reqCap * sizeof(LaunchParams));
+    }
+    if (work.size() > itemCap) {
+      itemCap = work.size() * 2;
+      dItems.alloc(itemCap * sizeof(WorkItem));
+    }
+    HIP_OK(hipMemcpyAsync(dReqs.p, reqs.data(), reqs.size() * sizeof(LaunchParams), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(dItems.p, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemsetAsync(dQueue.p, 0, sizeof(QueueState), stream));
+    int blocks = numCUs * blocksPerCU;
+    size_t threads = (size_t)blocks * renderBlockThreads();
+    HIP_OK(hipEventRecord(ev0, stream));
+    HIP_OK(launchPersistent(dReqs.as<LaunchParams>(), dItems.as<WorkItem>(), (int)work.size(),
+                            dQueue.as<QueueState>(), tileSize, waves, blocks, spill.as<uint32_t>(), (int)threads,
+                            rayCounter.as<unsigned long long>(), stream));
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    kernelMs += ms;
+    batches++;
+    items += work.size();
+    for (size_t i = 0; i < batch.size(); i++) {
+      ServerRequest& r = *batch[i];
+      paths += (uint64_t)r.width * r.height * reqs[i].sppThisLaunch;
+      r.done += reqs[i].sppThisLaunch;
+    }
+  }
+
+  void finish(ServerRequest& r) {
+    size_t n = (size_t)r.width * r.height * 4;
+    r.beauty.resize(n);
+    r.alb.resize(n);
+    r.nrm.resize(n);
+    HIP_OK(hipMemcpyAsync(r.beauty.data(), r.accum.p, n * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(r.alb.data(), r.albedo.p, n * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipMemcpyAsync(r.nrm.data(), r.normal.p, n * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    r.accum.release();
+    r.albedo.release();
+    r.normal.release();
+  }
+
+  void admitLocked() {
+    while (!waiting.empty()) {
+      ServerRequest* r = waiting.front();
+      if (used + r->bytes > budget) break;  // FIFO: wait for memory to free up
+      waiting.pop_front();
+      r->tAdmit = now();
+      r->status = REQ_RUNNING;
+      used += r->bytes;
+      active.push_back(r);
+    }
+  }
+
+  void loop() {
+    try {
+      while (true) {
+        std::vector<ServerRequest*> batch;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          admitLocked();
+          while (active.empty() && !stop) {
+            idleCv.notify_all();
+            cv.wait(lk);
+            admitLocked();
+          }
+          if (stop && active.empty()) break;
+          batch = active;
+        }
+        // films for newly admitted requests
+        for (ServerRequest* r : batch) {
+          if (!r->accum.p) {
+            size_t n = (size_t)r->width * r->height * 16;
+            r->accum.alloc(n);
+            r->albedo.alloc(n);
+            r->normal.alloc(n);
+            HIP_OK(hipMemsetAsync(r->accum.p, 0, n, stream));
+            HIP_OK(hipMemsetAsync(r->albedo.p, 0, n, stream));
+            HIP_OK(hipMemsetAsync(r->normal.p, 0, n, stream));
+          }
+        }
+        double tb = now();
+        runBatch(batch);
+        std::vector<ServerRequest*> finished;
+        for (ServerRequest* r : batch)
+          if (r->done >= r->spp) finished.push_back(r);
+        for (ServerRequest* r : finished) finish(*r);
+        busyMs += now() - tb;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          for (ServerRequest* r : finished) {
+            r->tDone = now();
+            r->status = REQ_DONE;
+            used -= r->bytes;
+            active.erase(std::find(active.begin(), active.end(), r));
+            completed.push_back(r->id);
+          }
+        }
+      }
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(mu);
+      workerError = e.what();
+      for (ServerRequest* r : active) {
+        r->status = REQ_FAILED;
+        completed.push_back(r->id);
+      }
+      active.clear();
+    }
+    idleCv.notify_all();
+  }
+
+  void start() {
+    if (started) return;
+    ensureDevice();
+    started = true;
+    worker = std::thread([this] { loop(); });
+  }
+};
+
+extern "C" sundog_server* sundog_server_create(uint64_t hbm_budget_bytes, int pass_spp, int tile_size) {
+  try {
+    if (pass_spp <= 0) pass_spp = 16;
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+      throw std::runtime_error("server: tile size must be 8, 16 or 32");
+    std::unique_ptr<sundog_server> s(new sundog_server());
+    s->budget = hbm_budget_bytes;
+    s->passSpp = pass_spp;
+    s->tileSize = tile_size;
+    s->t0 = Clock::now();
+    return s.release();
+  } catch (const std::exception& e) {
+    setLastError(e.what());
+    return nullptr;
+  }
+}
+
+extern "C" void sundog_server_destroy(sundog_server* s) { delete s; }
+
+extern "C" int sundog_server_add_scene(sundog_server* s, sundog_scene* h, const sundog_render_options* opt) {
+  try {
+    if (!s || !h) throw std::runtime_error("server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->started) throw std::runtime_error("server: scenes must be added before the first submit");
+    sundog_session* sc = sundog_session_create(h, opt);
+    if (!sc) throw std::runtime_error(sundog_last_error());
+    s->scenes.emplace_back(sc);
+    s->spillRows = std::max(s->spillRows, sc->spillRows);
+    return (int)s->scenes.size() - 1;
+  } catch (const std::exception& e) {
+    setLastError(e.what());
+    return -1;
+  }
+}
+
+extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int value) {
+  return guardedS([&] {
+    if (!s || !key) throw std::runtime_error("server: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::string k = key;
+    if (k == "tile") {
+      if (value != 8 && value != 16 && value != 32) throw std::runtime_error("tuning: tile must be 8, 16 or 32");
+      s->tileSize = value;
+    } else if (k == "waves") {
+      if (value < 2 || value > 4) throw std::runtime_error("tuning: waves must be 2, 3 or 4");
+      s->waves = value;
+      if (s->stream) s->applyTuning();
+    } else if (k == "pass_spp") {
+      if (value <= 0) throw std::runtime_error("tuning: pass_spp must be positive");
+      s->passSpp = value;
+    } else {
+      throw std::runtime_error("tuning: unknown key '" + k + "'");
+    }
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int64_t sundog_server_submit(sundog_server* s, int scene_id, int width, int height, int spp,
+                                        int64_t seed) {
+  try {
+    if (!s) throw std::runtime_error("server: null handle");
+    if (scene_id < 0 || scene_id >= (int)s->scenes.size()) throw std::runtime_error("submit: scene id out of range");
+    if (width <= 0 || height <= 0 || spp <= 0) throw std::runtime_error("submit: width, height and spp must be positive");
+    std::unique_ptr<ServerRequest> r(new ServerRequest());
+    r->scene = scene_id;
+    r->width = width;
+    r->height = height;
+    r->spp = spp;
+    r->seed = seed >= 0 ? (uint32_t)seed : s->scenes[scene_id]->P.seed;
+    r->bytes = sundog_server::filmBytes(width, height);
+    int64_t id;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!s->workerError.empty()) throw std::runtime_error("server failed: " + s->workerError);
+      id = s->nextId++;
+      r->id = id;
+      r->tSubmit = s->now();
+      ServerRequest* raw = r.get();
+      s->all[id] = std::move(r);
+      if (raw->bytes > s->budget) {
+        raw->status = REQ_REJECTED;
+        raw->tDone = raw->tSubmit;
+        s->completed.push_back(id);
+      } else {
+        s->waiting.push_back(raw);
+      }
+    }
+    s->start();
+    s->cv.notify_all();
+    return id;
+  } catch (const std::exception& e) {
+    setLastError(e.what());
+    return -1;
+  }
+}
+
+extern "C" int sundog_server_poll(sundog_server* s, sundog_request_info* out, int max) {
+  if (!s || (!out && max > 0)) {
+    setLastError("poll: null argument");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  int n = 0;
+  while (n < max && !s->completed.empty()) {
+    int64_t id = s->completed.front();
+    s->completed.pop_front();
+    const ServerRequest& r = *s->all[id];
+    sundog_request_info& i = out[n++];
+    i = sundog_request_info{};
+    i.id = id;
+    i.status = r.status;
+    i.scene = r.scene;
+    i.width = r.width;
+    i.height = r.height;
+    i.spp = r.spp;
+    i.t_submit_ms = r.tSubmit;
+    i.t_admit_ms = r.tAdmit;
+    i.t_done_ms = r.tDone;
+    i.paths = (uint64_t)r.width * r.height * r.done;
+  }
+  return n;
+}
+
+extern "C" int sundog_server_fetch(sundog_server* s, int64_t id, float* beauty, float* albedo, float* normal) {
+  return guardedS([&] {
+    if (!s) throw std::runtime_error("fetch: null handle");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) throw std::runtime_error("fetch: unknown request id");
+    ServerRequest& r = *it->second;
+    if (r.status != REQ_DONE) throw std::runtime_error("fetch: request is not complete");
+    size_t n = (size_t)r.width * r.height * 4;
+    if (beauty) memcpy(beauty, r.beauty.data(), n * 4);
+    if (albedo) memcpy(albedo, r.alb.data(), n * 4);
+    if (normal) memcpy(normal, r.nrm.data(), n * 4);
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_server_release(sundog_server* s, int64_t id) {
+  return guardedS([&] {
+    if (!s) throw std::runtime_error("release: null handle");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) throw std::runtime_error("release: unknown request id");
+    if (it->second->status == REQ_QUEUED || it->second->status == REQ_RUNNING)
+      throw std::runtime_error("release: request still in flight");
+    s->all.erase(it);
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_server_wait_idle(sundog_server* s, double timeout_s) {
+  return guardedS([&] {
+    if (!s) throw std::runtime_error("wait: null handle");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto idle = [&] { return (s->active.empty() && s->waiting.empty()) || !s->workerError.empty(); };
+    if (timeout_s <= 0) {
+      s->idleCv.wait(lk, idle);
+    } else if (!s->idleCv.wait_for(lk, std::chrono::duration<double>(timeout_s), idle)) {
+      throw std::runtime_error("wait: timed out");
+    }
+    if (!s->workerError.empty()) throw std::runtime_error("server failed: " + s->workerError);
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_server_get_stats(sundog_server* s, sundog_server_stats* out) {
+  return guardedS([&] {
+    if (!s || !out) throw std::runtime_error("stats: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = sundog_server_stats{};
+    out->paths = s->paths;
+    if (s->rayCounter.p) {
+      unsigned long long rays = 0;
+      HIP_OK(hipMemcpy(&rays, s->rayCounter.p, 8, hipMemcpyDeviceToHost));
+      out->rays_traced = rays;
+    }
+    out->batches = s->batches;
+    out->work_items = s->items;
+    out->kernel_ms = s->kernelMs;
+    out->busy_ms = s->busyMs;
+    out->used_bytes = s->used;
+    out->budget_bytes = s->budget;
+    out->active = (int)s->active.size();
+    out->waiting = (int)s->waiting.size();
+    out->tile_size = s->tileSize;
+    out->waves = s->waves;
+    out->pass_spp = s->passSpp;
+    out->blocks_per_cu = s->blocksPerCU;
+    return SUNDOG_OK;
+  });
+}
+
+// The per-request baseline: one request rendered alone on the resident
+// scene with the megakernel, in chunks of 16 spp with a synchronisation
+// after every chunk (the reference's launch pattern, src/capi_render.cpp:232-243).
+extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int width, int height, int spp,
+                                           int64_t seed, float* beauty, double* ms_out) {
+  return guardedS([&] {
+    if (!s) throw std::runtime_error("serial: null handle");
+    if (scene_id < 0 || scene_id >= (int)s->scenes.size()) throw std::runtime_error("serial: scene id out of range");
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!s->active.empty() || !s->waiting.empty())
+        throw std::runtime_error("serial: the batching loop is busy");
+    }
+    s->ensureDevice();
+    ServerRequest r;
+    r.scene = scene_id;
+    r.width = width;
+    r.height = height;
+    r.spp = spp;
+    r.seed = seed >= 0 ? (uint32_t)seed : s->scenes[scene_id]->P.seed;
+    size_t n = (size_t)width * height * 16;
+    r.accum.alloc(n);
+    r.albedo.alloc(n);
+    r.normal.alloc(n);
+    HIP_OK(hipMemsetAsync(r.accum.p, 0, n, s->stream));
+    HIP_OK(hipMemsetAsync(r.albedo.p, 0, n, s->stream));
+    HIP_OK(hipMemsetAsync(r.normal.p, 0, n, s->stream));
+    LaunchParams P = s->requestParams(r);
+    sundog_session& sc = *s->scenes[scene_id];
+    size_t threads = (size_t)((width + 15) / 16) * ((height + 15) / 16) * renderBlockThreads();
+    DevBuf sp;
+    sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * 4));
+    P.spill = sp.as<uint32_t>();
+    P.spillStride = (int)threads;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    auto t = Clock::now();
+    for (int done = 0; done < spp; done += 16) {
+      P.sampleOffset = done;
+      P.sppThisLaunch = std::min(16, spp - done);
+      HIP_OK(launchRender(P, false, sc.waves, s->stream));
+      HIP_OK(hipStreamSynchronize(s->stream));
+    }
+    double ms = std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+    if (ms_out) *ms_out = ms;
+    if (beauty) HIP_OK(hipMemcpy(beauty, r.accum.p, n, hipMemcpyDeviceToHost));
+    return SUNDOG_OK;
+  });
+}

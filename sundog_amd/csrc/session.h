@@ -1,0 +1,570 @@
+// Resident render session: device scene tables, two-level BVH, film and
+// launch state of one scene on one GPU. Shared by the session entry points
+// and the serving loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sundog_api.h"
+#include "bvh.h"
+#include "dev.h"
+#include "prepare.h"
+
+namespace sdk {
+
+hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s);
+hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool brute, int numInst, float* t,
+                       int* inst, int* prim, int* counters, hipStream_t s);
+hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
+hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
+hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s);
+hipError_t launchQueuePrep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq,
+                           hipStream_t s);
+hipError_t launchPersistent(const LaunchParams* reqs, const WorkItem* items, int numItems, QueueState* q,
+                            int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
+                            unsigned long long* rayCounter, hipStream_t s);
+int persistentBlocksPerCU(int waves);
+int stackLdsEntries();
+int renderBlockThreads();
+
+#define HIP_OK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                               #expr);                                                        \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    HIP_OK(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T>
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+// Environment CDFs over luminance * sin(theta), prefix sums in double, stored
+// as float with the last entry pinned to 1 (reference src/env_light.cpp:27-59).
+inline bool buildEnvCdf(const HostImage& img, std::vector<float>& marg, std::vector<float>& cond) {
+  int w = img.width, h = img.height;
+  marg.assign(h + 1, 0.0f);
+  cond.assign((size_t)h * (w + 1), 0.0f);
+  double total = 0.0;
+  for (int row = 0; row < h; row++) {
+    float sinT = sinf(kPi * (row + 0.5f) / h);
+    float* cdf = &cond[(size_t)row * (w + 1)];
+    double sum = 0.0;
+    cdf[0] = 0.0f;
+    for (int col = 0; col < w; col++) {
+      const float* p = &img.rgba[4 * ((size_t)row * w + col)];
+      float lum = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+      sum += (double)fmaxf(lum, 0.0f) * sinT;
+      cdf[col + 1] = (float)sum;
+    }
+    if (sum > 0.0) {
+      for (int c = 1; c <= w; c++) cdf[c] = (float)(cdf[c] / sum);
+    } else {
+      for (int c = 0; c <= w; c++) cdf[c] = (float)c / w;
+    }
+    cdf[w] = 1.0f;
+    total += sum;
+    marg[row + 1] = (float)total;
+  }
+  bool degenerate = total <= 0.0;
+  if (!degenerate) {
+    for (int r = 1; r <= h; r++) marg[r] = (float)(marg[r] / total);
+  } else {
+    for (int r = 0; r <= h; r++) marg[r] = (float)r / h;
+  }
+  marg[h] = 1.0f;
+  return !degenerate;
+}
+
+inline Aabb quadricBox(int kind) {
+  const float e = 1e-4f;
+  Aabb b;
+  switch (kind) {
+    case GK_RECT:
+    case GK_DISK:
+      b.lo = v3(-1, -e, -1);
+      b.hi = v3(1, e, 1);
+      break;
+    case GK_PARABOLA:
+      b.lo = v3(-1, -e, -1);
+      b.hi = v3(1, 0.5f + e, 1);
+      break;
+    default:
+      b.lo = v3(-1, -1, -1);
+      b.hi = v3(1, 1, 1);
+  }
+  return b;
+}
+
+inline Aabb transformBox(const Affine& m, const Aabb& b) {
+  Aabb r;
+  for (int i = 0; i < 8; i++) {
+    vec3 c = vec3{(i & 1) ? b.hi.x : b.lo.x, (i & 2) ? b.hi.y : b.lo.y, (i & 4) ? b.hi.z : b.lo.z};
+    r.grow(m.point(c));
+  }
+  // widen by a few ulps of the magnitude so rounding in the transform never
+  // shrinks the box below the surface it bounds
+  vec3 mag = vmax(vec3{fabsf(r.lo.x), fabsf(r.lo.y), fabsf(r.lo.z)}, vec3{fabsf(r.hi.x), fabsf(r.hi.y), fabsf(r.hi.z)});
+  vec3 pad = mag * 2e-6f + v3(1e-7f);
+  r.lo = r.lo - pad;
+  r.hi = r.hi + pad;
+  return r;
+}
+
+inline void relocate(std::vector<BvhNode4>& nodes, size_t from, size_t count, uint32_t nodeBase,
+                     uint32_t primBase, const std::vector<uint32_t>* leafMap) {
+  for (size_t i = from; i < from + count; i++) {
+    for (int c = 0; c < 4; c++) {
+      uint32_t r = (uint32_t)nodes[i].child[c];
+      if (r == kRefEmpty) continue;
+      if (r & kRefLeaf) {
+        uint32_t first = r & 0x07FFFFFFu, hi = r & 0xF8000000u;
+        uint32_t nf = leafMap ? (*leafMap)[first] : first + primBase;
+        if (nf >= (1u << 27)) throw std::runtime_error("bvh: primitive index exceeds the packed range");
+        nodes[i].child[c] = (int32_t)(hi | nf);
+      } else {
+        nodes[i].child[c] = (int32_t)(r + nodeBase);
+      }
+    }
+  }
+}
+
+}  // namespace sdk
+
+using namespace sdk;
+
+struct sundog_session {
+  hipStream_t stream = nullptr;
+  Prepared prep;
+  LaunchParams P{};
+  int kernel = SUNDOG_KERNEL_MEGA;
+  bool countWork = false;
+  int waves = 2;  // occupancy variant of the render kernel (2, 3 or 4 waves per SIMD)
+  int numInstances = 0;
+  int stackNeed = 0;
+  int spillRows = 0;
+  DevBuf textures, texels, materials, lights, flames, instances, nodes, tris, meshes;
+  DevBuf positions, indices, normals, uvs, lightPos, lightCdf;
+  DevBuf envTex, margCdf, condCdf;
+  DevBuf accum, albedo, normal, rayCounter, workCounters, spill;
+  size_t spillThreads = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double renderMs = 0.0;
+  double buildMs = 0.0;
+  uint64_t paths = 0;
+  int tlasNodes = 0, blasNodes = 0;
+
+  ~sundog_session() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  uint64_t deviceBytes() const {
+    const DevBuf* all[] = {&textures, &texels, &materials, &lights, &flames, &instances, &nodes, &tris,
+                           &meshes, &positions, &indices, &normals, &uvs, &lightPos, &lightCdf, &envTex,
+                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill};
+    uint64_t b = 0;
+    for (const DevBuf* d : all) b += d->bytes;
+    return b;
+  }
+
+  void ensureSpill(size_t threads) {
+    if (spillRows <= 0) {
+      if (!spill.p) spill.alloc(16);
+      P.spill = spill.as<uint32_t>();
+      P.spillStride = 0;
+      P.spillRows = 0;
+      return;
+    }
+    if (threads > spillThreads) {
+      HIP_OK(hipStreamSynchronize(stream));
+      spill.alloc((size_t)spillRows * threads * sizeof(uint32_t));
+      spillThreads = threads;
+    }
+    P.spill = spill.as<uint32_t>();
+    P.spillStride = (int)threads;
+    P.spillRows = spillRows;
+  }
+
+  void build(const Scene& s) {
+    auto t0 = std::chrono::steady_clock::now();
+    prepareScene(s, prep);
+    const Scene& S = prep.scene;
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&ev0));
+    HIP_OK(hipEventCreate(&ev1));
+
+    // textures + texel pool
+    std::vector<DevTexture> dt;
+    std::vector<float4> tx;
+    for (size_t i = 0; i < S.textures.size(); i++) {
+      const Texture& t = S.textures[i];
+      DevTexture d{};
+      d.kind = t.kind;
+      d.a[0] = t.a.x; d.a[1] = t.a.y; d.a[2] = t.a.z;
+      d.b[0] = t.b.x; d.b[1] = t.b.y; d.b[2] = t.b.z;
+      d.scale[0] = t.scale.x;
+      d.scale[1] = t.scale.y;
+      d.width = t.width;
+      d.imgOffset = -1;
+      if (t.kind == TX_IMAGE) {
+        const HostImage& im = prep.texImages[i];
+        d.imgOffset = (int)tx.size();
+        d.imgW = im.width;
+        d.imgH = im.height;
+        for (size_t k = 0; k < (size_t)im.width * im.height; k++)
+          tx.push_back(make_float4(im.rgba[4 * k], im.rgba[4 * k + 1], im.rgba[4 * k + 2], im.rgba[4 * k + 3]));
+      }
+      dt.push_back(d);
+    }
+    textures.upload(dt, stream);
+    texels.upload(tx, stream);
+
+    std::vector<DevMaterial> dm;
+    for (const Material& m : S.materials) {
+      DevMaterial d{};
+      d.kind = m.kind;
+      d.texId = m.texId;
+      d.color[0] = m.color.x; d.color[1] = m.color.y; d.color[2] = m.color.z;
+      d.roughness = m.roughness;
+      d.ior = m.ior;
+      d.intensity = m.intensity;
+      d.twoSided = m.twoSided;
+      d.absorb[0] = m.absorb.x; d.absorb[1] = m.absorb.y; d.absorb[2] = m.absorb.z;
+      d.waveAmp = m.waveAmp;
+      d.waveFreq = m.waveFreq;
+      d.waveOctaves = m.waveOctaves;
+      dm.push_back(d);
+    }
+    materials.upload(dm, stream);
+
+    // mesh pools
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> idx;
+    std::vector<DevMesh> dmesh;
+    for (const HostMesh& m : prep.meshes) {
+      DevMesh d{};
+      d.posOffset = (int)(pos.size() / 3);
+      d.idxOffset = (int)idx.size();
+      d.numTris = (int)m.numTris();
+      d.numVerts = (int)m.positions.size();
+      for (const vec3& p : m.positions) { pos.push_back(p.x); pos.push_back(p.y); pos.push_back(p.z); }
+      idx.insert(idx.end(), m.indices.begin(), m.indices.end());
+      d.nrmOffset = -1;
+      if (!m.normals.empty()) {
+        d.nrmOffset = (int)(nrm.size() / 3);
+        for (const vec3& n : m.normals) { nrm.push_back(n.x); nrm.push_back(n.y); nrm.push_back(n.z); }
+      }
+      d.uvOffset = -1;
+      if (!m.uvs.empty()) {
+        d.uvOffset = (int)(uv.size() / 2);
+        for (const vec2& t : m.uvs) { uv.push_back(t.x); uv.push_back(t.y); }
+      }
+      dmesh.push_back(d);
+    }
+
+    // lights (+ mesh-light pools)
+    std::vector<DevLight> dl;
+    std::vector<float> lpos, lcdf;
+    for (size_t i = 0; i < S.lights.size(); i++) {
+      const Light& l = S.lights[i];
+      DevLight d{};
+      d.kind = l.kind;
+      d.p[0] = l.p.x; d.p[1] = l.p.y; d.p[2] = l.p.z;
+      d.u[0] = l.u.x; d.u[1] = l.u.y; d.u[2] = l.u.z;
+      d.v[0] = l.v.x; d.v[1] = l.v.y; d.v[2] = l.v.z;
+      d.n[0] = l.n.x; d.n[1] = l.n.y; d.n[2] = l.n.z;
+      d.L[0] = l.L.x; d.L[1] = l.L.y; d.L[2] = l.L.z;
+      d.radius = l.radius;
+      d.area = l.area;
+      d.dir[0] = l.dir.x; d.dir[1] = l.dir.y; d.dir[2] = l.dir.z;
+      d.twoSided = l.twoSided;
+      d.texId = l.texId;
+      d.flameId = l.flameId;
+      d.mPosOffset = d.mIdxOffset = d.mUvOffset = d.mCdfOffset = -1;
+      d.mNumTris = 0;
+      d.mNgSign = 1.0f;
+      dl.push_back(d);
+    }
+    for (const MeshLightPatch& p : prep.meshLights) {
+      DevLight& d = dl[p.lightId];
+      const Object& o = S.objects[p.objectId];
+      d.mPosOffset = (int)(lpos.size() / 3);
+      for (const vec3& w : p.worldPos) { lpos.push_back(w.x); lpos.push_back(w.y); lpos.push_back(w.z); }
+      d.mIdxOffset = dmesh[o.meshId].idxOffset;
+      d.mUvOffset = dmesh[o.meshId].uvOffset;
+      d.mCdfOffset = (int)lcdf.size();
+      lcdf.insert(lcdf.end(), p.cdf.begin(), p.cdf.end());
+      d.mNumTris = (int)(p.cdf.size() - 1);
+      d.area = p.area;
+      d.mNgSign = p.ngSign;
+    }
+    lights.upload(dl, stream);
+    lightPos.upload(lpos, stream);
+    lightCdf.upload(lcdf, stream);
+
+    std::vector<DevFlame> df;
+    for (const Flame& f : S.flames) {
+      DevFlame d{};
+      d.base[0] = f.base.x; d.base[1] = f.base.y; d.base[2] = f.base.z;
+      d.height = f.height;
+      d.radius = f.radius;
+      d.intensity = f.intensity;
+      d.sigma = f.sigma;
+      d.noiseScale = f.noiseScale;
+      d.seed = f.seed;
+      d.octaves = f.octaves;
+      d.step = f.step;
+      d.bounds = f.bounds;
+      df.push_back(d);
+    }
+    flames.upload(df, stream);
+
+    // ---- acceleration structures ----
+    std::vector<BvhNode4> allNodes;
+    std::vector<float4> triPool;
+    std::vector<uint32_t> blasRoot(prep.meshes.size(), 0);
+    std::vector<Aabb> meshBounds(prep.meshes.size());
+    std::vector<Bvh4> blases(prep.meshes.size());
+    int blasStack = 0;
+    std::vector<bool> meshUsed(prep.meshes.size(), false);
+    for (const Object& o : S.objects)
+      if (o.geomKind == GK_MESH) meshUsed[o.meshId] = true;
+    for (size_t mi = 0; mi < prep.meshes.size(); mi++) {
+      if (!meshUsed[mi]) continue;
+      const HostMesh& m = prep.meshes[mi];
+      std::vector<Aabb> boxes(m.numTris());
+      for (size_t t = 0; t < m.numTris(); t++) {
+        boxes[t].grow(m.positions[m.indices[3 * t]]);
+        boxes[t].grow(m.positions[m.indices[3 * t + 1]]);
+        boxes[t].grow(m.positions[m.indices[3 * t + 2]]);
+      }
+      blases[mi] = buildBvh4(boxes, 4);
+      meshBounds[mi] = blases[mi].bounds;
+      blasStack = std::max(blasStack, blases[mi].stackBound);
+    }
+
+    // top level over all objects
+    std::vector<Aabb> instBoxes(S.objects.size());
+    for (size_t i = 0; i < S.objects.size(); i++) {
+      const Object& o = S.objects[i];
+      Aabb ob = o.geomKind == GK_MESH ? meshBounds[o.meshId] : quadricBox(o.geomKind);
+      instBoxes[i] = transformBox(o.xform, ob);
+    }
+    int tlasStack = 0;
+    if (!instBoxes.empty()) {
+      Bvh4 tlas = buildBvh4(instBoxes, 1);
+      tlasStack = tlas.stackBound;
+      allNodes = tlas.nodes;
+      relocate(allNodes, 0, allNodes.size(), 0, 0, &tlas.primOrder);
+    } else {
+      BvhNode4 nd;
+      for (int i = 0; i < 4; i++) {
+        nd.lox[i] = nd.loy[i] = nd.loz[i] = nd.hix[i] = nd.hiy[i] = nd.hiz[i] = 3e38f;
+        nd.child[i] = (int32_t)kRefEmpty;
+        nd.count[i] = -1;
+      }
+      allNodes.push_back(nd);
+    }
+    tlasNodes = (int)allNodes.size();
+    for (size_t mi = 0; mi < prep.meshes.size(); mi++) {
+      if (!meshUsed[mi]) continue;
+      const Bvh4& b = blases[mi];
+      uint32_t nodeBase = (uint32_t)allNodes.size();
+      uint32_t primBase = (uint32_t)(triPool.size() / 3);
+      allNodes.insert(allNodes.end(), b.nodes.begin(), b.nodes.end());
+      relocate(allNodes, nodeBase, b.nodes.size(), nodeBase, primBase, nullptr);
+      blasRoot[mi] = nodeBase;
+      const HostMesh& m = prep.meshes[mi];
+      for (uint32_t t : b.primOrder) {
+        vec3 a = m.positions[m.indices[3 * t]], bb = m.positions[m.indices[3 * t + 1]],
+             c = m.positions[m.indices[3 * t + 2]];
+        float4 v0 = make_float4(a.x, a.y, a.z, 0.0f);
+        int32_t tid = (int32_t)t;
+        memcpy(&v0.w, &tid, 4);
+        triPool.push_back(v0);
+        triPool.push_back(make_float4(bb.x, bb.y, bb.z, 0.0f));
+        triPool.push_back(make_float4(c.x, c.y, c.z, 0.0f));
+      }
+    }
+    blasNodes = (int)allNodes.size() - tlasNodes;
+    stackNeed = tlasStack + 1 + blasStack;
+    spillRows = std::max(0, stackNeed - stackLdsEntries());
+
+    std::vector<DevInstance> di;
+    for (size_t i = 0; i < S.objects.size(); i++) {
+      const Object& o = S.objects[i];
+      DevInstance d{};
+      memcpy(d.inv, prep.inverses[i].m, sizeof d.inv);
+      d.geomKind = o.geomKind;
+      d.meshId = o.meshId;
+      d.blasRoot = o.geomKind == GK_MESH ? blasRoot[o.meshId] : 0;
+      d.mats = (uint32_t)o.matFront | ((uint32_t)o.matBack << 16);
+      d.lightId = o.lightId;
+      d.cutoutTex = o.cutoutTexId;
+      bool masked = o.matFront == kMatNone || o.matBack == kMatNone || o.cutoutTexId >= 0;
+      bool transmissive = materialTransmissive(S, o.matFront) || materialTransmissive(S, o.matBack);
+      d.flags = (masked ? INST_MASKED : 0) | (transmissive ? INST_TRANSMISSIVE : 0);
+      di.push_back(d);
+    }
+    numInstances = (int)di.size();
+    instances.upload(di, stream);
+    nodes.upload(allNodes, stream);
+    tris.upload(triPool, stream);
+    meshes.upload(dmesh, stream);
+    positions.upload(pos, stream);
+    indices.upload(idx, stream);
+    normals.upload(nrm, stream);
+    uvs.upload(uv, stream);
+
+    // environment
+    int envImportance = 0;
+    if (S.bgKind == BG_ENVMAP) {
+      std::vector<float> marg, cond;
+      bool ok = buildEnvCdf(prep.env, marg, cond);
+      if (!ok && S.env.importance)
+        fprintf(stderr, "envmap: %s has zero luminance, falling back to uniform sampling\n",
+                S.env.file.c_str());
+      envImportance = (S.env.importance && ok) ? 1 : 0;
+      std::vector<float4> et((size_t)prep.env.width * prep.env.height);
+      for (size_t k = 0; k < et.size(); k++)
+        et[k] = make_float4(prep.env.rgba[4 * k], prep.env.rgba[4 * k + 1], prep.env.rgba[4 * k + 2],
+                            prep.env.rgba[4 * k + 3]);
+      envTex.upload(et, stream);
+      margCdf.upload(marg, stream);
+      condCdf.upload(cond, stream);
+    }
+
+    // film
+    size_t npx = (size_t)S.render.width * S.render.height;
+    accum.alloc(npx * 16);
+    albedo.alloc(npx * 16);
+    normal.alloc(npx * 16);
+    rayCounter.alloc(8);
+    workCounters.alloc(32);
+
+    // launch parameters
+    LaunchParams& p = P;
+    p.width = S.render.width;
+    p.height = S.render.height;
+    p.sppTotal = S.render.spp;
+    p.maxDepth = S.render.maxDepth;
+    p.clampVal = S.render.clampVal;
+    p.seed = S.render.seed;
+    p.countRays = 0;
+    p.transparentShadows = S.render.transparentShadows ? 1 : 0;
+    p.numLights = (int)S.lights.size();
+    p.numFlames = (int)S.flames.size();
+    p.accum = accum.as<float4>();
+    p.albedo = albedo.as<float4>();
+    p.normal = normal.as<float4>();
+    p.rayCounter = rayCounter.as<unsigned long long>();
+    p.workCounters = workCounters.as<unsigned long long>();
+    Camera c = makeCamera(S.camera, S.render.width, S.render.height);
+    auto st3 = [](float* d, vec3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
+    st3(p.cam.origin, c.origin);
+    st3(p.cam.lowerLeft, c.lowerLeft);
+    st3(p.cam.horizontal, c.horizontal);
+    st3(p.cam.vertical, c.vertical);
+    st3(p.cam.u, c.u);
+    st3(p.cam.v, c.v);
+    st3(p.cam.w, c.w);
+    p.cam.lensRadius = c.lensRadius;
+    p.bgKind = S.bgKind;
+    st3(p.bgA, S.bgA);
+    st3(p.bgB, S.bgB);
+    p.hasEnv = S.bgKind == BG_ENVMAP ? 1 : 0;
+    p.envTex = envTex.as<float4>();
+    p.margCdf = margCdf.as<float>();
+    p.condCdf = condCdf.as<float>();
+    p.envW = prep.env.width;
+    p.envH = prep.env.height;
+    p.envRotate = S.env.rotateDeg * kPi / 180.0f;
+    p.envIntensity = S.env.intensity;
+    p.envImportance = envImportance;
+    p.textures = textures.as<DevTexture>();
+    p.texels = texels.as<float4>();
+    p.materials = materials.as<DevMaterial>();
+    p.lights = lights.as<DevLight>();
+    p.flames = flames.as<DevFlame>();
+    p.instances = instances.as<DevInstance>();
+    p.nodes = nodes.as<BvhNode4>();
+    p.tris = tris.as<float4>();
+    p.meshes = meshes.as<DevMesh>();
+    p.positions = positions.as<float>();
+    p.indices = indices.as<uint32_t>();
+    p.normals = normals.as<float>();
+    p.uvs = uvs.as<float>();
+    p.lightPos = lightPos.as<float>();
+    p.lightCdf = lightCdf.as<float>();
+    p.tlasRoot = 0;
+    reset();
+    HIP_OK(hipStreamSynchronize(stream));
+    buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  void reset() {
+    size_t npx = (size_t)P.width * P.height;
+    HIP_OK(hipMemsetAsync(accum.p, 0, npx * 16, stream));
+    HIP_OK(hipMemsetAsync(albedo.p, 0, npx * 16, stream));
+    HIP_OK(hipMemsetAsync(normal.p, 0, npx * 16, stream));
+    HIP_OK(hipMemsetAsync(rayCounter.p, 0, 8, stream));
+    HIP_OK(hipMemsetAsync(workCounters.p, 0, 32, stream));
+    renderMs = 0.0;
+    paths = 0;
+  }
+
+  void renderPass(int offset, int count) {
+    if (count <= 0) return;
+    if (offset < 0) throw std::runtime_error("render: negative sample offset");
+    size_t gx = (P.width + 15) / 16, gy = (P.height + 15) / 16;
+    ensureSpill(gx * gy * renderBlockThreads());
+    P.sampleOffset = offset;
+    P.sppThisLaunch = count;
+    HIP_OK(hipEventRecord(ev0, stream));
+    HIP_OK(launchRender(P, countWork, waves, stream));
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    renderMs += ms;
+    paths += (uint64_t)P.width * P.height * count;
+  }
+};

@@ -503,6 +503,9 @@ class Session(object):
     def count_rays(self, on=True):
         _lib.check(self.lib.sundog_session_set_count_rays(self.ptr, 1 if on else 0), self.lib)
 
+    def tune(self, key, value):
+        _lib.check(self.lib.sundog_session_set_tuning(self.ptr, key.encode(), int(value)), self.lib)
+
     def count_work(self, on=True):
         _lib.check(self.lib.sundog_session_set_count_work(self.ptr, 1 if on else 0), self.lib)
 

@@ -31,6 +31,18 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB per launch."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,10 +51,16 @@ def main():
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-tile-stride", type=int, default=16)
-    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--cpu-tile-stride", type=int, default=4)
+    ap.add_argument("--cpu-spp", type=int, default=192)
     ap.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB, 4 times "
+                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(the render path shards by pixel tiles, see DESIGN.md)" % args.gpus, file=sys.stderr)

@@ -1,0 +1,857 @@
+// Learned denoiser: a 5-level BF16 U-Net trained by this library's own MFMA
+// kernels. It stands in for the reference's vendor denoiser
+// (src/denoise.cpp:18-53: HDR beauty + albedo + normal guides in, denoised
+// HDR out).
+//
+// Network (channels 32/64/96/128/160, 3x3 convolutions, ReLU):
+//   in(9 -> padded 16) -> e0(32,32) -> pool -> e1(64,64) -> pool -> e2(96,96)
+//   -> pool -> e3(128,128) -> pool -> e4(160,160) -> up+cat(e3) -> d3(128,128)
+//   -> up+cat(e2) -> d2(96,96) -> up+cat(e1) -> d1(64,64) -> up+cat(e0)
+//   -> d0(32,32) -> out(3, padded 32), linear
+// Inputs: log(1+max(beauty,0)) RGB, albedo RGB, normal XYZ. Output: the
+// log(1+x) of the denoised radiance. Loss: L1 in that log-HDR space.
+//
+// Activations are NHWC BF16, gradients NHWC FP32, master weights FP32 with
+// Adam; BF16 weight copies are kept in two layouts, [co][tap][ci] for the
+// forward pass and [ci][tap][co] for backward-data, so every MFMA operand a
+// lane needs is 16 contiguous bytes.
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (FP32
+// accumulate): forward M = pixels, N = Cout, K = 9*Cin; backward-data M =
+// pixels, N = Cin, K = 9*Cout; backward-weights M = Cout, N = Cin, K = pixels
+// per tap, split over pixel chunks whose FP32 partials are reduced in a fixed
+// order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sundog_api.h"
+#include "session.h"
+
+namespace sdn {
+
+using sdk::DevBuf;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__host__ __device__ inline float bf2f(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);  // inf / nan
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ inline bf16x8 asBf(u16x8 v) { return __builtin_bit_cast(bf16x8, v); }
+
+constexpr int kTile = 32;  // MFMA 32x32x16
+
+// ---------------------------------------------------------------------------
+// forward / backward-data convolution (implicit GEMM, one wave = 32x32 tile)
+// ---------------------------------------------------------------------------
+// in: [P][Cin] (BF16, or FP32 when kInF32), w: [Cout][9][Cin] BF16.
+// out[p][co] = bias[co] + sum_t sum_ci in[p + tapOff(t)][ci] * w[co][t][ci];
+// tapSign = +1 for forward, -1 for backward-data (the transposed conv reads
+// the neighbour on the opposite side). Cin % 16 == 0, Cout % 32 == 0.
+template <bool kInF32, bool kOutF32>
+__global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, const uint16_t* __restrict__ w,
+                                              const float* __restrict__ bias, void* __restrict__ outV, int N,
+                                              int H, int W, int Cin, int Cout, int tapSign, int relu) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int P = N * H * W;
+  const int m0 = (blockIdx.x * 4 + wave) * kTile;
+  const int co0 = blockIdx.y * kTile;
+  if (m0 >= P) return;
+  const int row = lane & 31, half = lane >> 5;
+  const int p = m0 + row;
+  const bool pv = p < P;
+  const int n = pv ? p / (H * W) : 0;
+  const int yx = pv ? p - n * H * W : 0;
+  const int y = yx / W, x = yx - (yx / W) * W;
+  f32x16 acc = {};
+  const uint16_t* wrow = w + (size_t)(co0 + row) * 9 * Cin;
+  for (int t = 0; t < 9; t++) {
+    const int sy = y + tapSign * (t / 3 - 1), sx = x + tapSign * (t % 3 - 1);
+    const bool valid = pv && sy >= 0 && sy < H && sx >= 0 && sx < W;
+    const size_t src = ((size_t)n * H + (valid ? sy : 0)) * W + (valid ? sx : 0);
+    for (int c = 0; c < Cin; c += 16) {
+      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (valid) {
+        if (kInF32) {
+          const float* ip = (const float*)inV + src * Cin + c + 8 * half;
+          float4 f0 = *(const float4*)ip, f1 = *(const float4*)(ip + 4);
+          a = u16x8{f2bf(f0.x), f2bf(f0.y), f2bf(f0.z), f2bf(f0.w), f2bf(f1.x), f2bf(f1.y), f2bf(f1.z), f2bf(f1.w)};
+        } else {
+          a = *(const u16x8*)((const uint16_t*)inV + src * Cin + c + 8 * half);
+        }
+      }
+      u16x8 b = *(const u16x8*)(wrow + (size_t)t * Cin + c + 8 * half);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc, 0, 0, 0);
+    }
+  }
+  const int co = co0 + row;  // D column = lane % 32
+  const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    int i = 8 * (r / 4) + 4 * half + (r % 4);
+    int pp = m0 + i;
+    if (pp < P) {
+      float v = acc[r] + bv;
+      if (relu) v = fmaxf(v, 0.0f);
+      if (kOutF32) ((float*)outV)[(size_t)pp * Cout + co] = v;
+      else ((uint16_t*)outV)[(size_t)pp * Cout + co] = f2bf(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward-weights: partial[k][co][t][ci] = sum over pixel chunk k of
+// dy[p][co] * x[p + off(t)][ci]; one wave per (co tile, ci tile, tap, chunk)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_conv_dw(const float* __restrict__ dy, const uint16_t* __restrict__ x,
+                                                float* __restrict__ partial, int N, int H, int W, int Cin,
+                                                int Cout, int chunk) {
+  const int lane = threadIdx.x;
+  const int co0 = blockIdx.x * kTile;
+  const int ciTiles = (Cin + kTile - 1) / kTile;
+  const int ci0 = (blockIdx.y % ciTiles) * kTile;
+  const int t = blockIdx.y / ciTiles;
+  const int k = blockIdx.z;
+  const int P = N * H * W;
+  const int row = lane & 31, half = lane >> 5;
+  const int dy_ = t / 3 - 1, dx_ = t % 3 - 1;
+  const int pBeg = k * chunk, pEnd = min(P, pBeg + chunk);
+  const int co = co0 + row, ci = ci0 + row;
+  const bool ciOk = ci < Cin;
+  f32x16 acc = {};
+  for (int q0 = pBeg; q0 < pEnd; q0 += 16) {
+    u16x8 a, b;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      int q = q0 + 8 * half + j;
+      float av = 0.0f;
+      uint16_t bv = 0;
+      if (q < pEnd) {
+        av = dy[(size_t)q * Cout + co];
+        int n = q / (H * W), yx = q - n * H * W;
+        int y = yx / W, xx = yx - (yx / W) * W;
+        int sy = y + dy_, sx = xx + dx_;
+        if (ciOk && sy >= 0 && sy < H && sx >= 0 && sx < W) bv = x[(((size_t)n * H + sy) * W + sx) * Cin + ci];
+      }
+      a[j] = f2bf(av);
+      b[j] = bv;
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc, 0, 0, 0);
+  }
+  // D[i = co offset][j = ci offset]; lane holds column j = row
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    int i = 8 * (r / 4) + 4 * half + (r % 4);
+    if (ciOk)
+      partial[(((size_t)k * Cout + co0 + i) * 9 + t) * Cin + ci] = acc[r];
+  }
+}
+
+// dW[e] = sum_k partial[k][e] in chunk order
+__global__ void k_reduce_chunks(const float* __restrict__ partial, float* __restrict__ out, int chunks, size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < chunks; k++) s += partial[(size_t)k * n + e];
+  out[e] = s;
+}
+
+// db[co] = sum_p dy[p][co], fixed order: per-block partial sums then reduce
+__global__ void k_bias_grad(const float* __restrict__ dy, float* __restrict__ partial, int P, int C, int rowsPerBlock) {
+  int co = threadIdx.x;
+  if (co >= C) return;
+  int p0 = blockIdx.x * rowsPerBlock, p1 = min(P, p0 + rowsPerBlock);
+  float s = 0.0f;
+  for (int p = p0; p < p1; p++) s += dy[(size_t)p * C + co];
+  partial[(size_t)blockIdx.x * C + co] = s;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise / resampling kernels
+// ---------------------------------------------------------------------------
+__global__ void k_relu_bwd(float* __restrict__ g, const uint16_t* __restrict__ y, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(bf2f(y[i]) > 0.0f)) g[i] = 0.0f;
+}
+
+__global__ void k_pool_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int N, int H, int W, int C) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int Ho = H / 2, Wo = W / 2;
+  size_t total = (size_t)N * Ho * Wo * C;
+  if (i >= total) return;
+  int c = i % C;
+  size_t p = i / C;
+  int xo = p % Wo, yo = (p / Wo) % Ho, n = p / ((size_t)Wo * Ho);
+  size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+  float s = bf2f(in[b]) + bf2f(in[b + C]) + bf2f(in[b + (size_t)W * C]) + bf2f(in[b + (size_t)W * C + C]);
+  out[i] = f2bf(0.25f * s);
+}
+
+// dIn[y][x] = 0.25 * dOut[y/2][x/2]  (accumulates into dIn)
+__global__ void k_pool_bwd(const float* __restrict__ gout, float* __restrict__ gin, int N, int H, int W, int C) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)N * H * W * C;
+  if (i >= total) return;
+  int c = i % C;
+  size_t p = i / C;
+  int x = p % W, y = (p / W) % H, n = p / ((size_t)W * H);
+  gin[i] += 0.25f * gout[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// out = concat(up2(a) [Ca channels], skip [Cb channels]) at the skip's size
+__global__ void k_upcat_fwd(const uint16_t* __restrict__ a, const uint16_t* __restrict__ skip,
+                            uint16_t* __restrict__ out, int N, int H, int W, int Ca, int Cb) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int C = Ca + Cb;
+  size_t total = (size_t)N * H * W * C;
+  if (i >= total) return;
+  int c = i % C;
+  size_t p = i / C;
+  int x = p % W, y = (p / W) % H, n = p / ((size_t)W * H);
+  if (c < Ca) out[i] = a[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c];
+  else out[i] = skip[p * Cb + (c - Ca)];
+}
+
+// gA[y/2][x/2] += sum over the 4 children (gather form, no atomics);
+// gSkip[p] += the skip part
+__global__ void k_upcat_bwd_a(const float* __restrict__ g, float* __restrict__ ga, int N, int H, int W, int Ca,
+                              int Cb) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int Ho = H / 2, Wo = W / 2, C = Ca + Cb;
+  size_t total = (size_t)N * Ho * Wo * Ca;
+  if (i >= total) return;
+  int c = i % Ca;
+  size_t p = i / Ca;
+  int xo = p % Wo, yo = (p / Wo) % Ho, n = p / ((size_t)Wo * Ho);
+  size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+  ga[i] += g[b] + g[b + C] + g[b + (size_t)W * C] + g[b + (size_t)W * C + C];
+}
+__global__ void k_upcat_bwd_skip(const float* __restrict__ g, float* __restrict__ gs, int N, int H, int W, int Ca,
+                                 int Cb) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)N * H * W * Cb;
+  if (i >= total) return;
+  int c = i % Cb;
+  size_t p = i / Cb;
+  gs[i] += g[p * (Ca + Cb) + Ca + c];
+}
+
+// network input: log1p(beauty), albedo, normal, zero pad to 16 channels
+__global__ void k_pack_input(const float* __restrict__ src, uint16_t* __restrict__ dst, size_t P) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * 16) return;
+  size_t p = i / 16;
+  int c = i % 16;
+  float v = 0.0f;
+  if (c < 3) v = log1pf(fmaxf(src[p * 9 + c], 0.0f));
+  else if (c < 9) v = src[p * 9 + c];
+  dst[i] = f2bf(v);
+}
+
+// L1 loss in log space on the first 3 channels of pred [P][32]; writes the
+// gradient (sign / count) and per-block partial loss sums
+__global__ void k_l1(const float* __restrict__ pred, const float* __restrict__ target, float* __restrict__ grad,
+                     float* __restrict__ partial, size_t P) {
+  __shared__ float s[256];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < P * 32) {
+    size_t p = i / 32;
+    int c = i % 32;
+    float g = 0.0f;
+    if (c < 3) {
+      float d = pred[i] - log1pf(fmaxf(target[p * 3 + c], 0.0f));
+      l = fabsf(d);
+      g = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) / (float)(P * 3);
+    }
+    grad[i] = g;
+  }
+  s[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ void k_unpack_output(const float* __restrict__ pred, float* __restrict__ out, size_t P) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * 3) return;
+  out[i] = expm1f(pred[(i / 3) * 32 + (i % 3)]);
+}
+
+// Adam on FP32 masters, then the two BF16 layouts. Weights of one conv are
+// [co][t][ci] (Cin padded), followed by bias[co].
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, size_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.0f - b1) * gi;
+  float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_to_bf16_layouts(const float* __restrict__ w, uint16_t* __restrict__ wf, uint16_t* __restrict__ wt,
+                                  int Cin, int Cout) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)Cout * 9 * Cin;
+  if (i >= n) return;
+  int ci = i % Cin;
+  int t = (i / Cin) % 9;
+  int co = i / ((size_t)9 * Cin);
+  uint16_t b = f2bf(w[i]);
+  wf[i] = b;
+  wt[((size_t)ci * 9 + t) * Cout + co] = b;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+#define DN_OK(expr) HIP_OK(expr)
+
+static inline unsigned blocks(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+struct Conv {
+  int cin, cout;  // cin padded to a multiple of 16, cout to a multiple of 32
+  bool relu;
+  size_t wOff, bOff;  // into the flat parameter vector
+  DevBuf wf, wt;      // BF16 layouts
+};
+
+struct Net {
+  std::vector<Conv> convs;
+  size_t nParams = 0;
+  DevBuf params, grads, adamM, adamV;
+  int step = 0;
+
+  void addConv(int cin, int cout, bool relu) {
+    Conv c;
+    c.cin = cin;
+    c.cout = cout;
+    c.relu = relu;
+    c.wOff = nParams;
+    nParams += (size_t)cout * 9 * cin;
+    c.bOff = nParams;
+    nParams += cout;
+    convs.push_back(std::move(c));
+  }
+};
+
+// channels per level and the conv list in execution order
+static const int kCh[5] = {32, 64, 96, 128, 160};
+
+}  // namespace sdn
+
+using namespace sdn;
+
+struct sundog_denoiser {
+  Net net;
+  hipStream_t stream = nullptr;
+  // per-resolution scratch
+  int N = 0, H = 0, W = 0;
+  std::vector<DevBuf> y;           // conv outputs (BF16)
+  std::vector<int> yH, yW;         // spatial size per conv
+  DevBuf input, pooled[4], cat[4], pred;
+  std::vector<DevBuf> gy;          // gradients w.r.t. conv outputs (FP32)
+  DevBuf gPool[4], gCat[4], gInput, gPred, partial, lossPartial, target, devIn;
+  size_t partialCap = 0;
+
+  ~sundog_denoiser() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void init(uint64_t seed) {
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    Net& n = net;
+    int prev = 16;
+    for (int l = 0; l < 5; l++) {
+      n.addConv(prev, kCh[l], true);
+      n.addConv(kCh[l], kCh[l], true);
+      prev = kCh[l];
+    }
+    for (int l = 3; l >= 0; l--) {
+      n.addConv(kCh[l + 1] + kCh[l], kCh[l], true);
+      n.addConv(kCh[l], kCh[l], true);
+    }
+    n.addConv(kCh[0], 32, false);  // output: first 3 channels used
+    // He-normal initialisation from a PCG32 stream (Box-Muller)
+    std::vector<float> p(n.nParams, 0.0f);
+    uint64_t state = 0, inc = (54u << 1) | 1u;
+    auto next = [&]() {
+      uint64_t old = state;
+      state = old * 6364136223846793005ULL + inc;
+      uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+      return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    };
+    next();
+    state += seed;
+    next();
+    for (size_t ci = 0; ci < n.convs.size(); ci++) {
+      const Conv& c = n.convs[ci];
+      float stdv = sqrtf(2.0f / (9.0f * c.cin));
+      if (ci == n.convs.size() - 1) stdv *= 0.1f;
+      for (size_t k = 0; k < (size_t)c.cout * 9 * c.cin; k += 2) {
+        float u1 = ((next() >> 8) + 1) * 0x1p-24f, u2 = (next() >> 8) * 0x1p-24f;
+        float r = sqrtf(-2.0f * logf(u1));
+        p[c.wOff + k] = stdv * r * cosf(6.2831853f * u2);
+        if (k + 1 < (size_t)c.cout * 9 * c.cin) p[c.wOff + k + 1] = stdv * r * sinf(6.2831853f * u2);
+      }
+    }
+    n.params.upload(p, stream);
+    n.grads.alloc(n.nParams * 4);
+    n.adamM.alloc(n.nParams * 4);
+    n.adamV.alloc(n.nParams * 4);
+    HIP_OK(hipMemsetAsync(n.adamM.p, 0, n.nParams * 4, stream));
+    HIP_OK(hipMemsetAsync(n.adamV.p, 0, n.nParams * 4, stream));
+    for (Conv& c : n.convs) {
+      c.wf.alloc((size_t)c.cout * 9 * c.cin * 2);
+      c.wt.alloc((size_t)c.cout * 9 * c.cin * 2);
+    }
+    syncWeights();
+    HIP_OK(hipStreamSynchronize(stream));
+  }
+
+  void syncWeights() {
+    for (Conv& c : net.convs) {
+      size_t n = (size_t)c.cout * 9 * c.cin;
+      hipLaunchKernelGGL(k_to_bf16_layouts, dim3(blocks(n)), dim3(256), 0, stream,
+                         net.params.as<float>() + c.wOff, c.wf.as<uint16_t>(), c.wt.as<uint16_t>(), c.cin, c.cout);
+    }
+    HIP_OK(hipGetLastError());
+  }
+
+  void resize(int n, int h, int w) {
+    if (n == N && h == H && w == W) return;
+    if (h % 16 || w % 16) throw std::runtime_error("denoiser: height and width must be multiples of 16");
+    N = n;
+    H = h;
+    W = w;
+    size_t P0 = (size_t)n * h * w;
+    size_t nc = net.convs.size();
+    y.clear();
+    y.resize(nc);
+    gy.clear();
+    gy.resize(nc);
+    yH.assign(nc, 0);
+    yW.assign(nc, 0);
+    int ci = 0;
+    for (int l = 0; l < 5; l++)
+      for (int k = 0; k < 2; k++, ci++) {
+        yH[ci] = h >> l;
+        yW[ci] = w >> l;
+      }
+    for (int l = 3; l >= 0; l--)
+      for (int k = 0; k < 2; k++, ci++) {
+        yH[ci] = h >> l;
+        yW[ci] = w >> l;
+      }
+    yH[ci] = h;
+    yW[ci] = w;
+    for (size_t k = 0; k < nc; k++) {
+      size_t px = (size_t)n * yH[k] * yW[k];
+      if (k + 1 < nc) y[k].alloc(px * net.convs[k].cout * 2);
+      gy[k].alloc(px * net.convs[k].cout * 4);
+    }
+    input.alloc(P0 * 16 * 2);
+    gInput.alloc(P0 * 16 * 4);
+    pred.alloc(P0 * 32 * 4);
+    for (int l = 0; l < 4; l++) {
+      size_t pl = (size_t)n * (h >> (l + 1)) * (w >> (l + 1));
+      pooled[l].alloc(pl * kCh[l] * 2);
+      gPool[l].alloc(pl * kCh[l] * 4);
+      size_t pc = (size_t)n * (h >> l) * (w >> l);
+      cat[l].alloc(pc * (kCh[l + 1] + kCh[l]) * 2);
+      gCat[l].alloc(pc * (kCh[l + 1] + kCh[l]) * 4);
+    }
+    target.alloc(P0 * 3 * 4);
+    devIn.alloc(P0 * 9 * 4);
+    lossPartial.alloc(blocks(P0 * 32) * 4);
+  }
+
+  void conv(int k, const void* in, bool inF32, void* out, bool outF32, int h, int w) {
+    const Conv& c = net.convs[k];
+    int P = N * h * w;
+    dim3 grid((P + 127) / 128, c.cout / 32);
+    const float* b = net.params.as<float>() + c.bOff;
+    if (outF32)
+      hipLaunchKernelGGL((k_conv<false, true>), grid, dim3(256), 0, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w,
+                         c.cin, c.cout, 1, c.relu ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_conv<false, false>), grid, dim3(256), 0, stream, in, c.wf.as<uint16_t>(), b, out, N, h,
+                         w, c.cin, c.cout, 1, c.relu ? 1 : 0);
+    (void)inF32;
+    HIP_OK(hipGetLastError());
+  }
+
+  // forward; in = packed BF16 input (already in `input`)
+  void forward() {
+    const uint16_t* cur = input.as<uint16_t>();
+    int k = 0;
+    for (int l = 0; l < 5; l++) {
+      int h = H >> l, w = W >> l;
+      if (l > 0) {
+        size_t tot = (size_t)N * h * w * kCh[l - 1];
+        hipLaunchKernelGGL(k_pool_fwd, dim3(blocks(tot)), dim3(256), 0, stream, y[k - 1].as<uint16_t>(),
+                           pooled[l - 1].as<uint16_t>(), N, h * 2, w * 2, kCh[l - 1]);
+        cur = pooled[l - 1].as<uint16_t>();
+      }
+      conv(k, cur, false, y[k].p, false, h, w);
+      conv(k + 1, y[k].p, false, y[k + 1].p, false, h, w);
+      cur = y[k + 1].as<uint16_t>();
+      k += 2;
+    }
+    for (int l = 3; l >= 0; l--) {
+      int h = H >> l, w = W >> l;
+      int skipConv = 2 * l + 1;
+      size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]);
+      hipLaunchKernelGGL(k_upcat_fwd, dim3(blocks(tot)), dim3(256), 0, stream, cur, y[skipConv].as<uint16_t>(),
+                         cat[l].as<uint16_t>(), N, h, w, kCh[l + 1], kCh[l]);
+      conv(k, cat[l].p, false, y[k].p, false, h, w);
+      conv(k + 1, y[k].p, false, y[k + 1].p, false, h, w);
+      cur = y[k + 1].as<uint16_t>();
+      k += 2;
+    }
+    conv(k, cur, false, pred.p, true, H, W);
+    HIP_OK(hipGetLastError());
+  }
+
+  // dW, db for conv k given dy (FP32, already relu-masked) and its input x
+  void convGradW(int k, const float* dy, const uint16_t* x, int h, int w) {
+    const Conv& c = net.convs[k];
+    int P = N * h * w;
+    int chunk = 4096;
+    int chunks = (P + chunk - 1) / chunk;
+    size_t nW = (size_t)c.cout * 9 * c.cin;
+    size_t need = std::max((size_t)chunks * nW, (size_t)((P + 255) / 256) * c.cout) * 4;
+    if (need > partialCap) {
+      partial.alloc(need);
+      partialCap = need;
+    }
+    int ciTiles = (c.cin + 31) / 32;
+    dim3 grid(c.cout / 32, ciTiles * 9, chunks);
+    hipLaunchKernelGGL(k_conv_dw, grid, dim3(64), 0, stream, dy, x, partial.as<float>(), N, h, w, c.cin, c.cout,
+                       chunk);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(nW)), dim3(256), 0, stream, partial.as<float>(),
+                       net.grads.as<float>() + c.wOff, chunks, nW);
+    int rows = 256;
+    int nb = (P + rows - 1) / rows;
+    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(std::max(64, c.cout)), 0, stream, dy, partial.as<float>(), P,
+                       c.cout, rows);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(c.cout)), dim3(256), 0, stream, partial.as<float>(),
+                       net.grads.as<float>() + c.bOff, nb, (size_t)c.cout);
+    HIP_OK(hipGetLastError());
+  }
+
+  // dx = conv^T(dy); dx may be null (network input)
+  void convGradX(int k, const float* dy, float* dx, int h, int w) {
+    const Conv& c = net.convs[k];
+    int P = N * h * w;
+    dim3 grid((P + 127) / 128, c.cin / 32 > 0 ? c.cin / 32 : 1);
+    if (c.cin % 32) throw std::runtime_error("denoiser: backward-data needs Cin % 32 == 0");
+    hipLaunchKernelGGL((k_conv<true, true>), grid, dim3(256), 0, stream, dy, c.wt.as<uint16_t>(), nullptr, dx, N,
+                       h, w, c.cout, c.cin, -1, 0);
+    HIP_OK(hipGetLastError());
+  }
+
+  void reluBwd(int k, int h, int w) {
+    size_t n = (size_t)N * h * w * net.convs[k].cout;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(n)), dim3(256), 0, stream, gy[k].as<float>(), y[k].as<uint16_t>(), n);
+  }
+
+  // backward from gPred (FP32 [P][32]) to parameter gradients
+  void backward() {
+    int nc = (int)net.convs.size();
+    int k = nc - 1;
+    // output conv (linear)
+    const uint16_t* dIn = y[k - 1].as<uint16_t>();
+    convGradW(k, gPred.as<float>(), dIn, H, W);
+    convGradX(k, gPred.as<float>(), gy[k - 1].as<float>(), H, W);
+    k--;
+    for (int l = 0; l <= 3; l++) {
+      int h = H >> l, w = W >> l;
+      // second conv of decoder level l: k
+      reluBwd(k, h, w);
+      convGradW(k, gy[k].as<float>(), y[k - 1].as<uint16_t>(), h, w);
+      convGradX(k, gy[k].as<float>(), gy[k - 1].as<float>(), h, w);
+      k--;
+      reluBwd(k, h, w);
+      convGradW(k, gy[k].as<float>(), cat[l].as<uint16_t>(), h, w);
+      convGradX(k, gy[k].as<float>(), gCat[l].as<float>(), h, w);
+      k--;
+      // split the concat gradient: upsampled part -> producer of the lower
+      // level (conv k), skip part -> encoder output 2l+1
+      int ca = kCh[l + 1], cb = kCh[l];
+      size_t na = (size_t)N * (h / 2) * (w / 2) * ca;
+      HIP_OK(hipMemsetAsync(gy[k].p, 0, na * 4, stream));
+      hipLaunchKernelGGL(k_upcat_bwd_a, dim3(blocks(na)), dim3(256), 0, stream, gCat[l].as<float>(),
+                         gy[k].as<float>(), N, h, w, ca, cb);
+      // skip gradient accumulates into the encoder output's gradient, which
+      // is zeroed before the decoder pass starts (see trainStep)
+      size_t ns = (size_t)N * h * w * cb;
+      hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(blocks(ns)), dim3(256), 0, stream, gCat[l].as<float>(),
+                         gy[2 * l + 1].as<float>(), N, h, w, ca, cb);
+    }
+    // encoder, deepest first; k == 9 (second conv of level 4)
+    for (int l = 4; l >= 0; l--) {
+      int h = H >> l, w = W >> l;
+      reluBwd(k, h, w);
+      convGradW(k, gy[k].as<float>(), y[k - 1].as<uint16_t>(), h, w);
+      convGradX(k, gy[k].as<float>(), gy[k - 1].as<float>(), h, w);
+      k--;
+      reluBwd(k, h, w);
+      const uint16_t* xin = l == 0 ? input.as<uint16_t>() : pooled[l - 1].as<uint16_t>();
+      convGradW(k, gy[k].as<float>(), xin, h, w);
+      if (l > 0) {
+        convGradX(k, gy[k].as<float>(), gPool[l - 1].as<float>(), h, w);
+        // pool backward accumulates into the previous encoder output's gradient
+        size_t nb = (size_t)N * (h * 2) * (w * 2) * kCh[l - 1];
+        hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(nb)), dim3(256), 0, stream, gPool[l - 1].as<float>(),
+                           gy[k - 1].as<float>(), N, h * 2, w * 2, kCh[l - 1]);
+      }
+      k--;
+    }
+    HIP_OK(hipGetLastError());
+  }
+
+  void uploadBatch(const float* in, const float* tgt, int n, int h, int w) {
+    resize(n, h, w);
+    size_t P = (size_t)n * h * w;
+    HIP_OK(hipMemcpyAsync(devIn.p, in, P * 9 * 4, hipMemcpyHostToDevice, stream));
+    if (tgt) HIP_OK(hipMemcpyAsync(target.p, tgt, P * 3 * 4, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks(P * 16)), dim3(256), 0, stream, devIn.as<float>(),
+                       input.as<uint16_t>(), P);
+    HIP_OK(hipGetLastError());
+  }
+
+  float lossAndGrad() {
+    size_t P = (size_t)N * H * W;
+    gPred.alloc(P * 32 * 4);
+    unsigned nb = blocks(P * 32);
+    hipLaunchKernelGGL(k_l1, dim3(nb), dim3(256), 0, stream, pred.as<float>(), target.as<float>(),
+                       gPred.as<float>(), lossPartial.as<float>(), P);
+    std::vector<float> lp(nb);
+    HIP_OK(hipMemcpyAsync(lp.data(), lossPartial.p, nb * 4, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (float v : lp) s += v;
+    return (float)(s / (double)(P * 3));
+  }
+
+  void zeroEncoderSkipGrads() {
+    // encoder outputs receive their gradient from two places (pool of the
+    // next level and the decoder skip); zero them before accumulating
+    for (int l = 0; l < 4; l++) {
+      int k = 2 * l + 1;
+      size_t n = (size_t)N * (H >> l) * (W >> l) * kCh[l];
+      HIP_OK(hipMemsetAsync(gy[k].p, 0, n * 4, stream));
+    }
+  }
+
+  float forwardBackward() {
+    forward();
+    float loss = lossAndGrad();
+    zeroEncoderSkipGrads();
+    backward();
+    return loss;
+  }
+
+  void adam(float lr) {
+    net.step++;
+    float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.0f - powf(b1, (float)net.step), c2 = 1.0f - powf(b2, (float)net.step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(net.nParams)), dim3(256), 0, stream, net.params.as<float>(),
+                       net.adamM.as<float>(), net.adamV.as<float>(), net.grads.as<float>(), net.nParams, lr, b1, b2,
+                       1e-8f, c1, c2);
+    syncWeights();
+  }
+};
+
+namespace {
+template <class F>
+int guardedD(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    sdk::setLastError(e.what());
+    return SUNDOG_ERROR;
+  }
+}
+}  // namespace
+
+extern "C" sundog_denoiser* sundog_denoiser_create(uint64_t seed) {
+  try {
+    std::unique_ptr<sundog_denoiser> d(new sundog_denoiser());
+    d->init(seed);
+    return d.release();
+  } catch (const std::exception& e) {
+    sdk::setLastError(e.what());
+    return nullptr;
+  }
+}
+
+extern "C" void sundog_denoiser_destroy(sundog_denoiser* d) { delete d; }
+
+extern "C" int64_t sundog_denoiser_param_count(sundog_denoiser* d) { return d ? (int64_t)d->net.nParams : -1; }
+
+extern "C" int sundog_denoiser_get_params(sundog_denoiser* d, float* out) {
+  return guardedD([&] {
+    if (!d || !out) throw std::runtime_error("denoiser: null argument");
+    HIP_OK(hipStreamSynchronize(d->stream));
+    HIP_OK(hipMemcpy(out, d->net.params.p, d->net.nParams * 4, hipMemcpyDeviceToHost));
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_set_params(sundog_denoiser* d, const float* in) {
+  return guardedD([&] {
+    if (!d || !in) throw std::runtime_error("denoiser: null argument");
+    HIP_OK(hipMemcpyAsync(d->net.params.p, in, d->net.nParams * 4, hipMemcpyHostToDevice, d->stream));
+    d->syncWeights();
+    HIP_OK(hipStreamSynchronize(d->stream));
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_forward_backward(sundog_denoiser* d, const float* input, const float* target, int n,
+                                                int h, int w, float* loss, float* grads, float* pred_out) {
+  return guardedD([&] {
+    if (!d || !input || !target) throw std::runtime_error("denoiser: null argument");
+    d->uploadBatch(input, target, n, h, w);
+    float l = d->forwardBackward();
+    if (loss) *loss = l;
+    if (grads) {
+      HIP_OK(hipStreamSynchronize(d->stream));
+      HIP_OK(hipMemcpy(grads, d->net.grads.p, d->net.nParams * 4, hipMemcpyDeviceToHost));
+    }
+    if (pred_out) {
+      size_t P = (size_t)n * h * w;
+      std::vector<float> p(P * 32);
+      HIP_OK(hipStreamSynchronize(d->stream));
+      HIP_OK(hipMemcpy(p.data(), d->pred.p, P * 32 * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < P; i++)
+        for (int c = 0; c < 3; c++) pred_out[i * 3 + c] = p[i * 32 + c];
+    }
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_train_step(sundog_denoiser* d, const float* input, const float* target, int n, int h,
+                                          int w, float lr, float* loss) {
+  return guardedD([&] {
+    if (!d || !input || !target) throw std::runtime_error("denoiser: null argument");
+    d->uploadBatch(input, target, n, h, w);
+    float l = d->forwardBackward();
+    d->adam(lr);
+    if (loss) *loss = l;
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_infer(sundog_denoiser* d, const float* input, int n, int h, int w, float* out) {
+  return guardedD([&] {
+    if (!d || !input || !out) throw std::runtime_error("denoiser: null argument");
+    d->uploadBatch(input, nullptr, n, h, w);
+    d->forward();
+    size_t P = (size_t)n * h * w;
+    DevBuf o;
+    o.alloc(P * 3 * 4);
+    hipLaunchKernelGGL(k_unpack_output, dim3(blocks(P * 3)), dim3(256), 0, d->stream, d->pred.as<float>(),
+                       o.as<float>(), P);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, o.p, P * 3 * 4, hipMemcpyDeviceToHost, d->stream));
+    HIP_OK(hipStreamSynchronize(d->stream));
+    return SUNDOG_OK;
+  });
+}
+
+// Timed forward pass at the current resolution (for MFMA utilisation); ms
+// of `iters` forwards measured with HIP events on the denoiser's stream.
+extern "C" int sundog_denoiser_time_forward(sundog_denoiser* d, int n, int h, int w, int iters, double* ms_out,
+                                            double* flops_out) {
+  return guardedD([&] {
+    if (!d) throw std::runtime_error("denoiser: null handle");
+    d->resize(n, h, w);
+    HIP_OK(hipMemsetAsync(d->input.p, 0, (size_t)n * h * w * 16 * 2, d->stream));
+    d->forward();
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, d->stream));
+    for (int i = 0; i < iters; i++) d->forward();
+    HIP_OK(hipEventRecord(e1, d->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    double fl = 0;
+    for (size_t k = 0; k < d->net.convs.size(); k++) {
+      const Conv& c = d->net.convs[k];
+      fl += 2.0 * 9.0 * c.cin * c.cout * (double)n * d->yH[k] * d->yW[k];
+    }
+    if (ms_out) *ms_out = ms / iters;
+    if (flops_out) *flops_out = fl;
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_save(sundog_denoiser* d, const char* path) {
+  return guardedD([&] {
+    if (!d || !path) throw std::runtime_error("denoiser: null argument");
+    std::vector<float> p(d->net.nParams);
+    HIP_OK(hipStreamSynchronize(d->stream));
+    HIP_OK(hipMemcpy(p.data(), d->net.params.p, p.size() * 4, hipMemcpyDeviceToHost));
+    FILE* f = fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("denoiser: cannot write ") + path);
+    const char magic[8] = {'S', 'D', 'U', 'N', 'E', 'T', '0', '1'};
+    uint64_t n = p.size();
+    fwrite(magic, 1, 8, f);
+    fwrite(&n, 8, 1, f);
+    fwrite(p.data(), 4, p.size(), f);
+    fclose(f);
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_denoiser_load(sundog_denoiser* d, const char* path) {
+  return guardedD([&] {
+    if (!d || !path) throw std::runtime_error("denoiser: null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("denoiser: cannot read ") + path);
+    char magic[8];
+    uint64_t n = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "SDUNET01", 8) == 0 && fread(&n, 8, 1, f) == 1 &&
+              n == d->net.nParams;
+    std::vector<float> p(ok ? n : 0);
+    ok = ok && fread(p.data(), 4, n, f) == n;
+    fclose(f);
+    if (!ok) throw std::runtime_error(std::string("denoiser: bad weights file ") + path);
+    return sundog_denoiser_set_params(d, p.data());
+  });
+}

@@ -922,9 +922,11 @@ DEV int quadricRoots(int kind, vec3 o, vec3 d, float tmin, float tmax, float t[2
   int n = 0;
   switch (kind) {
     case GK_SPHERE: {
-      float a = dot(d, d);
-      float b = dot(o, d);
-      float c = dot(o, o) - 1.0f;
+      // products written out here: the pragma above governs only
+      // expressions parsed in this function, not inlined helpers like dot()
+      float a = d.x * d.x + d.y * d.y + d.z * d.z;
+      float b = o.x * d.x + o.y * d.y + o.z * d.z;
+      float c = (o.x * o.x + o.y * o.y + o.z * o.z) - 1.0f;
       float disc = b * b - a * c;
       if (disc <= 0.0f) return 0;
       float s = sqrtf(disc);
@@ -1141,6 +1143,13 @@ struct WorkCount {
   uint32_t nodes = 0, tris = 0, quads = 0, insts = 0;
 };
 
+// Upper t bound for primitive candidates: a hair above the current hit so
+// that exact ties still reach the (t, instance, primitive) tie rule.
+DEV float candBound(float t) {
+#pragma clang fp contract(off)
+  return t * 1.0000004f + 1e-30f;
+}
+
 struct Hit {
   float t;
   int inst;
@@ -1290,7 +1299,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
       }
       wc.quads++;
       float ts[2];
-      int n = quadricRoots(in.geomKind, oo, dd, tmin, hit.t * 1.0000004f + 1e-30f, ts);
+      int n = quadricRoots(in.geomKind, oo, dd, tmin, candBound(hit.t), ts);
       for (int k = 0; k < n; k++) {
         if (closer(ts[k], ii, k, hit) && maskAccepts(P, in, k, 0.0f, 0.0f, oo, dd, ts[k])) {
           hit.t = ts[k];
@@ -1311,7 +1320,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
         const float4* tv = P.tris + 3 * (size_t)(first + k);
         float4 a = tv[0], b = tv[1], c = tv[2];
         float t, b1, b2;
-        if (triHit(rt, obj.o, f4xyz(a), f4xyz(b), f4xyz(c), tmin, hit.t * 1.0000004f + 1e-30f, t, b1, b2)) {
+        if (triHit(rt, obj.o, f4xyz(a), f4xyz(b), f4xyz(c), tmin, candBound(hit.t), t, b1, b2)) {
           int prim = __float_as_int(a.w);
           if (closer(t, curInst, prim, hit) && maskAccepts(P, in, prim, b1, b2, obj.o, obj.d, t)) {
             hit.t = t;
@@ -1859,7 +1868,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_brute(LaunchParams P, int numI
     vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
     if (in.geomKind != GK_MESH) {
       float ts[2];
-      int nr = quadricRoots(in.geomKind, oo, dd, tmin, h.t * 1.0000004f + 1e-30f, ts);
+      int nr = quadricRoots(in.geomKind, oo, dd, tmin, candBound(h.t), ts);
       for (int k = 0; k < nr; k++) {
         if (closer(ts[k], ii, k, h) && maskAccepts(P, in, k, 0, 0, oo, dd, ts[k])) {
           h.t = ts[k];
@@ -1878,7 +1887,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_brute(LaunchParams P, int numI
     for (int t = 0; t < m.numTris; t++) {
       vec3 v0 = ld3i(pos, idx[3 * t]), v1 = ld3i(pos, idx[3 * t + 1]), v2 = ld3i(pos, idx[3 * t + 2]);
       float tt, b1, b2;
-      if (triHit(rt, oo, v0, v1, v2, tmin, h.t * 1.0000004f + 1e-30f, tt, b1, b2) && closer(tt, ii, t, h) &&
+      if (triHit(rt, oo, v0, v1, v2, tmin, candBound(h.t), tt, b1, b2) && closer(tt, ii, t, h) &&
           maskAccepts(P, in, t, b1, b2, oo, dd, tt)) {
         h.t = tt;
         h.inst = ii;

@@ -51,8 +51,8 @@ def main():
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-tile-stride", type=int, default=4)
-    ap.add_argument("--cpu-spp", type=int, default=192)
+    ap.add_argument("--cpu-tile-stride", type=int, default=8)
+    ap.add_argument("--cpu-spp", type=int, default=128)
     ap.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 4 times "

@@ -333,4 +333,38 @@ SUNDOG_API struct sundog_flat_scene* sundog_scene_flatten(sundog_scene*,
                                                           const sundog_render_options* opt);
 SUNDOG_API void sundog_flat_free(struct sundog_flat_scene*);
 
+/* ---- learned denoiser -------------------------------------------------------
+ * Replaces the vendor AI denoiser behind src/denoise.cpp:18-53 (HDR beauty +
+ * albedo + normal guides in, denoised HDR out) with a BF16 U-Net whose
+ * forward and backward passes are this library's own MFMA kernels.
+ *
+ * Image batches are NHWC float32 on the host: input [n][h][w][9] = beauty
+ * RGB, albedo RGB, normal XYZ; target and output [n][h][w][3] (linear HDR).
+ * h and w must be multiples of 16. Parameters are one flat float32 vector
+ * (per convolution: weights [cout][3*3][cin], then bias[cout]; cin padded to
+ * 16 for the first layer, cout padded to 32 for the output layer). */
+typedef struct sundog_denoiser sundog_denoiser;
+/* He-normal initialisation from a PCG32 stream; NULL on error. */
+SUNDOG_API sundog_denoiser* sundog_denoiser_create(uint64_t seed);
+SUNDOG_API void sundog_denoiser_destroy(sundog_denoiser*);
+SUNDOG_API int64_t sundog_denoiser_param_count(sundog_denoiser*);
+SUNDOG_API int sundog_denoiser_get_params(sundog_denoiser*, float* out);
+SUNDOG_API int sundog_denoiser_set_params(sundog_denoiser*, const float* in);
+/* One forward + backward of the L1 loss in log(1+x) space: loss, the full
+ * gradient vector and the network output (log space, [n][h][w][3]); each
+ * output pointer may be NULL. Parameters are not changed. */
+SUNDOG_API int sundog_denoiser_forward_backward(sundog_denoiser*, const float* input, const float* target, int n,
+                                                int h, int w, float* loss, float* grads, float* pred);
+/* forward_backward followed by one Adam step (beta 0.9 / 0.999). */
+SUNDOG_API int sundog_denoiser_train_step(sundog_denoiser*, const float* input, const float* target, int n, int h,
+                                          int w, float lr, float* loss);
+/* Denoised linear HDR [n][h][w][3]. */
+SUNDOG_API int sundog_denoiser_infer(sundog_denoiser*, const float* input, int n, int h, int w, float* out);
+/* Mean ms of `iters` forward passes at (n, h, w) on device-resident input,
+ * and the algorithmic FLOPs of one forward. */
+SUNDOG_API int sundog_denoiser_time_forward(sundog_denoiser*, int n, int h, int w, int iters, double* ms,
+                                            double* flops);
+SUNDOG_API int sundog_denoiser_save(sundog_denoiser*, const char* path);
+SUNDOG_API int sundog_denoiser_load(sundog_denoiser*, const char* path);
+
 #endif /* SUNDOG_API_H */

@@ -157,6 +157,17 @@ SIGNATURES = {
     "sundog_server_render_serial": (c_i, [c_p, c_i, c_i, c_i, c_i, c_i64, P_f, ctypes.POINTER(c_d)]),
     "sundog_scene_flatten": (c_p, [c_p, ctypes.POINTER(RenderOptions)]),
     "sundog_flat_free": (None, [c_p]),
+    "sundog_denoiser_create": (c_p, [c_u64]),
+    "sundog_denoiser_destroy": (None, [c_p]),
+    "sundog_denoiser_param_count": (c_i64, [c_p]),
+    "sundog_denoiser_get_params": (c_i, [c_p, P_f]),
+    "sundog_denoiser_set_params": (c_i, [c_p, P_f]),
+    "sundog_denoiser_forward_backward": (c_i, [c_p, P_f, P_f, c_i, c_i, c_i, P_f, P_f, P_f]),
+    "sundog_denoiser_train_step": (c_i, [c_p, P_f, P_f, c_i, c_i, c_i, c_f, P_f]),
+    "sundog_denoiser_infer": (c_i, [c_p, P_f, c_i, c_i, c_i, P_f]),
+    "sundog_denoiser_time_forward": (c_i, [c_p, c_i, c_i, c_i, c_i, ctypes.POINTER(c_d), ctypes.POINTER(c_d)]),
+    "sundog_denoiser_save": (c_i, [c_p, c_s]),
+    "sundog_denoiser_load": (c_i, [c_p, c_s]),
 }
 
 _LIB = None

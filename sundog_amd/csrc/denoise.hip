@@ -491,6 +491,7 @@ struct sundog_denoiser {
     }
     target.alloc(P0 * 3 * 4);
     devIn.alloc(P0 * 9 * 4);
+    gPred.alloc(P0 * 32 * 4);
     lossPartial.alloc(blocks(P0 * 32) * 4);
   }
 
@@ -651,7 +652,6 @@ struct sundog_denoiser {
 
   float lossAndGrad() {
     size_t P = (size_t)N * H * W;
-    gPred.alloc(P * 32 * 4);
     unsigned nb = blocks(P * 32);
     hipLaunchKernelGGL(k_l1, dim3(nb), dim3(256), 0, stream, pred.as<float>(), target.as<float>(),
                        gPred.as<float>(), lossPartial.as<float>(), P);

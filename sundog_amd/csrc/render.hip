@@ -1535,12 +1535,15 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
     bool specular = true, aovDone = false, alive = false;
     uint32_t medLo = 0, medHi = 0;
     int medDepth = 0, depth = 0, s = 0, si = 0;
+    // the pixel's running means stay in registers for this launch's samples
+    // (same operations in the same order as folding each sample in memory)
+    float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
 
     while (true) {
       if (!alive) {
         if (si > 0) {  // the previous sample finished: fold it into the film
           L = sanitize(L);
-          P.accum[pixel] = runningMean(P.accum[pixel], L, s);
+          fAcc = runningMean(fAcc, L, s);
         }
         if (si == P.sppThisLaunch) break;
         s = P.sampleOffset + si;
@@ -1606,8 +1609,8 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
         L += c;
         if (!aovDone) {
           vec3 b = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
-          P.albedo[pixel] = runningMean(P.albedo[pixel], b, s);
-          P.normal[pixel] = runningMean(P.normal[pixel], v3(0.0f), s);
+          fAlb = runningMean(fAlb, b, s);
+          fNrm = runningMean(fNrm, v3(0.0f), s);
         }
         continue;
       }
@@ -1624,8 +1627,8 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
 
       if (!aovDone) {
         vec3 nc = vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))};
-        P.albedo[pixel] = runningMean(P.albedo[pixel], albedo, s);
-        P.normal[pixel] = runningMean(P.normal[pixel], nc, s);
+        fAlb = runningMean(fAlb, albedo, s);
+        fNrm = runningMean(fNrm, nc, s);
         aovDone = true;
       }
 
@@ -1707,6 +1710,9 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
       depth++;
       alive = depth < P.maxDepth;
     }
+    P.accum[pixel] = fAcc;
+    P.albedo[pixel] = fAlb;
+    P.normal[pixel] = fNrm;
   }
 }
 

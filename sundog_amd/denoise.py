@@ -1,0 +1,281 @@
+"""Learned denoiser: ctypes wrapper of sundog_denoiser_*, training data from
+this renderer's own output, and a training / timing CLI.
+
+The reference hands beauty + albedo + normal films to a vendor HDR denoiser
+(src/denoise.cpp:18-53, called from src/capi_render.cpp:247-256). Here the
+same three films feed a BF16 U-Net whose forward and backward passes run on
+the library's own MFMA kernels.
+
+    python -m sundog_amd.denoise train --steps 2000 --out weights.sdn
+    python -m sundog_amd.denoise time --height 1088 --width 1920
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+from . import _lib
+from . import configs
+from .scenelib import Scene, translate, scale, rotate_x, rotate_y, rotate_z
+
+
+def _fp(a):
+    return a.ctypes.data_as(_lib.P_f) if a is not None else None
+
+
+class Denoiser(object):
+    def __init__(self, seed=1, lib=None):
+        self.lib = lib or _lib.load()
+        self.ptr = self.lib.sundog_denoiser_create(int(seed))
+        if not self.ptr:
+            raise _lib.SundogError(_lib.last_error(self.lib))
+        self.n_params = int(self.lib.sundog_denoiser_param_count(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            self.lib.sundog_denoiser_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _lib.check(self.lib.sundog_denoiser_get_params(self.ptr, _fp(p)), self.lib)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("expected %d parameters, got %d" % (self.n_params, p.size))
+        _lib.check(self.lib.sundog_denoiser_set_params(self.ptr, _fp(p)), self.lib)
+
+    @staticmethod
+    def _check_batch(inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        if inp.ndim == 3:
+            inp = inp[None]
+        if inp.ndim != 4 or inp.shape[3] != 9:
+            raise ValueError("input must be [n][h][w][9]")
+        return inp
+
+    def forward_backward(self, inp, target, want_grads=True, want_pred=True):
+        inp = self._check_batch(inp)
+        n, h, w, _ = inp.shape
+        target = np.ascontiguousarray(target, np.float32).reshape(n, h, w, 3)
+        loss = ctypes.c_float(0)
+        g = np.zeros(self.n_params, np.float32) if want_grads else None
+        pred = np.zeros((n, h, w, 3), np.float32) if want_pred else None
+        _lib.check(self.lib.sundog_denoiser_forward_backward(self.ptr, _fp(inp), _fp(target), n, h, w,
+                                                             ctypes.byref(loss), _fp(g), _fp(pred)), self.lib)
+        return loss.value, g, pred
+
+    def train_step(self, inp, target, lr):
+        inp = self._check_batch(inp)
+        n, h, w, _ = inp.shape
+        target = np.ascontiguousarray(target, np.float32).reshape(n, h, w, 3)
+        loss = ctypes.c_float(0)
+        _lib.check(self.lib.sundog_denoiser_train_step(self.ptr, _fp(inp), _fp(target), n, h, w, float(lr),
+                                                       ctypes.byref(loss)), self.lib)
+        return loss.value
+
+    def infer(self, inp):
+        inp = self._check_batch(inp)
+        n, h, w, _ = inp.shape
+        out = np.zeros((n, h, w, 3), np.float32)
+        _lib.check(self.lib.sundog_denoiser_infer(self.ptr, _fp(inp), n, h, w, _fp(out)), self.lib)
+        return out
+
+    def infer_image(self, beauty, albedo, normal):
+        """Denoise one frame of any size: pads to a multiple of 16 by edge
+        replication and crops back."""
+        inp = make_input(beauty, albedo, normal)
+        h, w = inp.shape[:2]
+        ph, pw = (-h) % 16, (-w) % 16
+        if ph or pw:
+            inp = np.pad(inp, ((0, ph), (0, pw), (0, 0)), mode="edge")
+        return self.infer(inp)[0, :h, :w]
+
+    def time_forward(self, n, h, w, iters=10):
+        ms, fl = ctypes.c_double(0), ctypes.c_double(0)
+        _lib.check(self.lib.sundog_denoiser_time_forward(self.ptr, n, h, w, iters, ctypes.byref(ms),
+                                                         ctypes.byref(fl)), self.lib)
+        return ms.value, fl.value
+
+    def save(self, path):
+        _lib.check(self.lib.sundog_denoiser_save(self.ptr, path.encode()), self.lib)
+
+    def load(self, path):
+        _lib.check(self.lib.sundog_denoiser_load(self.ptr, path.encode()), self.lib)
+
+
+def make_input(beauty, albedo, normal):
+    """Films (h, w, >=3) -> network input (h, w, 9)."""
+    return np.concatenate([np.asarray(beauty, np.float32)[..., :3], np.asarray(albedo, np.float32)[..., :3],
+                           np.asarray(normal, np.float32)[..., :3]], axis=-1)
+
+
+def rel_rmse(a, b):
+    a = np.asarray(a, np.float64)[..., :3]
+    b = np.asarray(b, np.float64)[..., :3]
+    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-30)))
+
+
+# ---------------------------------------------------------------------------
+# training data: seed-randomised scene variants rendered by this library
+# ---------------------------------------------------------------------------
+
+def cornell_variant(v, width, height, spp, seed):
+    rng = np.random.default_rng(1000 + v)
+    s = Scene()
+    s.render(width=width, height=height, spp=spp, max_depth=8, seed=seed, clamp=0.0)
+    cam_x, cam_y = rng.uniform(-0.4, 0.4, 2)
+    s.camera(lookfrom=[cam_x, cam_y, 3.9], lookat=[0, 0, 0], vfov=float(rng.uniform(34, 46)))
+    s.background_solid(color=[0, 0, 0])
+    cols = rng.uniform(0.05, 0.85, (5, 3))
+    s.lambert("grey", color=list(cols[0] * 0.3 + 0.5))
+    s.lambert("left", color=list(cols[1]))
+    s.lambert("right", color=list(cols[2]))
+    s.lambert("ball", color=list(cols[3]))
+    s.metal("chrome", color=list(0.5 + 0.5 * cols[4]), roughness=float(rng.uniform(0.0, 0.4)))
+    s.dielectric("glass", ior=1.5, roughness=float(rng.choice([0.0, 0.0, 0.2])))
+    s.emissive("lamp", color=list(rng.uniform(5, 20, 3)), intensity=1.0)
+    s.add("rect", "grey", translate(0, -1, 0))
+    s.add("rect", "grey", rotate_x(180), translate(0, 1, 0))
+    s.add("rect", "grey", rotate_x(90), translate(0, 0, -1))
+    s.add("rect", "left", rotate_z(-90), translate(-1, 0, 0))
+    s.add("rect", "right", rotate_z(90), translate(1, 0, 0))
+    mats = ["ball", "glass", "chrome"]
+    for i in range(3):
+        r = float(rng.uniform(0.2, 0.4))
+        x, z = rng.uniform(-0.6, 0.6, 2)
+        s.add("sphere", mats[(i + v) % 3], scale(r), translate(float(x), -1 + r, float(z)))
+    lw, ld = rng.uniform(0.15, 0.4, 2)
+    s.add("rect", "lamp", scale(float(lw), 1, float(ld)), rotate_x(180), translate(0, 0.999, 0))
+    return s
+
+
+def instanced_variant(v, asset_dir, width, height, spp, seed):
+    rng = np.random.default_rng(2000 + v)
+    s = configs.instanced(asset_dir, width=width, height=height, spp=spp, max_depth=8, seed=seed,
+                          instances=int(rng.integers(64, 512)), subdiv=3, layout_seed=3 + v)
+    return s
+
+
+def render_pair(scene_fn, noisy_spp, ref_spp, seed):
+    """(input [h][w][9], target [h][w][3]) of one scene variant: the noisy
+    render and an independent high-spp reference."""
+    noisy = scene_fn(noisy_spp, seed).build()
+    sess = noisy.session()
+    sess.render_all()
+    b, a, n = sess.download()
+    sess.close()
+    ref = scene_fn(ref_spp, seed + 7919).build()
+    sr = ref.session()
+    sr.render_all(chunk=64)
+    rb, _, _ = sr.download()
+    sr.close()
+    return make_input(b, a, n), rb[..., :3].copy()
+
+
+def dataset(asset_dir, n_variants, width, height, noisy_spp, ref_spp, first_variant=0, seed=11):
+    xs, ys = [], []
+    for v in range(first_variant, first_variant + n_variants):
+        if v % 3 == 2:
+            fn = lambda spp, sd, v=v: instanced_variant(v, asset_dir, width, height, spp, sd)  # noqa: E731
+        else:
+            fn = lambda spp, sd, v=v: cornell_variant(v, width, height, spp, sd)  # noqa: E731
+        x, y = render_pair(fn, noisy_spp, ref_spp, seed + 101 * v)
+        xs.append(x)
+        ys.append(y)
+    return np.stack(xs), np.stack(ys)
+
+
+def random_crops(xs, ys, rng, batch, crop):
+    n, h, w, _ = xs.shape
+    bi = rng.integers(0, n, batch)
+    oy = rng.integers(0, h - crop + 1, batch)
+    ox = rng.integers(0, w - crop + 1, batch)
+    cx = np.stack([xs[b, y:y + crop, x:x + crop] for b, y, x in zip(bi, oy, ox)])
+    cy = np.stack([ys[b, y:y + crop, x:x + crop] for b, y, x in zip(bi, oy, ox)])
+    flip = rng.integers(0, 2, batch).astype(bool)
+    cx[flip] = cx[flip, :, ::-1]
+    cy[flip] = cy[flip, :, ::-1]
+    # a horizontal flip mirrors the camera-space normal's x component
+    cx[flip, :, :, 6] *= -1.0
+    return cx, cy
+
+
+def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0):
+    rng = np.random.default_rng(seed)
+    losses = []
+    for i in range(steps):
+        cx, cy = random_crops(xs, ys, rng, batch, crop)
+        cur_lr = lr * (0.1 ** (i / max(steps, 1)))
+        losses.append(den.train_step(cx, cy, cur_lr))
+        if log_every and (i + 1) % log_every == 0:
+            print("step %d loss %.4f" % (i + 1, float(np.mean(losses[-log_every:]))), flush=True)
+    return losses
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--steps", type=int, default=2000)
+    t.add_argument("--variants", type=int, default=12)
+    t.add_argument("--size", type=int, default=128)
+    t.add_argument("--noisy-spp", type=int, default=8)
+    t.add_argument("--ref-spp", type=int, default=1024)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--crop", type=int, default=64)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--out", default="")
+    t.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
+    tm = sub.add_parser("time")
+    tm.add_argument("--batch", type=int, default=1)
+    tm.add_argument("--height", type=int, default=1088)
+    tm.add_argument("--width", type=int, default=1920)
+    tm.add_argument("--iters", type=int, default=10)
+    args = ap.parse_args(argv)
+
+    if args.cmd == "time":
+        den = Denoiser(seed=1)
+        ms, fl = den.time_forward(args.batch, args.height, args.width, args.iters)
+        out = {"forward_ms": ms, "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12,
+               "mfma_frac_bf16_dense": fl / (ms * 1e-3) / 2.5e15,
+               "shape": [args.batch, args.height, args.width]}
+        print(json.dumps(out), flush=True)
+        return out
+
+    t0 = time.time()
+    xs, ys = dataset(args.asset_dir, args.variants, args.size, args.size, args.noisy_spp, args.ref_spp)
+    xt, yt = dataset(args.asset_dir, 3, args.size, args.size, args.noisy_spp, args.ref_spp,
+                     first_variant=args.variants)
+    print("data: %d train + %d held-out images, %.1f s" % (len(xs), len(xt), time.time() - t0), flush=True)
+    den = Denoiser(seed=1)
+    t1 = time.time()
+    losses = train(den, xs, ys, args.steps, args.batch, args.crop, args.lr, log_every=max(args.steps // 10, 1))
+    train_s = time.time() - t1
+    noisy = [rel_rmse(x[..., :3], y) for x, y in zip(xt, yt)]
+    den_out = den.infer(xt)
+    denoised = [rel_rmse(d, y) for d, y in zip(den_out, yt)]
+    out = {"steps": args.steps, "train_s": train_s, "final_loss": float(np.mean(losses[-50:])),
+           "heldout_rel_rmse_noisy": noisy, "heldout_rel_rmse_denoised": denoised}
+    print(json.dumps(out), flush=True)
+    if args.out:
+        den.save(args.out)
+    return out
+
+
+if __name__ == "__main__":
+    sys.exit(0 if main() else 1)

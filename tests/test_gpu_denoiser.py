@@ -1,0 +1,113 @@
+"""GPU denoiser: MFMA forward/backward against the CPU restatement (with the
+same BF16 rounding points), determinism, training, and the gain of a
+network trained on this renderer's output over the noisy input."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import denoiser_ref as dr  # noqa: E402
+
+from sundog_amd import denoise  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(rng, n, h, w):
+    x = np.zeros((n, h, w, 9), np.float32)
+    x[..., :3] = rng.gamma(0.8, 1.5, (n, h, w, 3))
+    x[..., 3:6] = rng.uniform(0, 1, (n, h, w, 3))
+    nn = rng.standard_normal((n, h, w, 3))
+    x[..., 6:9] = nn / np.linalg.norm(nn, axis=-1, keepdims=True)
+    y = (x[..., :3] * rng.uniform(0.5, 1.5, (n, h, w, 3))).astype(np.float32)
+    return x, y
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("shape", [(2, 32, 32), (1, 32, 48)])
+def test_forward_backward_matches_oracle(shape):
+    n, h, w = shape
+    den = denoise.Denoiser(seed=3)
+    p = den.get_params()
+    rng = np.random.default_rng(h * 100 + w)
+    # non-zero biases so every bias path is exercised
+    offs, _ = dr.param_offsets()
+    for (cin, cout, _), (wo, bo) in zip(dr.conv_specs(), offs):
+        p[bo:bo + cout] = 0.02 * rng.standard_normal(cout)
+    den.set_params(p)
+    x, y = _batch(rng, n, h, w)
+    loss, grads, pred = den.forward_backward(x, y)
+    pred_o, cache = dr.forward(p, x, emulate=True)
+    loss_o, gpred = dr.loss_grad(pred_o, y, emulate=True)
+    grads_o = dr.backward(p, cache, gpred, emulate=True)
+    # forward: FP32 accumulation order differs from the restatement's, and a
+    # BF16 store can then round the other way; both stay far below a layout
+    # or indexing error, which gives O(1) differences
+    assert _rel(pred, pred_o[..., :3]) < 1e-2, _rel(pred, pred_o[..., :3])
+    assert abs(loss - loss_o) <= 1e-3 * abs(loss_o)
+    for k, ((cin, cout, _), (wo, bo)) in enumerate(zip(dr.conv_specs(), offs)):
+        ew = _rel(grads[wo:bo], grads_o[wo:bo])
+        eb = _rel(grads[bo:bo + cout], grads_o[bo:bo + cout])
+        assert ew < 3e-2 and eb < 3e-2, (k, ew, eb)
+
+
+def test_deterministic_and_params_unchanged():
+    den = denoise.Denoiser(seed=5)
+    p0 = den.get_params()
+    rng = np.random.default_rng(2)
+    x, y = _batch(rng, 2, 32, 32)
+    l1, g1, p1 = den.forward_backward(x, y)
+    l2, g2, p2 = den.forward_backward(x, y)
+    assert l1 == l2
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
+    assert np.array_equal(p1.view(np.uint32), p2.view(np.uint32))
+    assert np.array_equal(den.get_params(), p0)
+
+
+def test_save_load_and_infer(tmp_path):
+    den = denoise.Denoiser(seed=9)
+    rng = np.random.default_rng(4)
+    x, y = _batch(rng, 1, 32, 32)
+    for _ in range(3):
+        den.train_step(x, y, 1e-3)
+    out = den.infer(x)
+    path = str(tmp_path / "w.sdn")
+    den.save(path)
+    d2 = denoise.Denoiser(seed=1)
+    d2.load(path)
+    assert np.array_equal(d2.get_params(), den.get_params())
+    assert np.array_equal(d2.infer(x), out)
+    assert out.shape == (1, 32, 32, 3) and np.isfinite(out).all()
+    # infer returns expm1 of the network output
+    _, _, pred = den.forward_backward(x, y)
+    assert np.allclose(out, np.expm1(pred), rtol=1e-5, atol=1e-6)
+
+
+def test_training_reduces_loss():
+    den = denoise.Denoiser(seed=1)
+    rng = np.random.default_rng(8)
+    x, y = _batch(rng, 4, 32, 32)
+    first = den.train_step(x, y, 2e-3)
+    for _ in range(60):
+        last = den.train_step(x, y, 2e-3)
+    assert last < 0.6 * first, (first, last)
+
+
+def test_denoiser_gain_on_renders(asset_dir):
+    """Trained on noisy/reference pairs of seed-randomised scene variants,
+    the network must lower the error of held-out noisy renders."""
+    xs, ys = denoise.dataset(asset_dir, 9, 96, 96, noisy_spp=8, ref_spp=512)
+    xt, yt = denoise.dataset(asset_dir, 3, 96, 96, noisy_spp=8, ref_spp=512, first_variant=40)
+    den = denoise.Denoiser(seed=1)
+    denoise.train(den, xs, ys, steps=600, batch=8, crop=64, lr=2e-3)
+    out = den.infer(xt)
+    noisy = np.mean([denoise.rel_rmse(x[..., :3], y) for x, y in zip(xt, yt)])
+    den_err = np.mean([denoise.rel_rmse(d, y) for d, y in zip(out, yt)])
+    print("held-out relRMSE noisy %.4f denoised %.4f" % (noisy, den_err))
+    assert den_err < 0.8 * noisy, (noisy, den_err)

@@ -21,7 +21,9 @@ Network, parameter layout and rounding points follow the kernels:
     output convolution (cout padded to 32, 3 used) stores FP32;
   * 2x2 average pooling of BF16 values, rounded to BF16; nearest 2x
     upsampling concatenated with the encoder skip (upsampled part first);
-  * loss: mean |pred - log1p(max(target, 0))| over the 3 used channels;
+  * final output (log space): the network's first 3 output channels plus
+    log1p(max(beauty, 0)) in float32 (a residual on the noisy input);
+  * loss: mean |final - log1p(max(target, 0))| over the 3 channels;
   * backward: upstream gradients are rounded to BF16 where the kernels feed
     them to MFMA (backward-data and backward-weights), bias gradients and
     data gradients stay unrounded.
@@ -170,15 +172,24 @@ def forward(params, inp, emulate=True):
     return pred, cache
 
 
-def loss_grad(pred, target, emulate=True):
-    """L1 in log space over the 3 used channels: (loss, dL/dpred [N,H,W,32])."""
+def final_output(pred, inp, emulate=True):
+    """Network output [N,H,W,32] -> final log-space image [N,H,W,3]."""
+    b = np.maximum(np.asarray(inp, np.float32)[..., :3], 0.0)
+    if emulate:
+        return (pred[..., :3].astype(np.float32) + np.log1p(b)).astype(np.float64)
+    return pred[..., :3] + np.log1p(b.astype(np.float64))
+
+
+def loss_grad(final, target, emulate=True):
+    """L1 in log space: (loss, dL/dnet [N,H,W,32]) for the final output
+    [N,H,W,3] (d final / d net = 1 on the 3 used channels)."""
     t = np.log1p(np.maximum(np.asarray(target, np.float32), 0.0)).astype(np.float64)
     if emulate:
-        d = (pred[..., :3].astype(np.float32) - t.astype(np.float32)).astype(np.float64)
+        d = (final.astype(np.float32) - t.astype(np.float32)).astype(np.float64)
     else:
-        d = pred[..., :3] - t
+        d = final - t
     cnt = d.size
-    g = np.zeros_like(pred)
+    g = np.zeros(final.shape[:3] + (OUT_PAD,))
     g[..., :3] = np.sign(d) / cnt
     if emulate:
         g = g.astype(np.float32).astype(np.float64)

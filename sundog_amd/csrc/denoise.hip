@@ -9,7 +9,9 @@
 //   -> up+cat(e2) -> d2(96,96) -> up+cat(e1) -> d1(64,64) -> up+cat(e0)
 //   -> d0(32,32) -> out(3, padded 32), linear
 // Inputs: log(1+max(beauty,0)) RGB, albedo RGB, normal XYZ. Output: the
-// log(1+x) of the denoised radiance. Loss: L1 in that log-HDR space.
+// log(1+x) of the denoised radiance, predicted as a residual added to the
+// noisy input's log(1+x) (so the untrained network starts at the identity).
+// Loss: L1 in that log-HDR space.
 //
 // Activations are NHWC BF16, gradients NHWC FP32, master weights FP32 with
 // Adam; BF16 weight copies are kept in two layouts, [co][tap][ci] for the
@@ -268,9 +270,21 @@ __global__ void k_pack_input(const float* __restrict__ src, uint16_t* __restrict
   dst[i] = f2bf(v);
 }
 
-// L1 loss in log space on the first 3 channels of pred [P][32]; writes the
-// gradient (sign / count) and per-block partial loss sums
-__global__ void k_l1(const float* __restrict__ pred, const float* __restrict__ target, float* __restrict__ grad,
+// Final log-space output: the network predicts a residual on top of the
+// noisy input, out[p][c] = net[p][c] + log1p(max(beauty[p][c], 0)), c < 3.
+__global__ void k_final(const float* __restrict__ net, const float* __restrict__ src, float* __restrict__ out,
+                        size_t P) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * 3) return;
+  size_t p = i / 3;
+  int c = i % 3;
+  out[i] = net[p * 32 + c] + log1pf(fmaxf(src[p * 9 + c], 0.0f));
+}
+
+// L1 loss in log space on the final output [P][3]; writes the gradient
+// w.r.t. the network output [P][32] (sign / count on the 3 used channels,
+// zero elsewhere) and per-block partial loss sums
+__global__ void k_l1(const float* __restrict__ fin, const float* __restrict__ target, float* __restrict__ grad,
                      float* __restrict__ partial, size_t P) {
   __shared__ float s[256];
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,7 +294,7 @@ __global__ void k_l1(const float* __restrict__ pred, const float* __restrict__ t
     int c = i % 32;
     float g = 0.0f;
     if (c < 3) {
-      float d = pred[i] - log1pf(fmaxf(target[p * 3 + c], 0.0f));
+      float d = fin[p * 3 + c] - log1pf(fmaxf(target[p * 3 + c], 0.0f));
       l = fabsf(d);
       g = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) / (float)(P * 3);
     }
@@ -295,10 +309,10 @@ __global__ void k_l1(const float* __restrict__ pred, const float* __restrict__ t
   if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-__global__ void k_unpack_output(const float* __restrict__ pred, float* __restrict__ out, size_t P) {
+__global__ void k_unpack_output(const float* __restrict__ fin, float* __restrict__ out, size_t P) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P * 3) return;
-  out[i] = expm1f(pred[(i / 3) * 32 + (i % 3)]);
+  out[i] = expm1f(fin[i]);
 }
 
 // Adam on FP32 masters, then the two BF16 layouts. Weights of one conv are
@@ -377,7 +391,7 @@ struct sundog_denoiser {
   int N = 0, H = 0, W = 0;
   std::vector<DevBuf> y;           // conv outputs (BF16)
   std::vector<int> yH, yW;         // spatial size per conv
-  DevBuf input, pooled[4], cat[4], pred;
+  DevBuf input, pooled[4], cat[4], pred, fin;
   std::vector<DevBuf> gy;          // gradients w.r.t. conv outputs (FP32)
   DevBuf gPool[4], gCat[4], gInput, gPred, partial, lossPartial, target, devIn;
   size_t partialCap = 0;
@@ -481,6 +495,7 @@ struct sundog_denoiser {
     input.alloc(P0 * 16 * 2);
     gInput.alloc(P0 * 16 * 4);
     pred.alloc(P0 * 32 * 4);
+    fin.alloc(P0 * 3 * 4);
     for (int l = 0; l < 4; l++) {
       size_t pl = (size_t)n * (h >> (l + 1)) * (w >> (l + 1));
       pooled[l].alloc(pl * kCh[l] * 2);
@@ -539,6 +554,9 @@ struct sundog_denoiser {
       k += 2;
     }
     conv(k, cur, false, pred.p, true, H, W);
+    size_t P = (size_t)N * H * W;
+    hipLaunchKernelGGL(k_final, dim3(blocks(P * 3)), dim3(256), 0, stream, pred.as<float>(), devIn.as<float>(),
+                       fin.as<float>(), P);
     HIP_OK(hipGetLastError());
   }
 
@@ -653,7 +671,7 @@ struct sundog_denoiser {
   float lossAndGrad() {
     size_t P = (size_t)N * H * W;
     unsigned nb = blocks(P * 32);
-    hipLaunchKernelGGL(k_l1, dim3(nb), dim3(256), 0, stream, pred.as<float>(), target.as<float>(),
+    hipLaunchKernelGGL(k_l1, dim3(nb), dim3(256), 0, stream, fin.as<float>(), target.as<float>(),
                        gPred.as<float>(), lossPartial.as<float>(), P);
     std::vector<float> lp(nb);
     HIP_OK(hipMemcpyAsync(lp.data(), lossPartial.p, nb * 4, hipMemcpyDeviceToHost, stream));
@@ -751,11 +769,8 @@ extern "C" int sundog_denoiser_forward_backward(sundog_denoiser* d, const float*
     }
     if (pred_out) {
       size_t P = (size_t)n * h * w;
-      std::vector<float> p(P * 32);
       HIP_OK(hipStreamSynchronize(d->stream));
-      HIP_OK(hipMemcpy(p.data(), d->pred.p, P * 32 * 4, hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < P; i++)
-        for (int c = 0; c < 3; c++) pred_out[i * 3 + c] = p[i * 32 + c];
+      HIP_OK(hipMemcpy(pred_out, d->fin.p, P * 3 * 4, hipMemcpyDeviceToHost));
     }
     return SUNDOG_OK;
   });
@@ -781,7 +796,7 @@ extern "C" int sundog_denoiser_infer(sundog_denoiser* d, const float* input, int
     size_t P = (size_t)n * h * w;
     DevBuf o;
     o.alloc(P * 3 * 4);
-    hipLaunchKernelGGL(k_unpack_output, dim3(blocks(P * 3)), dim3(256), 0, d->stream, d->pred.as<float>(),
+    hipLaunchKernelGGL(k_unpack_output, dim3(blocks(P * 3)), dim3(256), 0, d->stream, d->fin.as<float>(),
                        o.as<float>(), P);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out, o.p, P * 3 * 4, hipMemcpyDeviceToHost, d->stream));
@@ -798,6 +813,7 @@ extern "C" int sundog_denoiser_time_forward(sundog_denoiser* d, int n, int h, in
     if (!d) throw std::runtime_error("denoiser: null handle");
     d->resize(n, h, w);
     HIP_OK(hipMemsetAsync(d->input.p, 0, (size_t)n * h * w * 16 * 2, d->stream));
+    HIP_OK(hipMemsetAsync(d->devIn.p, 0, (size_t)n * h * w * 9 * 4, d->stream));
     d->forward();
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));

@@ -112,8 +112,11 @@ def test_backward_matches_finite_differences():
 def test_loss_gradient_is_sign_over_count():
     rng = np.random.default_rng(3)
     pred = rng.standard_normal((1, 4, 4, 32))
+    inp = _input(rng, 1, 4, 4)
     tgt = rng.gamma(1.0, 1.0, (1, 4, 4, 3)).astype(np.float32)
-    loss, g = dr.loss_grad(pred, tgt, emulate=False)
-    d = pred[..., :3] - np.log1p(tgt.astype(np.float64))
+    fin = dr.final_output(pred, inp, emulate=False)
+    assert np.allclose(fin, pred[..., :3] + np.log1p(inp[..., :3].astype(np.float64)))
+    loss, g = dr.loss_grad(fin, tgt, emulate=False)
+    d = fin - np.log1p(tgt.astype(np.float64))
     assert np.isclose(loss, np.abs(d).mean())
     assert np.array_equal(g[..., :3] * d.size, np.sign(d)) and not g[..., 3:].any()

@@ -27,8 +27,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "paths/s & Mrays/s at 1080p; p50/p99 request latency at ≤1% RMSE"
-FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# MI355X FP32 vector peak without packed FP32 (256 CUs x 64 lanes x 2 flops
+# per FMA x 2.4 GHz); the published 157.3 TFLOP/s counts packed-FP32 FMAs,
+# which this build does not emit.
+FP32_PEAK_TFLOPS = 78.6
+FP32_PEAK_PACKED_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_render_instanced_1080p.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter profile of this workload, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return prof["traffic_bytes"] if prof.get("kernel") == kernel else None
 
 
 def copy_probe(mib):
@@ -50,6 +66,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--waves", type=int, default=2, choices=[2, 3, 4],
+                    help="minimum waves per SIMD the megakernel's register budget allows")
+    ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
@@ -75,15 +94,20 @@ def main():
     scene = configs.instanced(args.asset_dir, width=W, height=H, spp=SPP, max_depth=DEPTH,
                               instances=args.instances).build()
     sess = scene.session()
+    sess.tune("waves", args.waves)
+    sess.set_kernel(args.kernel)
     setup_s = time.time() - t0
     k = args.spp_per_step
 
-    # instrumented pass: traversal work per ray query (same kernel, counting variant)
+    # instrumented pass: traversal work per ray query (the megakernel's
+    # counting variant; the wavefront variant runs the same traversal code)
+    sess.set_kernel("megakernel")
     sess.count_rays(True)
     sess.count_work(True)
     sess.render(0, k)
     sess.sync()
     st = sess.stats()
+    sess.set_kernel(args.kernel)
     queries = max(st["rays_traced"], 1)
     work = {"box_tests": 4 * st["node_visits"], "tri_tests": st["tri_tests"],
             "quad_tests": st["quad_tests"], "inst_transforms": st["inst_transforms"]}
@@ -110,6 +134,8 @@ def main():
     kernel_ms_avg = st["render_ms"] / args.steps
     flops_per_launch = flops_per_query * rays / args.steps
     achieved_tflops = flops_per_launch / (kernel_ms_avg * 1e-3) / 1e12
+    kname = ("sdk::k_render<false, %d>" % args.waves if args.kernel == "megakernel"
+             else "wavefront pass (k_wf_shade + k_wf_closest + k_wf_shadow)")
     beauty, _, _ = sess.download()
     finite = bool(np.isfinite(beauty).all())
 
@@ -130,7 +156,8 @@ def main():
             "workload": "instanced icospheres: %d x 20,480-tri instances, %dx%d, %d spp, depth %d"
                         % (args.instances, W, H, SPP, DEPTH),
             "step": "one %d-spp progressive pass over the full frame" % k,
-            "kernel": "megakernel",
+            "kernel": args.kernel,
+            "waves_per_simd": args.waves,
         },
         "mrays_per_s": rays / wall / 1e6,
         "rays_per_path": rays / max(paths, 1),
@@ -140,8 +167,9 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "k_render<false>",
+            "traffic": measured_traffic(kname),
+            "peak_packed_fp32": FP32_PEAK_PACKED_TFLOPS,
+            "kernel": kname,
             "kernel_ms_avg": kernel_ms_avg,
             "algorithmic_flops_per_query": flops_per_query,
             "work_per_query": {k2: v / queries for k2, v in work.items()},

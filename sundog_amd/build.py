@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
           "-I" + os.path.join(ROOT, "include")]
-HIP_FLAGS = ["--offload-arch=gfx950", "-x", "hip", "-ffp-contract=fast-honor-pragmas",
+HIP_FLAGS = ["--offload-arch=gfx950", "-x", "hip", "-ffp-contract=on",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-pass-failed"]
 

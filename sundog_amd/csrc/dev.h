@@ -151,4 +151,30 @@ struct QueueState {
   uint32_t pad[2];
 };
 
+// Wavefront variant. Path state lives in kWfPlanes float4 planes of
+// numSlots entries (plane j of slot i at state[j * numSlots + i]; one slot
+// per pixel), so every state access of a wave is one coalesced 1 KiB
+// transaction. Between kernels, slots move through three lists: closest-hit
+// queries, shadow queries, and kWfKeys shading buckets (one per material
+// kind of the surface to shade, plus misses and new samples) that order the
+// shading kernel's work by material.
+constexpr int kWfKeys = 16;
+constexpr int kWfPlanes = 16;
+
+struct WfCounters {
+  uint32_t closestN, shadowN, pad0, pad1;
+  uint32_t bucketN[kWfKeys];
+};
+
+struct WfParams {
+  float4* state;
+  int32_t numSlots;
+  int32_t first;       // first shading pass of a launch: every slot, no list
+  uint32_t* closestQ;  // numSlots entries
+  uint32_t* shadowQ;   // numSlots entries
+  uint32_t* buckets;   // kWfKeys lists of numSlots entries
+  WfCounters* cnt;
+  unsigned long long* rayCounter;
+};
+
 }  // namespace sdk

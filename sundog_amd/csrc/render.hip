@@ -1508,6 +1508,273 @@ DEV float4 runningMean(float4 m, vec3 x, int s) {
                      1.0f);
 }
 
+// ---------------------------------------------------------------------------
+// path state machine (device/programs.cu:311-572). A path advances through
+// stage functions between ray queries: pathBegin (camera ray of the next
+// sample), pathOnHit (after a closest-hit query: media, flames, background,
+// emission, AOVs, NEE light sample), pathOnShadow (after the NEE shadow
+// query: flame transmittance, MIS-weighted light contribution) and
+// pathScatter (BSDF sample, medium stack, Russian roulette). The megakernel
+// keeps the state in registers and traces inline; the wavefront kernels keep
+// it in HBM between kernels. Both run the same stage functions, so both give
+// the same film bits.
+//
+// The NEE shadow query splits a bounce: flame transmittance along the
+// shadow ray draws random numbers only when the ray is unblocked
+// (device/programs.cu:493-505), so everything after the query waits for it.
+// ---------------------------------------------------------------------------
+
+enum PathNext { PN_TRACE = 0, PN_SHADOW = 1, PN_END = 2 };
+
+struct PathState {
+  vec3 o, d;  // the path ray of the next closest-hit query
+  vec3 L, beta, prevX;
+  float prevPdf;
+  uint64_t rngState;  // PCG32 state; the increment is (seed << 1) | 1
+  uint32_t medLo, medHi;
+  int depth, medDepth;
+  int s, si;  // sample index; samples begun in this launch
+  bool specular;
+  vec3 aovA, aovN;  // this sample's first-hit albedo and camera-space normal
+};
+
+// What pathOnShadow and pathScatter need from the hit being shaded.
+struct SurfState {
+  vec3 x, ns, albedo;
+  float etaRel, etaExt;
+  uint32_t matId;
+  bool frontface;
+  // NEE light sample waiting for its shadow query
+  vec3 wi, Li;
+  float pdf, dist, cosS;
+  int k;
+  bool isDelta, transmitNee;
+};
+
+DEV int numStrata(const LaunchParams& P) {
+  int nStrata = (int)floorf(sqrtf((float)P.sppTotal) + 0.5f);
+  if (nStrata * nStrata > P.sppTotal) nStrata--;
+  return nStrata < 1 ? 1 : nStrata;
+}
+DEV int neeStrategies(const LaunchParams& P) { return P.numLights + (P.hasEnv ? 1 : 0); }
+
+// Starts the next sample of the pixel: stratified jitter over the total
+// spp, thin-lens or pinhole camera ray (device/programs.cu:316-350). Returns
+// false when the launch's samples are all begun.
+DEV bool pathBegin(const LaunchParams& P, PathState& ps, int px, int py) {
+  if (ps.si == P.sppThisLaunch) return false;
+  const int pixel = py * P.width + px;
+  const int nStrata = numStrata(P);
+  ps.s = P.sampleOffset + ps.si;
+  ps.si++;
+  Pcg rng = Pcg::make(((uint64_t)pixel << 32) ^ (uint64_t)ps.s, (uint64_t)P.seed);
+  float jx, jy;
+  if (ps.s < nStrata * nStrata) {
+    jx = ((ps.s % nStrata) + rng.rnd()) / nStrata;
+    jy = ((ps.s / nStrata) + rng.rnd()) / nStrata;
+  } else {
+    jx = rng.rnd();
+    jy = rng.rnd();
+  }
+  float u = (px + jx) / P.width;
+  float v = ((P.height - 1 - py) + jy) / P.height;
+  vec3 org = ld3(P.cam.origin);
+  if (P.cam.lensRadius > 0.0f) {
+    vec2 rd = concentricDisk(rng.rnd2()) * P.cam.lensRadius;
+    org = org + ld3(P.cam.u) * rd.x + ld3(P.cam.v) * rd.y;
+  }
+  ps.d = normalize(ld3(P.cam.lowerLeft) + u * ld3(P.cam.horizontal) + v * ld3(P.cam.vertical) - org);
+  ps.o = org;
+  ps.prevX = org;
+  ps.L = v3(0.0f);
+  ps.beta = v3(1.0f);
+  ps.prevPdf = 0.0f;
+  ps.specular = true;
+  ps.medDepth = 0;
+  ps.depth = 0;
+  ps.aovA = v3(0.0f);
+  ps.aovN = v3(0.0f);
+  ps.rngState = rng.state;
+  return true;
+}
+
+DEV Pcg pathRng(const LaunchParams& P, const PathState& ps) {
+  Pcg r;
+  r.state = ps.rngState;
+  r.inc = ((uint64_t)P.seed << 1u) | 1u;
+  return r;
+}
+
+// BSDF sample, medium stack update, throughput, Russian roulette from depth
+// 4 with q = clamp(max beta, 0.05, 0.95) (device/programs.cu:524-557).
+DEV int pathScatter(const LaunchParams& P, PathState& ps, const SurfState& sf, Pcg& rng) {
+  const DevMaterial& mat = P.materials[sf.matId];
+  BsdfSample bs = bsdfSample(mat, sf.albedo, ps.d, sf.ns, sf.frontface, sf.etaExt, rng);
+  if (!bs.valid) return PN_END;
+  if ((mat.kind == MT_WATER || mat.kind == MT_DIELECTRIC) && dot(bs.wi, sf.ns) < 0.0f) {
+    if (sf.frontface) {
+      if (ps.medDepth < kMedMax) medSet(ps.medLo, ps.medHi, ps.medDepth, sf.matId);
+      ps.medDepth++;
+    } else if (ps.medDepth > 0) {
+      ps.medDepth--;
+    }
+  }
+  ps.beta *= bs.weight;
+  ps.specular = bs.isDelta;
+  ps.prevPdf = bs.isDelta ? 0.0f : bs.pdf;
+  ps.prevX = sf.x;
+  vec3 offN = dot(bs.wi, sf.ns) >= 0.0f ? sf.ns : -sf.ns;
+  ps.o = offsetRay(sf.x, offN);
+  ps.d = bs.wi;
+  if (ps.depth >= 4) {
+    float q = clampf(maxc(ps.beta), 0.05f, 0.95f);
+    if (rng.rnd() >= q) return PN_END;
+    ps.beta *= 1.0f / q;
+  }
+  if (maxc(ps.beta) <= 0.0f) return PN_END;
+  ps.depth++;
+  return ps.depth < P.maxDepth ? PN_TRACE : PN_END;
+}
+
+// After the closest-hit query of ps.o/ps.d (isHit, hit; oObj/dObj is the
+// ray in the hit instance's object space). Returns PN_SHADOW when an NEE
+// shadow query (shadowRay) must run before the bounce can finish.
+DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& hit, vec3 oObj, vec3 dObj,
+                  SurfState& sf) {
+  const int nStrat = neeStrategies(P);
+  Pcg rng = pathRng(P, ps);
+  if (ps.medDepth > 0) {
+    const DevMaterial& mm = P.materials[medGet(ps.medLo, ps.medHi, min(ps.medDepth, kMedMax) - 1)];
+    vec3 sigma = ld3(mm.absorb);
+    if (maxc(sigma) > 0.0f) ps.beta *= vexp(-sigma * (isHit ? hit.t : 1e4f));
+  }
+  if (P.numFlames > 0) {
+    vec3 trans = v3(1.0f);
+    vec3 Lv = marchFlames(P, ps.o, ps.d, isHit ? hit.t : 1e16f, rng, trans);
+    vec3 cv = ps.beta * Lv;
+    if (ps.depth >= 1 && P.clampVal > 0.0f) cv = vmin(cv, v3(P.clampVal));
+    ps.L += cv;
+    ps.beta *= trans;
+  }
+  if (!isHit) {
+    vec3 bg = evalBackground(P, ps.d);
+    float w = 1.0f;
+    if (P.hasEnv && !ps.specular) {
+      float pdfE = envPdf(P, ps.d) / nStrat;
+      w = ps.prevPdf / (ps.prevPdf + pdfE);
+    }
+    vec3 c = ps.beta * bg * w;
+    if (ps.depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
+    ps.L += c;
+    if (ps.depth == 0) {
+      ps.aovA = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
+      ps.aovN = v3(0.0f);
+    }
+    ps.rngState = rng.state;
+    return PN_END;
+  }
+
+  const DevInstance& in = P.instances[hit.inst];
+  ShadePoint sp = shadePoint(P, in, hit.prim, hit.b1, hit.b2, oObj, dObj, hit.t, true);
+  sf.frontface = sp.frontface;
+  sf.matId = sp.frontface ? (in.mats & 0xFFFFu) : (in.mats >> 16);
+  vec3 ns = sp.frontface ? sp.nFront : -sp.nFront;
+  sf.x = ps.o + hit.t * ps.d;
+  const DevMaterial& mat = P.materials[sf.matId];
+  sf.albedo = materialAlbedo(P, mat, sp.u, sp.v);
+  if (mat.kind == MT_WATER && mat.waveAmp > 0.0f)
+    ns = waterNormal(sf.x, ns, -ps.d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
+  sf.ns = ns;
+
+  if (ps.depth == 0) {
+    ps.aovA = sf.albedo;
+    ps.aovN = vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))};
+  }
+
+  if (mat.kind == MT_EMISSIVE) {
+    if (sp.frontface || mat.twoSided) {
+      vec3 Le = sf.albedo * mat.intensity;
+      float w = 1.0f;
+      if (!ps.specular && in.lightId >= 0 && P.numLights > 0) {
+        float pdfL = lightPdfSolidAngle(P.lights[in.lightId], ps.prevX, sf.x, ns) / nStrat;
+        w = ps.prevPdf / (ps.prevPdf + pdfL);
+      }
+      vec3 c = ps.beta * Le * w;
+      if (ps.depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
+      ps.L += c;
+    }
+    ps.rngState = rng.state;
+    return PN_END;
+  }
+
+  sf.etaExt = 1.0f;
+  if (mat.kind == MT_DIELECTRIC || mat.kind == MT_WATER) {
+    int outer = min(sp.frontface ? ps.medDepth : ps.medDepth - 1, kMedMax);
+    if (outer > 0) sf.etaExt = P.materials[medGet(ps.medLo, ps.medHi, outer - 1)].ior;
+  }
+  sf.etaRel = sp.frontface ? sf.etaExt / mat.ior : mat.ior / sf.etaExt;
+
+  if (nStrat > 0 && !bsdfIsDelta(mat)) {
+    int k = min((int)(rng.rnd() * nStrat), nStrat - 1);
+    LightSample ls = (k < P.numLights) ? sampleLight(P, P.lights[k], sf.x, rng) : sampleEnv(P, rng);
+    float cosS = ls.valid ? dot(ls.wi, ns) : 0.0f;
+    bool transmitNee = cosS < 0.0f && mat.kind == MT_DIELECTRIC;
+    if (ls.valid && (cosS > 0.0f || transmitNee)) {
+      sf.wi = ls.wi;
+      sf.Li = ls.Li;
+      sf.pdf = ls.pdf;
+      sf.dist = ls.dist;
+      sf.cosS = cosS;
+      sf.k = k;
+      sf.isDelta = ls.isDelta;
+      sf.transmitNee = transmitNee;
+      ps.rngState = rng.state;
+      return PN_SHADOW;
+    }
+  }
+  int r = pathScatter(P, ps, sf, rng);
+  ps.rngState = rng.state;
+  return r;
+}
+
+// The NEE shadow ray of a suspended bounce.
+DEV void shadowRay(const SurfState& sf, vec3& so, vec3& wi, float& tmax) {
+  so = offsetRay(sf.x, sf.transmitNee ? -sf.ns : sf.ns);
+  wi = sf.wi;
+  tmax = sf.dist * 0.999f;
+}
+
+// After the shadow query (transmittance tr): MIS-weighted light
+// contribution (device/programs.cu:481-524), then the BSDF sample.
+DEV int pathOnShadow(const LaunchParams& P, PathState& ps, const SurfState& sf, vec3 tr) {
+  const int nStrat = neeStrategies(P);
+  Pcg rng = pathRng(P, ps);
+  if (maxc(tr) > 0.0f) {
+    const DevMaterial& mat = P.materials[sf.matId];
+    if (P.numFlames > 0 && P.transparentShadows) {
+      vec3 so, wi;
+      float tmaxS;
+      shadowRay(sf, so, wi, tmaxS);
+      int skip = (sf.k < P.numLights) ? P.lights[sf.k].flameId : -1;
+      tr = tr * flameTransmittance(P, so, wi, tmaxS, rng, skip);
+    }
+    float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
+    vec3 f = bsdfEval(mat, sf.albedo, -ps.d, sf.wi, sf.ns, sf.etaRel);
+    float w = 1.0f;
+    if (!sf.isDelta) {
+      float pdfB = bsdfPdf(mat, -ps.d, sf.wi, sf.ns, sf.etaRel);
+      w = pdfLe / (pdfLe + pdfB);
+    }
+    vec3 c = ps.beta * f * fabsf(sf.cosS) * sf.Li * w / pdfLe;
+    c = c * tr;
+    if (ps.depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
+    ps.L += sanitize(c);
+  }
+  int r = pathScatter(P, ps, sf, rng);
+  ps.rngState = rng.state;
+  return r;
+}
+
 // 16x16-pixel tiles; each wave64 covers an 8x8 quarter.
 DEV void pixelOf(int& px, int& py) {
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1515,205 +1782,45 @@ DEV void pixelOf(int& px, int& py) {
   py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
-// One work-item renders samples [sampleOffset, +sppThisLaunch) of one pixel
-// in order. The sample loop and the bounce loop are merged: a lane whose
-// path ends starts its next sample at once instead of idling until the
-// slowest path of its wave finishes; per-pixel sample order (and so the
-// running mean) is unchanged.
+// Megakernel driver: one work-item renders samples [sampleOffset,
+// +sppThisLaunch) of one pixel in order, tracing inline. A lane whose path
+// ends starts its next sample at once instead of idling until the slowest
+// path of its wave finishes; per-pixel sample order (and so the running
+// mean) is unchanged. The pixel's running means stay in registers for the
+// launch (the same operations in the same order as folding each sample in
+// memory).
 DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount& wc, uint32_t& rays) {
-  {
-    const int pixel = py * P.width + px;
-    int nStrata = (int)floorf(sqrtf((float)P.sppTotal) + 0.5f);
-    if (nStrata * nStrata > P.sppTotal) nStrata--;
-    if (nStrata < 1) nStrata = 1;
-    const bool hasEnv = P.hasEnv != 0;
-    const int nStrat = P.numLights + (hasEnv ? 1 : 0);
-
-    Pcg rng;
-    vec3 L = v3(0.0f), beta = v3(1.0f), prevX = v3(0.0f), o = v3(0.0f), d = v3(0.0f);
-    float prevPdf = 0.0f;
-    bool specular = true, aovDone = false, alive = false;
-    uint32_t medLo = 0, medHi = 0;
-    int medDepth = 0, depth = 0, s = 0, si = 0;
-    // the pixel's running means stay in registers for this launch's samples
-    // (same operations in the same order as folding each sample in memory)
-    float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
-
-    while (true) {
-      if (!alive) {
-        if (si > 0) {  // the previous sample finished: fold it into the film
-          L = sanitize(L);
-          fAcc = runningMean(fAcc, L, s);
-        }
-        if (si == P.sppThisLaunch) break;
-        s = P.sampleOffset + si;
-        si++;
-        rng = Pcg::make(((uint64_t)pixel << 32) ^ (uint64_t)s, (uint64_t)P.seed);
-        float jx, jy;
-        if (s < nStrata * nStrata) {
-          jx = ((s % nStrata) + rng.rnd()) / nStrata;
-          jy = ((s / nStrata) + rng.rnd()) / nStrata;
-        } else {
-          jx = rng.rnd();
-          jy = rng.rnd();
-        }
-        float u = (px + jx) / P.width;
-        float v = ((P.height - 1 - py) + jy) / P.height;
-        vec3 org = ld3(P.cam.origin);
-        if (P.cam.lensRadius > 0.0f) {
-          vec2 rd = concentricDisk(rng.rnd2()) * P.cam.lensRadius;
-          org = org + ld3(P.cam.u) * rd.x + ld3(P.cam.v) * rd.y;
-        }
-        d = normalize(ld3(P.cam.lowerLeft) + u * ld3(P.cam.horizontal) + v * ld3(P.cam.vertical) - org);
-        o = org;
-        prevX = org;
-        L = v3(0.0f);
-        beta = v3(1.0f);
-        prevPdf = 0.0f;
-        specular = true;
-        aovDone = false;
-        medDepth = 0;
-        depth = 0;
-        alive = true;
-      }
-
-      // ---- one bounce (device/programs.cu:374-558) ----
-      Hit hit;
-      vec3 oObj = v3(0.0f), dObj = v3(0.0f);
-      bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oObj, dObj, wc);
+  const int pixel = py * P.width + px;
+  float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
+  PathState ps;
+  ps.si = 0;
+  ps.medLo = ps.medHi = 0;
+  bool alive = pathBegin(P, ps, px, py);
+  while (alive) {
+    Hit hit;
+    vec3 oObj = v3(0.0f), dObj = v3(0.0f);
+    bool isHit = traceClosest(P, st, ps.o, ps.d, 0.0f, 1e16f, hit, oObj, dObj, wc);
+    rays++;
+    SurfState sf;
+    int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
+    if (r == PN_SHADOW) {
+      vec3 so, wi;
+      float tmaxS;
+      shadowRay(sf, so, wi, tmaxS);
       rays++;
-      alive = false;  // set again at the bottom when the path continues
-
-      if (medDepth > 0) {
-        const DevMaterial& mm = P.materials[medGet(medLo, medHi, min(medDepth, kMedMax) - 1)];
-        vec3 sigma = ld3(mm.absorb);
-        if (maxc(sigma) > 0.0f) beta *= vexp(-sigma * (isHit ? hit.t : 1e4f));
-      }
-      if (P.numFlames > 0) {
-        vec3 trans = v3(1.0f);
-        vec3 Lv = marchFlames(P, o, d, isHit ? hit.t : 1e16f, rng, trans);
-        vec3 cv = beta * Lv;
-        if (depth >= 1 && P.clampVal > 0.0f) cv = vmin(cv, v3(P.clampVal));
-        L += cv;
-        beta *= trans;
-      }
-      if (!isHit) {
-        vec3 bg = evalBackground(P, d);
-        float w = 1.0f;
-        if (hasEnv && !specular) {
-          float pdfE = envPdf(P, d) / nStrat;
-          w = prevPdf / (prevPdf + pdfE);
-        }
-        vec3 c = beta * bg * w;
-        if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
-        L += c;
-        if (!aovDone) {
-          vec3 b = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
-          fAlb = runningMean(fAlb, b, s);
-          fNrm = runningMean(fNrm, v3(0.0f), s);
-        }
-        continue;
-      }
-
-      const DevInstance& in = P.instances[hit.inst];
-      ShadePoint sp = shadePoint(P, in, hit.prim, hit.b1, hit.b2, oObj, dObj, hit.t, true);
-      uint32_t matId = sp.frontface ? (in.mats & 0xFFFFu) : (in.mats >> 16);
-      vec3 ns = sp.frontface ? sp.nFront : -sp.nFront;
-      vec3 x = o + hit.t * d;
-      const DevMaterial& mat = P.materials[matId];
-      vec3 albedo = materialAlbedo(P, mat, sp.u, sp.v);
-      if (mat.kind == MT_WATER && mat.waveAmp > 0.0f)
-        ns = waterNormal(x, ns, -d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
-
-      if (!aovDone) {
-        vec3 nc = vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))};
-        fAlb = runningMean(fAlb, albedo, s);
-        fNrm = runningMean(fNrm, nc, s);
-        aovDone = true;
-      }
-
-      if (mat.kind == MT_EMISSIVE) {
-        if (sp.frontface || mat.twoSided) {
-          vec3 Le = albedo * mat.intensity;
-          float w = 1.0f;
-          if (!specular && in.lightId >= 0 && P.numLights > 0) {
-            float pdfL = lightPdfSolidAngle(P.lights[in.lightId], prevX, x, ns) / nStrat;
-            w = prevPdf / (prevPdf + pdfL);
-          }
-          vec3 c = beta * Le * w;
-          if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
-          L += c;
-        }
-        continue;
-      }
-
-      float etaExt = 1.0f;
-      if (mat.kind == MT_DIELECTRIC || mat.kind == MT_WATER) {
-        int outer = min(sp.frontface ? medDepth : medDepth - 1, kMedMax);
-        if (outer > 0) etaExt = P.materials[medGet(medLo, medHi, outer - 1)].ior;
-      }
-      float etaRel = sp.frontface ? etaExt / mat.ior : mat.ior / etaExt;
-
-      if (nStrat > 0 && !bsdfIsDelta(mat)) {
-        int k = min((int)(rng.rnd() * nStrat), nStrat - 1);
-        LightSample ls = (k < P.numLights) ? sampleLight(P, P.lights[k], x, rng) : sampleEnv(P, rng);
-        float cosS = ls.valid ? dot(ls.wi, ns) : 0.0f;
-        bool transmitNee = cosS < 0.0f && mat.kind == MT_DIELECTRIC;
-        if (ls.valid && (cosS > 0.0f || transmitNee)) {
-          rays++;
-          vec3 so = offsetRay(x, transmitNee ? -ns : ns);
-          float tmaxS = ls.dist * 0.999f;
-          vec3 tr = traceShadow(P, st, so, ls.wi, tmaxS, wc);
-          if (maxc(tr) > 0.0f) {
-            if (P.numFlames > 0 && P.transparentShadows) {
-              int skip = (k < P.numLights) ? P.lights[k].flameId : -1;
-              tr = tr * flameTransmittance(P, so, ls.wi, tmaxS, rng, skip);
-            }
-            float pdfLe = (ls.isDelta ? 1.0f : ls.pdf) / nStrat;
-            vec3 f = bsdfEval(mat, albedo, -d, ls.wi, ns, etaRel);
-            float w = 1.0f;
-            if (!ls.isDelta) {
-              float pdfB = bsdfPdf(mat, -d, ls.wi, ns, etaRel);
-              w = pdfLe / (pdfLe + pdfB);
-            }
-            vec3 c = beta * f * fabsf(cosS) * ls.Li * w / pdfLe;
-            c = c * tr;
-            if (depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
-            L += sanitize(c);
-          }
-        }
-      }
-
-      BsdfSample bs = bsdfSample(mat, albedo, d, ns, sp.frontface, etaExt, rng);
-      if (!bs.valid) continue;
-      if ((mat.kind == MT_WATER || mat.kind == MT_DIELECTRIC) && dot(bs.wi, ns) < 0.0f) {
-        if (sp.frontface) {
-          if (medDepth < kMedMax) medSet(medLo, medHi, medDepth, matId);
-          medDepth++;
-        } else if (medDepth > 0) {
-          medDepth--;
-        }
-      }
-      beta *= bs.weight;
-      specular = bs.isDelta;
-      prevPdf = bs.isDelta ? 0.0f : bs.pdf;
-      prevX = x;
-      vec3 offN = dot(bs.wi, ns) >= 0.0f ? ns : -ns;
-      o = offsetRay(x, offN);
-      d = bs.wi;
-      if (depth >= 4) {
-        float q = clampf(maxc(beta), 0.05f, 0.95f);
-        if (rng.rnd() >= q) continue;
-        beta *= 1.0f / q;
-      }
-      if (maxc(beta) <= 0.0f) continue;
-      depth++;
-      alive = depth < P.maxDepth;
+      vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
+      r = pathOnShadow(P, ps, sf, tr);
     }
-    P.accum[pixel] = fAcc;
-    P.albedo[pixel] = fAlb;
-    P.normal[pixel] = fNrm;
+    if (r == PN_END) {
+      fAcc = runningMean(fAcc, sanitize(ps.L), ps.s);
+      fAlb = runningMean(fAlb, ps.aovA, ps.s);
+      fNrm = runningMean(fNrm, ps.aovN, ps.s);
+      alive = pathBegin(P, ps, px, py);
+    }
   }
+  P.accum[pixel] = fAcc;
+  P.albedo[pixel] = fAlb;
+  P.normal[pixel] = fNrm;
 }
 
 // Wave-level sums of the ray / work counters, one 64-bit atomic per wave.
@@ -1822,6 +1929,314 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(const LaunchParam
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
+}
+
+// ---------------------------------------------------------------------------
+// wavefront variant: the same path stages with the state in HBM planes, one
+// kernel per stage (shade, closest-hit query, shadow query), compacted
+// queues between them, and shading work ordered by material kind.
+// ---------------------------------------------------------------------------
+
+enum WfStage { WF_BEGIN = 0, WF_HIT = 1, WF_SHADOW = 2, WF_DONE = 3 };
+enum WfPlane {
+  WP_O = 0,   // o.xyz, prevPdf
+  WP_D,       // d.xyz, s
+  WP_L,       // L.xyz, si
+  WP_BETA,    // beta.xyz, stage | specular << 2 | depth << 8 | medDepth << 16
+  WP_PREVX,   // prevX.xyz, medLo
+  WP_AOVA,    // aovA.xyz, medHi
+  WP_AOVN,    // aovN.xyz, -
+  WP_RNG,     // rng state lo, hi, -, -
+  WP_RES0,    // closest: t, inst, prim, b1 | shadow: transmittance.xyz, -
+  WP_RES1,    // closest: b2, -, -, -
+  WP_SX,      // surface x.xyz, etaRel
+  WP_SNS,     // ns.xyz, etaExt
+  WP_SALB,    // albedo.xyz, matId | frontface << 16 | isDelta << 17 | transmitNee << 18
+  WP_SWI,     // wi.xyz, pdf
+  WP_SLI,     // Li.xyz, dist
+  WP_SMISC,   // cosS, k, -, -
+};
+constexpr int kKeyMiss = 0, kKeyHit = 1, kKeyShadow = 8;  // + material kind
+
+DEV float4 wfLd(const WfParams& W, int plane, int slot) { return W.state[(size_t)plane * W.numSlots + slot]; }
+DEV void wfSt(const WfParams& W, int plane, int slot, float4 v) { W.state[(size_t)plane * W.numSlots + slot] = v; }
+DEV float4 f4(vec3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+DEV float4 f4i(vec3 v, uint32_t w) { return make_float4(v.x, v.y, v.z, __uint_as_float(w)); }
+DEV uint32_t wbits(float4 v) { return __float_as_uint(v.w); }
+
+DEV int wfLoadPath(const WfParams& W, int slot, PathState& ps) {
+  float4 a = wfLd(W, WP_O, slot), b = wfLd(W, WP_D, slot), c = wfLd(W, WP_L, slot), e = wfLd(W, WP_BETA, slot);
+  float4 f = wfLd(W, WP_PREVX, slot), g = wfLd(W, WP_AOVA, slot), h = wfLd(W, WP_AOVN, slot);
+  float4 r = wfLd(W, WP_RNG, slot);
+  ps.o = f4xyz(a);
+  ps.prevPdf = a.w;
+  ps.d = f4xyz(b);
+  ps.s = (int)wbits(b);
+  ps.L = f4xyz(c);
+  ps.si = (int)wbits(c);
+  ps.beta = f4xyz(e);
+  uint32_t fl = wbits(e);
+  ps.specular = (fl >> 2) & 1u;
+  ps.depth = (int)((fl >> 8) & 0xFFu);
+  ps.medDepth = (int)((fl >> 16) & 0xFFu);
+  ps.prevX = f4xyz(f);
+  ps.medLo = wbits(f);
+  ps.aovA = f4xyz(g);
+  ps.medHi = wbits(g);
+  ps.aovN = f4xyz(h);
+  ps.rngState = (uint64_t)__float_as_uint(r.x) | ((uint64_t)__float_as_uint(r.y) << 32);
+  return (int)(fl & 3u);
+}
+
+DEV void wfStorePath(const WfParams& W, int slot, const PathState& ps, int stage) {
+  uint32_t fl = (uint32_t)stage | ((ps.specular ? 1u : 0u) << 2) | ((uint32_t)ps.depth << 8) |
+                ((uint32_t)ps.medDepth << 16);
+  wfSt(W, WP_O, slot, f4(ps.o, ps.prevPdf));
+  wfSt(W, WP_D, slot, f4i(ps.d, (uint32_t)ps.s));
+  wfSt(W, WP_L, slot, f4i(ps.L, (uint32_t)ps.si));
+  wfSt(W, WP_BETA, slot, f4i(ps.beta, fl));
+  wfSt(W, WP_PREVX, slot, f4i(ps.prevX, ps.medLo));
+  wfSt(W, WP_AOVA, slot, f4i(ps.aovA, ps.medHi));
+  wfSt(W, WP_AOVN, slot, f4(ps.aovN, 0.0f));
+  wfSt(W, WP_RNG, slot,
+       make_float4(__uint_as_float((uint32_t)ps.rngState), __uint_as_float((uint32_t)(ps.rngState >> 32)), 0.0f,
+                   0.0f));
+}
+
+DEV void wfLoadSurf(const WfParams& W, int slot, SurfState& sf) {
+  float4 a = wfLd(W, WP_SX, slot), b = wfLd(W, WP_SNS, slot), c = wfLd(W, WP_SALB, slot);
+  float4 d = wfLd(W, WP_SWI, slot), e = wfLd(W, WP_SLI, slot), f = wfLd(W, WP_SMISC, slot);
+  sf.x = f4xyz(a);
+  sf.etaRel = a.w;
+  sf.ns = f4xyz(b);
+  sf.etaExt = b.w;
+  sf.albedo = f4xyz(c);
+  uint32_t m = wbits(c);
+  sf.matId = m & 0xFFFFu;
+  sf.frontface = (m >> 16) & 1u;
+  sf.isDelta = (m >> 17) & 1u;
+  sf.transmitNee = (m >> 18) & 1u;
+  sf.wi = f4xyz(d);
+  sf.pdf = d.w;
+  sf.Li = f4xyz(e);
+  sf.dist = e.w;
+  sf.cosS = f.x;
+  sf.k = (int)__float_as_uint(f.y);
+}
+
+DEV void wfStoreSurf(const WfParams& W, int slot, const SurfState& sf) {
+  uint32_t m = sf.matId | ((sf.frontface ? 1u : 0u) << 16) | ((sf.isDelta ? 1u : 0u) << 17) |
+               ((sf.transmitNee ? 1u : 0u) << 18);
+  wfSt(W, WP_SX, slot, f4(sf.x, sf.etaRel));
+  wfSt(W, WP_SNS, slot, f4(sf.ns, sf.etaExt));
+  wfSt(W, WP_SALB, slot, f4i(sf.albedo, m));
+  wfSt(W, WP_SWI, slot, f4(sf.wi, sf.pdf));
+  wfSt(W, WP_SLI, slot, f4(sf.Li, sf.dist));
+  wfSt(W, WP_SMISC, slot, make_float4(sf.cosS, __uint_as_float((uint32_t)sf.k), 0.0f, 0.0f));
+}
+
+// Appends `value` for every lane with `want` to list[*n ...] with one atomic
+// per wave. Every lane of the wave must call it.
+DEV void wavePush(uint32_t* list, uint32_t* n, bool want, uint32_t value) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (want) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// Appends `value` to bucket `key` (kWfKeys lists of `cap` entries), one
+// atomic per distinct key in the wave. Every lane of the wave must call it.
+DEV void wavePushKeyed(uint32_t* buckets, int cap, uint32_t* bucketN, bool want, int key, uint32_t value) {
+  uint64_t pending = __ballot(want);
+  const int lane = threadIdx.x & 63;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lk = __shfl(key, leader, 64);
+    const uint64_t m = pending & __ballot(key == lk);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&bucketN[lk], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if ((m >> lane) & 1ull) buckets[(size_t)lk * cap + base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+    pending &= ~m;
+  }
+}
+
+DEV void wfFold(const LaunchParams& P, int pixel, const PathState& ps) {
+  P.accum[pixel] = runningMean(P.accum[pixel], sanitize(ps.L), ps.s);
+  P.albedo[pixel] = runningMean(P.albedo[pixel], ps.aovA, ps.s);
+  P.normal[pixel] = runningMean(P.normal[pixel], ps.aovN, ps.s);
+}
+
+// Shading pass: every slot of the current buckets (or every slot on the
+// first pass of a launch) resumes its path from the query it waited for and
+// runs until its next query, which it enqueues.
+__global__ __launch_bounds__(kBlock) void k_wf_shade(LaunchParams P, WfParams W) {
+  __shared__ uint32_t s_pre[kWfKeys + 1];
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < kWfKeys; k++) {
+      s_pre[k] = acc;
+      acc += W.first ? 0u : W.cnt->bucketN[k];
+    }
+    s_pre[kWfKeys] = W.first ? (uint32_t)W.numSlots : acc;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[kWfKeys];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < total;
+    int slot = 0, next = WF_DONE;
+    if (valid) {
+      if (W.first) {
+        slot = (int)i;
+      } else {
+        int b = 0;
+        while (b + 1 < kWfKeys && s_pre[b + 1] <= i) b++;
+        slot = (int)W.buckets[(size_t)b * W.numSlots + (i - s_pre[b])];
+      }
+      const int px = slot % P.width, py = slot / P.width;
+      PathState ps;
+      SurfState sf;
+      int r;
+      if (W.first) {
+        ps.si = 0;
+        ps.medLo = ps.medHi = 0;
+        r = pathBegin(P, ps, px, py) ? PN_TRACE : -1;
+      } else {
+        const int stage = wfLoadPath(W, slot, ps);
+        if (stage == WF_HIT) {
+          float4 r0 = wfLd(W, WP_RES0, slot), r1 = wfLd(W, WP_RES1, slot);
+          Hit hit;
+          hit.t = r0.x;
+          hit.inst = (int)__float_as_uint(r0.y);
+          hit.prim = (int)__float_as_uint(r0.z);
+          hit.b1 = r0.w;
+          hit.b2 = r1.x;
+          const bool isHit = hit.inst >= 0;
+          vec3 oObj = v3(0.0f), dObj = v3(0.0f);
+          if (isHit) {
+            const DevInstance& in = P.instances[hit.inst];
+            oObj = xfPoint(in.inv, ps.o);
+            dObj = xfVector(in.inv, ps.d);
+          }
+          r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
+        } else {  // WF_SHADOW
+          wfLoadSurf(W, slot, sf);
+          r = pathOnShadow(P, ps, sf, f4xyz(wfLd(W, WP_RES0, slot)));
+        }
+      }
+      while (r == PN_END) {
+        wfFold(P, slot, ps);
+        r = pathBegin(P, ps, px, py) ? PN_TRACE : -1;
+      }
+      if (r == PN_TRACE) {
+        next = WF_HIT;
+      } else if (r == PN_SHADOW) {
+        next = WF_SHADOW;
+        wfStoreSurf(W, slot, sf);
+      }
+      if (next != WF_DONE) wfStorePath(W, slot, ps, next);
+    }
+    wavePush(W.closestQ, &W.cnt->closestN, valid && next == WF_HIT, (uint32_t)slot);
+    wavePush(W.shadowQ, &W.cnt->shadowN, valid && next == WF_SHADOW, (uint32_t)slot);
+  }
+}
+
+// Closest-hit queries of the queue; each slot goes to the bucket of the
+// material it will shade (its instance's front material kind) or of misses.
+__global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams W) {
+  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  Stack st;
+  st.lds = s_stack + threadIdx.x;
+  st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.stride = P.spillStride;
+  st.sp = 0;
+  WorkCount wc;
+  uint32_t rays = 0;
+  const uint32_t total = W.cnt->closestN;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < total;
+    int slot = 0, key = kKeyMiss;
+    if (valid) {
+      slot = (int)W.closestQ[i];
+      vec3 o = f4xyz(wfLd(W, WP_O, slot)), d = f4xyz(wfLd(W, WP_D, slot));
+      Hit hit;
+      vec3 oo, dd;
+      const bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oo, dd, wc);
+      rays++;
+      const int inst = isHit ? hit.inst : -1;
+      wfSt(W, WP_RES0, slot,
+           make_float4(hit.t, __uint_as_float((uint32_t)inst), __uint_as_float((uint32_t)hit.prim), hit.b1));
+      wfSt(W, WP_RES1, slot, make_float4(hit.b2, 0.0f, 0.0f, 0.0f));
+      if (isHit) {
+        // sort key: the kind of the instance's front material (a face
+        // without a material, 0xFFFF, sorts with the first kind)
+        const uint32_t front = P.instances[inst].mats & 0xFFFFu;
+        key = kKeyHit + (front == 0xFFFFu ? 0 : P.materials[front].kind);
+      }
+    }
+    wavePushKeyed(W.buckets, W.numSlots, W.cnt->bucketN, valid, key, (uint32_t)slot);
+  }
+  flushCounters<false>(W.rayCounter != nullptr, W.rayCounter, nullptr, rays, wc);
+}
+
+// Shadow queries of the queue; each slot goes to the bucket of the material
+// its suspended bounce is shading.
+__global__ __launch_bounds__(kBlock) void k_wf_shadow(LaunchParams P, WfParams W) {
+  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  Stack st;
+  st.lds = s_stack + threadIdx.x;
+  st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.stride = P.spillStride;
+  st.sp = 0;
+  WorkCount wc;
+  uint32_t rays = 0;
+  const uint32_t total = W.cnt->shadowN;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < total;
+    int slot = 0, key = kKeyShadow;
+    if (valid) {
+      slot = (int)W.shadowQ[i];
+      SurfState sf;
+      float4 a = wfLd(W, WP_SX, slot), b = wfLd(W, WP_SNS, slot), c = wfLd(W, WP_SALB, slot);
+      float4 d = wfLd(W, WP_SWI, slot), e = wfLd(W, WP_SLI, slot);
+      sf.x = f4xyz(a);
+      sf.ns = f4xyz(b);
+      sf.transmitNee = (wbits(c) >> 18) & 1u;
+      sf.wi = f4xyz(d);
+      sf.dist = e.w;
+      vec3 so, wi;
+      float tmax;
+      shadowRay(sf, so, wi, tmax);
+      vec3 tr = traceShadow(P, st, so, wi, tmax, wc);
+      rays++;
+      wfSt(W, WP_RES0, slot, f4(tr, 0.0f));
+      key = kKeyShadow + P.materials[wbits(c) & 0xFFFFu].kind;
+    }
+    wavePushKeyed(W.buckets, W.numSlots, W.cnt->bucketN, valid, key, (uint32_t)slot);
+  }
+  flushCounters<false>(W.rayCounter != nullptr, W.rayCounter, nullptr, rays, wc);
+}
+
+// Clears the shading buckets (what = 0) or the two query queues (what = 1);
+// a kernel rather than a memset so a pass can be graph-captured.
+__global__ void k_wf_clear(WfCounters* c, int what) {
+  int t = threadIdx.x;
+  if (what == 0) {
+    if (t < kWfKeys) c->bucketN[t] = 0;
+  } else if (t == 0) {
+    c->closestN = 0;
+    c->shadowN = 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2018,5 +2433,30 @@ int persistentBlocksPerCU(int waves) {
 
 int stackLdsEntries() { return kStackLds; }
 int renderBlockThreads() { return kBlock; }
+
+hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(kBlock), 0, s, P, W);
+  return hipGetLastError();
+}
+hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_wf_closest, dim3(blocks), dim3(kBlock), 0, s, P, W);
+  return hipGetLastError();
+}
+hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_wf_shadow, dim3(blocks), dim3(kBlock), 0, s, P, W);
+  return hipGetLastError();
+}
+hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s) {
+  hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(64), 0, s, c, what);
+  return hipGetLastError();
+}
+// Resident blocks per CU of each wavefront kernel: 0 shade, 1 closest, 2 shadow.
+int wfBlocksPerCU(int which) {
+  int n = 0;
+  hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade, kBlock, 0)
+                 : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_closest, kBlock, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shadow, kBlock, 0);
+  return (e == hipSuccess && n > 0) ? n : 1;
+}
 
 }  // namespace sdk

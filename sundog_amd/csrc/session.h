@@ -34,8 +34,14 @@ hipError_t launchPersistent(const LaunchParams* reqs, const WorkItem* items, int
                             int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
                             unsigned long long* rayCounter, hipStream_t s);
 int persistentBlocksPerCU(int waves);
+hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
+hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
+hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
+hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s);
+int wfBlocksPerCU(int which);
 int stackLdsEntries();
 int renderBlockThreads();
+constexpr int kWfBlock = 256;  // threads per block of the wavefront kernels
 
 #define HIP_OK(expr)                                                                          \
   do {                                                                                        \
@@ -197,6 +203,11 @@ struct sundog_session {
   DevBuf envTex, margCdf, condCdf;
   DevBuf accum, albedo, normal, rayCounter, workCounters, spill;
   size_t spillThreads = 0;
+  // wavefront variant
+  DevBuf wfState, wfClosest, wfShadow, wfBuckets, wfCnt;
+  int wfSlots = 0;
+  uint32_t* wfHostCnt = nullptr;  // pinned copy of the counters
+  int wfIterations = 0;           // queue rounds of the last pass
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double renderMs = 0.0;
   double buildMs = 0.0;
@@ -205,6 +216,7 @@ struct sundog_session {
 
   ~sundog_session() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (wfHostCnt) (void)hipHostFree(wfHostCnt);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -213,7 +225,8 @@ struct sundog_session {
   uint64_t deviceBytes() const {
     const DevBuf* all[] = {&textures, &texels, &materials, &lights, &flames, &instances, &nodes, &tris,
                            &meshes, &positions, &indices, &normals, &uvs, &lightPos, &lightCdf, &envTex,
-                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill};
+                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill,
+                           &wfState, &wfClosest, &wfShadow, &wfBuckets, &wfCnt};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->bytes;
     return b;
@@ -568,12 +581,77 @@ struct sundog_session {
   void renderPass(int offset, int count) {
     if (count <= 0) return;
     if (offset < 0) throw std::runtime_error("render: negative sample offset");
+    if (kernel == SUNDOG_KERNEL_WAVEFRONT) {
+      wavefrontPass(offset, count);
+      return;
+    }
     size_t gx = (P.width + 15) / 16, gy = (P.height + 15) / 16;
     ensureSpill(gx * gy * renderBlockThreads());
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
     HIP_OK(hipEventRecord(ev0, stream));
     HIP_OK(launchRender(P, countWork, waves, stream));
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    renderMs += ms;
+    paths += (uint64_t)P.width * P.height * count;
+  }
+
+  // Wavefront pass: shade -> (closest-hit queries, shadow queries) rounds
+  // until no slot has a query left. The host reads the queue sizes after
+  // each round to know when the pass is over.
+  void wavefrontPass(int offset, int count) {
+    if (countWork) throw std::runtime_error("render: work counting needs the megakernel");
+    const int slots = P.width * P.height;
+    if (slots != wfSlots) {
+      HIP_OK(hipStreamSynchronize(stream));
+      wfState.alloc((size_t)kWfPlanes * slots * sizeof(float4));
+      wfClosest.alloc((size_t)slots * 4);
+      wfShadow.alloc((size_t)slots * 4);
+      wfBuckets.alloc((size_t)kWfKeys * slots * 4);
+      wfCnt.alloc(sizeof(WfCounters));
+      if (!wfHostCnt) HIP_OK(hipHostMalloc((void**)&wfHostCnt, sizeof(WfCounters), hipHostMallocDefault));
+      wfSlots = slots;
+    }
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    int cus = 0;
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int bShade = cus * wfBlocksPerCU(0), bClosest = cus * wfBlocksPerCU(1), bShadow = cus * wfBlocksPerCU(2);
+    ensureSpill((size_t)std::max(bClosest, bShadow) * renderBlockThreads());
+    P.sampleOffset = offset;
+    P.sppThisLaunch = count;
+    WfParams W{};
+    W.state = wfState.as<float4>();
+    W.numSlots = slots;
+    W.closestQ = wfClosest.as<uint32_t>();
+    W.shadowQ = wfShadow.as<uint32_t>();
+    W.buckets = wfBuckets.as<uint32_t>();
+    W.cnt = wfCnt.as<WfCounters>();
+    W.rayCounter = P.countRays ? rayCounter.as<unsigned long long>() : nullptr;
+    HIP_OK(hipEventRecord(ev0, stream));
+    HIP_OK(launchWfClear(W.cnt, 0, stream));
+    HIP_OK(launchWfClear(W.cnt, 1, stream));
+    W.first = 1;
+    HIP_OK(launchWfShade(P, W, std::min(bShade, (slots + kWfBlock - 1) / kWfBlock), stream));
+    W.first = 0;
+    wfIterations = 0;
+    while (true) {
+      HIP_OK(launchWfClear(W.cnt, 0, stream));
+      HIP_OK(launchWfClosest(P, W, bClosest, stream));
+      HIP_OK(launchWfShadow(P, W, bShadow, stream));
+      HIP_OK(launchWfClear(W.cnt, 1, stream));
+      HIP_OK(hipMemcpyAsync(wfHostCnt, W.cnt, sizeof(WfCounters), hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      uint64_t pending = 0;
+      for (int k = 0; k < kWfKeys; k++) pending += ((const WfCounters*)wfHostCnt)->bucketN[k];
+      if (pending == 0) break;
+      wfIterations++;
+      int blocks = (int)std::min<uint64_t>((uint64_t)bShade, (pending + kWfBlock - 1) / kWfBlock);
+      HIP_OK(launchWfShade(P, W, blocks, stream));
+    }
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0.0f;

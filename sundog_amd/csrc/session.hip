@@ -97,7 +97,8 @@ extern "C" int sundog_session_render(sundog_session* s, int sample_offset, int c
 extern "C" int sundog_session_set_kernel(sundog_session* s, int kernel) {
   return guarded([&] {
     if (!s) throw std::runtime_error("session: null handle");
-    if (kernel != SUNDOG_KERNEL_MEGA) throw std::runtime_error("session: unknown kernel variant");
+    if (kernel != SUNDOG_KERNEL_MEGA && kernel != SUNDOG_KERNEL_WAVEFRONT)
+      throw std::runtime_error("session: unknown kernel variant");
     s->kernel = kernel;
     return SUNDOG_OK;
   });

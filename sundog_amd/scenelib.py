@@ -506,6 +506,14 @@ class Session(object):
     def tune(self, key, value):
         _lib.check(self.lib.sundog_session_set_tuning(self.ptr, key.encode(), int(value)), self.lib)
 
+    KERNELS = {"megakernel": 0, "wavefront": 1}
+
+    def set_kernel(self, name):
+        """'megakernel' (one work-item per pixel, inline traces) or
+        'wavefront' (path state in HBM, queued traces, material-sorted
+        shading); both give the same film bits."""
+        _lib.check(self.lib.sundog_session_set_kernel(self.ptr, self.KERNELS[name]), self.lib)
+
     def count_work(self, on=True):
         _lib.check(self.lib.sundog_session_set_count_work(self.ptr, 1 if on else 0), self.lib)
 

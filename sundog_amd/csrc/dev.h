@@ -95,6 +95,7 @@ struct CameraDev {
 
 struct LaunchParams {
   int32_t width, height, sppTotal, sampleOffset, sppThisLaunch, maxDepth;
+  int32_t passSpp;  // samples per pass of a graph-replayed request (k_queue_prep)
   float clampVal;
   uint32_t seed;
   int32_t countRays, transparentShadows;
@@ -143,6 +144,25 @@ struct LaunchParams {
 struct WorkItem {
   int32_t request;
   int32_t tile;  // tile index in row-major order over the request's frame
+};
+
+// The persistent kernel takes the resident scenes' parameters by value (a
+// kernel argument, so every scene pointer is known to be global memory) and
+// reads only these per-request records from device memory.
+constexpr int kMaxScenesPerLaunch = 4;
+
+struct SceneTable {
+  LaunchParams scene[kMaxScenesPerLaunch];
+};
+
+struct RequestDev {
+  int32_t scene;  // index into the launch's SceneTable
+  int32_t width, height, sppTotal, sampleOffset, sppThisLaunch, passSpp;
+  uint32_t seed;
+  float4* accum;
+  float4* albedo;
+  float4* normal;
+  CameraDev cam;
 };
 
 struct QueueState {

@@ -1122,9 +1122,14 @@ DEV void boxes4(const BvhNode4& nd, vec3 org, vec3 inv, float tmin, float tmax, 
 // region (entry-major, one column per thread) beyond.
 // ---------------------------------------------------------------------------
 
+// The LDS column is held as an LDS-address-space pointer: with a generic
+// pointer the compiler merges the LDS and spill branches of pop() into one
+// flat load, which waits on every outstanding global load of the lane.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 struct Stack {
-  uint32_t* lds;   // this lane's column base: lds[i * kBlock]
-  uint32_t* spill; // this lane's spill base: spill[i * stride]
+  lds_u32* lds;     // this lane's column base: lds[i * kBlock]
+  uint32_t* spill;  // this lane's spill base: spill[i * stride]
   int stride;
   int sp;
   DEV void push(uint32_t v) {
@@ -1134,9 +1139,12 @@ struct Stack {
   }
   DEV uint32_t pop() {
     sp--;
-    return sp < kStackLds ? lds[sp * kBlock] : spill[(size_t)(sp - kStackLds) * stride];
+    if (sp < kStackLds) return lds[sp * kBlock];
+    return spill[(size_t)(sp - kStackLds) * stride];
   }
 };
+
+DEV lds_u32* ldsColumn(uint32_t* sharedArray) { return (lds_u32*)(sharedArray + threadIdx.x); }
 
 // Per-query traversal work, for the algorithmic-FLOP count of the roofline.
 struct WorkCount {
@@ -1857,7 +1865,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
   WorkCount wc;
   if (px < P.width && py < P.height) {
     Stack st;
-    st.lds = s_stack + threadIdx.x;
+    st.lds = ldsColumn(s_stack);
     st.spill = P.spill + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
     st.stride = P.spillStride;
     st.sp = 0;
@@ -1878,15 +1886,16 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
 // each request's sample offset for this pass (offset = first + pass * count,
 // clipped to the request's total). Runs as a kernel, never a memset, so it
 // can sit inside a captured graph.
-__global__ void k_queue_prep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq) {
+__global__ void k_queue_prep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq) {
   __shared__ uint32_t s_pass;
   if (threadIdx.x == 0) s_pass = q->pass;
   __syncthreads();
   for (int r = threadIdx.x; r < numReq; r += blockDim.x) {
-    LaunchParams& P = reqs[r];
-    int chunk = P.sppThisLaunch > 0 ? P.sppThisLaunch : 1;
+    RequestDev& R = reqs[r];
+    int chunk = R.passSpp > 0 ? R.passSpp : 1;
     int off = firstSample[r] + (int)s_pass * chunk;
-    P.sampleOffset = off;
+    R.sampleOffset = off;
+    R.sppThisLaunch = max(0, min(chunk, R.sppTotal - off));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1895,15 +1904,19 @@ __global__ void k_queue_prep(QueueState* q, LaunchParams* reqs, const int32_t* f
   }
 }
 
+// reqs[r].sppThisLaunch is already clipped to the request's remaining
+// samples (by the host or k_queue_prep). The request index is wave-uniform,
+// so the scene is picked from the kernel-argument table with scalar loads
+// and only the request's own fields come from device memory.
 template <int kWaves>
-__global__ __launch_bounds__(kBlock, kWaves) void k_persistent(const LaunchParams* reqs, const WorkItem* items,
-                                                               int numItems, QueueState* q, int tileSize,
-                                                               uint32_t* spill, int spillStride,
-                                                               unsigned long long* rayCounter) {
+__global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, const RequestDev* __restrict__ reqs,
+                                                               const WorkItem* __restrict__ items, int numItems,
+                                                               QueueState* q, int tileSize, uint32_t* spill,
+                                                               int spillStride, unsigned long long* rayCounter) {
   __shared__ uint32_t s_stack[kStackLds * kBlock];
   const int lane = threadIdx.x & 63;
   Stack st;
-  st.lds = s_stack + threadIdx.x;
+  st.lds = ldsColumn(s_stack);
   st.spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = spillStride;
   st.sp = 0;
@@ -1913,19 +1926,29 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(const LaunchParam
   while (true) {
     uint32_t item = 0;
     if (lane == 0) item = atomicAdd(&q->next, 1u);
-    item = __shfl(item, 0, 64);
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
     if (item >= (uint32_t)numItems) break;
-    const WorkItem w = items[item];
-    const LaunchParams& P = reqs[w.request];
-    if (P.sampleOffset >= P.sppTotal) continue;  // request already complete
-    LaunchParams Pl = P;
-    Pl.sppThisLaunch = min(P.sppThisLaunch, P.sppTotal - P.sampleOffset);
+    const int req = __builtin_amdgcn_readfirstlane(items[item].request);
+    const int tile = __builtin_amdgcn_readfirstlane(items[item].tile);
+    const RequestDev& R = reqs[req];
+    if (R.sppThisLaunch <= 0) continue;  // request already complete
+    LaunchParams P = S.scene[__builtin_amdgcn_readfirstlane(R.scene)];
+    P.width = R.width;
+    P.height = R.height;
+    P.sppTotal = R.sppTotal;
+    P.sampleOffset = R.sampleOffset;
+    P.sppThisLaunch = R.sppThisLaunch;
+    P.seed = R.seed;
+    P.accum = R.accum;
+    P.albedo = R.albedo;
+    P.normal = R.normal;
+    P.cam = R.cam;
     const int tilesX = (P.width + tileSize - 1) / tileSize;
-    const int tx = (w.tile % tilesX) * tileSize, ty = (w.tile / tilesX) * tileSize;
+    const int tx = (tile % tilesX) * tileSize, ty = (tile / tilesX) * tileSize;
     for (int base = 0; base < pxPerTile; base += 64) {
       int k = base + lane;
       int px = tx + (k % tileSize), py = ty + (k / tileSize);
-      if (k < pxPerTile && px < P.width && py < P.height) renderPixel(Pl, px, py, st, wc, rays);
+      if (k < pxPerTile && px < P.width && py < P.height) renderPixel(P, px, py, st, wc, rays);
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
@@ -2152,7 +2175,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(LaunchParams P, WfParams W)
 __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams W) {
   __shared__ uint32_t s_stack[kStackLds * kBlock];
   Stack st;
-  st.lds = s_stack + threadIdx.x;
+  st.lds = ldsColumn(s_stack);
   st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
@@ -2192,7 +2215,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams 
 __global__ __launch_bounds__(kBlock) void k_wf_shadow(LaunchParams P, WfParams W) {
   __shared__ uint32_t s_stack[kStackLds * kBlock];
   Stack st;
-  st.lds = s_stack + threadIdx.x;
+  st.lds = ldsColumn(s_stack);
   st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
@@ -2249,7 +2272,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(LaunchParams P, const float* r
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   Stack st;
-  st.lds = s_stack + threadIdx.x;
+  st.lds = ldsColumn(s_stack);
   st.spill = P.spill + i;
   st.stride = P.spillStride;
   st.sp = 0;
@@ -2327,7 +2350,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(LaunchParams P, const float* 
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   Stack st;
-  st.lds = s_stack + threadIdx.x;
+  st.lds = ldsColumn(s_stack);
   st.spill = P.spill + i;
   st.stride = P.spillStride;
   st.sp = 0;
@@ -2397,23 +2420,23 @@ hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launchQueuePrep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq,
+hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_queue_prep, dim3(1), dim3(256), 0, s, q, reqs, firstSample, numReq);
   return hipGetLastError();
 }
 
-hipError_t launchPersistent(const LaunchParams* reqs, const WorkItem* items, int numItems, QueueState* q,
-                            int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
+hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
+                            QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
                             unsigned long long* rayCounter, hipStream_t s) {
   if (waves >= 4)
-    hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+    hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
                        spill, spillStride, rayCounter);
   else if (waves == 3)
-    hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+    hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
                        spill, spillStride, rayCounter);
   else
-    hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), 0, s, reqs, items, numItems, q, tileSize,
+    hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
                        spill, spillStride, rayCounter);
   return hipGetLastError();
 }

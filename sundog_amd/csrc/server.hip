@@ -155,36 +155,84 @@ struct sundog_server {
     spill.alloc(std::max<size_t>(16, (size_t)rows * threads * sizeof(uint32_t)));
   }
 
-  // One batch: the next pass of every active request. Called by the worker
-  // with the lock released; only the worker touches `active`'s films.
+  RequestDev requestRecord(const ServerRequest& r, int sceneSlot) const {
+    const LaunchParams P = requestParams(r);
+    RequestDev d{};
+    d.scene = sceneSlot;
+    d.width = r.width;
+    d.height = r.height;
+    d.sppTotal = r.spp;
+    d.sampleOffset = r.done;
+    d.sppThisLaunch = std::min(passSpp, r.spp - r.done);
+    d.passSpp = passSpp;
+    d.seed = r.seed;
+    d.accum = P.accum;
+    d.albedo = P.albedo;
+    d.normal = P.normal;
+    d.cam = P.cam;
+    return d;
+  }
+
+  // One batch: the next pass of every active request. Requests are grouped
+  // so that one persistent launch references at most kMaxScenesPerLaunch
+  // scenes (their parameters travel as a kernel argument). Called by the
+  // worker with the lock released; only the worker touches `active`'s films.
   void runBatch(std::vector<ServerRequest*>& batch) {
-    std::vector<LaunchParams> reqs(batch.size());
+    std::vector<ServerRequest*> order(batch);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const ServerRequest* a, const ServerRequest* b) { return a->scene < b->scene; });
+    struct Group {
+      SceneTable table;
+      int numScenes = 0;
+      size_t reqBegin = 0, reqEnd = 0, itemBegin = 0, itemEnd = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<RequestDev> reqs;
     std::vector<WorkItem> work;
-    for (size_t i = 0; i < batch.size(); i++) {
-      ServerRequest& r = *batch[i];
-      reqs[i] = requestParams(r);
-      reqs[i].sampleOffset = r.done;
-      reqs[i].sppThisLaunch = std::min(passSpp, r.spp - r.done);
+    int lastScene = -1;
+    for (ServerRequest* rp : order) {
+      ServerRequest& r = *rp;
+      if (groups.empty() || (r.scene != lastScene && groups.back().numScenes == kMaxScenesPerLaunch)) {
+        groups.emplace_back();
+        groups.back().reqBegin = reqs.size();
+        groups.back().itemBegin = work.size();
+        lastScene = -1;
+      }
+      Group& g = groups.back();
+      if (r.scene != lastScene) {
+        LaunchParams P = scenes[r.scene]->P;
+        P.countRays = 1;
+        P.rayCounter = rayCounter.as<unsigned long long>();
+        P.workCounters = nullptr;
+        g.table.scene[g.numScenes++] = P;
+        lastScene = r.scene;
+      }
+      int local = (int)(reqs.size() - g.reqBegin);
+      reqs.push_back(requestRecord(r, g.numScenes - 1));
       int tx = (r.width + tileSize - 1) / tileSize, ty = (r.height + tileSize - 1) / tileSize;
-      for (int t = 0; t < tx * ty; t++) work.push_back(WorkItem{(int32_t)i, t});
+      for (int t = 0; t < tx * ty; t++) work.push_back(WorkItem{(int32_t)local, t});
+      g.reqEnd = reqs.size();
+      g.itemEnd = work.size();
     }
     if (reqs.size() > reqCap) {
       reqCap = reqs.size() * 2;
-      dReqs.alloc(reqCap * sizeof(LaunchParams));
+      dReqs.alloc(reqCap * sizeof(RequestDev));
     }
     if (work.size() > itemCap) {
       itemCap = work.size() * 2;
       dItems.alloc(itemCap * sizeof(WorkItem));
     }
-    HIP_OK(hipMemcpyAsync(dReqs.p, reqs.data(), reqs.size() * sizeof(LaunchParams), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(dReqs.p, reqs.data(), reqs.size() * sizeof(RequestDev), hipMemcpyHostToDevice, stream));
     HIP_OK(hipMemcpyAsync(dItems.p, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
-    HIP_OK(hipMemsetAsync(dQueue.p, 0, sizeof(QueueState), stream));
     int blocks = numCUs * blocksPerCU;
     size_t threads = (size_t)blocks * renderBlockThreads();
     HIP_OK(hipEventRecord(ev0, stream));
-    HIP_OK(launchPersistent(dReqs.as<LaunchParams>(), dItems.as<WorkItem>(), (int)work.size(),
-                            dQueue.as<QueueState>(), tileSize, waves, blocks, spill.as<uint32_t>(), (int)threads,
-                            rayCounter.as<unsigned long long>(), stream));
+    for (const Group& g : groups) {
+      HIP_OK(hipMemsetAsync(dQueue.p, 0, sizeof(QueueState), stream));
+      HIP_OK(launchPersistent(g.table, dReqs.as<RequestDev>() + g.reqBegin, dItems.as<WorkItem>() + g.itemBegin,
+                              (int)(g.itemEnd - g.itemBegin), dQueue.as<QueueState>(), tileSize, waves, blocks,
+                              spill.as<uint32_t>(), (int)threads, rayCounter.as<unsigned long long>(), stream));
+    }
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0;
@@ -192,10 +240,11 @@ struct sundog_server {
     kernelMs += ms;
     batches++;
     items += work.size();
-    for (size_t i = 0; i < batch.size(); i++) {
-      ServerRequest& r = *batch[i];
-      paths += (uint64_t)r.width * r.height * reqs[i].sppThisLaunch;
-      r.done += reqs[i].sppThisLaunch;
+    for (ServerRequest* rp : batch) {
+      ServerRequest& r = *rp;
+      int n = std::min(passSpp, r.spp - r.done);
+      paths += (uint64_t)r.width * r.height * n;
+      r.done += n;
     }
   }
 

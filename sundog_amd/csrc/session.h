@@ -28,10 +28,10 @@ hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool bru
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
 hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s);
-hipError_t launchQueuePrep(QueueState* q, LaunchParams* reqs, const int32_t* firstSample, int numReq,
+hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq,
                            hipStream_t s);
-hipError_t launchPersistent(const LaunchParams* reqs, const WorkItem* items, int numItems, QueueState* q,
-                            int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
+hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
+                            QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
                             unsigned long long* rayCounter, hipStream_t s);
 int persistentBlocksPerCU(int waves);
 hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from . import configs
-from .scenelib import Scene, translate, scale, rotate_x, rotate_y, rotate_z
+from .scenelib import Scene, rel_rmse, translate, scale, rotate_x, rotate_y, rotate_z
 
 
 def _fp(a):
@@ -122,12 +122,6 @@ def make_input(beauty, albedo, normal):
     """Films (h, w, >=3) -> network input (h, w, 9)."""
     return np.concatenate([np.asarray(beauty, np.float32)[..., :3], np.asarray(albedo, np.float32)[..., :3],
                            np.asarray(normal, np.float32)[..., :3]], axis=-1)
-
-
-def rel_rmse(a, b):
-    a = np.asarray(a, np.float64)[..., :3]
-    b = np.asarray(b, np.float64)[..., :3]
-    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-30)))
 
 
 # ---------------------------------------------------------------------------

@@ -17,7 +17,7 @@ namespace sdk {
 #define DEV __device__ __forceinline__
 
 constexpr int kBlock = 256;
-constexpr int kStackLds = 24;
+constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) entries per lane in LDS
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -1122,29 +1122,33 @@ DEV void boxes4(const BvhNode4& nd, vec3 org, vec3 inv, float tmin, float tmax, 
 // region (entry-major, one column per thread) beyond.
 // ---------------------------------------------------------------------------
 
-// The LDS column is held as an LDS-address-space pointer: with a generic
-// pointer the compiler merges the LDS and spill branches of pop() into one
-// flat load, which waits on every outstanding global load of the lane.
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
+// Stack entries are (node reference, entry distance) pairs: a popped entry
+// whose box starts beyond the current closest hit is dropped without
+// fetching its node. The LDS column is held as an LDS-address-space pointer:
+// with a generic pointer the compiler merges the LDS and spill branches of
+// pop() into one flat load, which waits on every outstanding global load of
+// the lane.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 
 struct Stack {
-  lds_u32* lds;     // this lane's column base: lds[i * kBlock]
-  uint32_t* spill;  // this lane's spill base: spill[i * stride]
+  lds_u64* lds;     // this lane's column base: lds[i * kBlock]
+  uint64_t* spill;  // this lane's spill base: spill[i * stride]
   int stride;
   int sp;
-  DEV void push(uint32_t v) {
-    if (sp < kStackLds) lds[sp * kBlock] = v;
-    else spill[(size_t)(sp - kStackLds) * stride] = v;
+  DEV void push(uint32_t ref, float t) {
+    uint64_t e = (uint64_t)ref | ((uint64_t)__float_as_uint(t) << 32);
+    if (sp < kStackLds) lds[sp * kBlock] = e;
+    else spill[(size_t)(sp - kStackLds) * stride] = e;
     sp++;
   }
-  DEV uint32_t pop() {
+  DEV uint2 pop() {
     sp--;
-    if (sp < kStackLds) return lds[sp * kBlock];
-    return spill[(size_t)(sp - kStackLds) * stride];
+    uint64_t e = sp < kStackLds ? lds[sp * kBlock] : spill[(size_t)(sp - kStackLds) * stride];
+    return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
   }
 };
 
-DEV lds_u32* ldsColumn(uint32_t* sharedArray) { return (lds_u32*)(sharedArray + threadIdx.x); }
+DEV lds_u64* ldsColumn(uint64_t* sharedArray) { return (lds_u64*)(sharedArray + threadIdx.x); }
 
 // Per-query traversal work, for the algorithmic-FLOP count of the roofline.
 struct WorkCount {
@@ -1283,11 +1287,15 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
   }
       SD_CSWAP(0, 1) SD_CSWAP(2, 3) SD_CSWAP(0, 2) SD_CSWAP(1, 3) SD_CSWAP(1, 2)
 #undef SD_CSWAP
-      int nh = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++) nh += (dist[i] < __builtin_inff() && ref[i] != kRefEmpty) ? 1 : 0;
-      if (nh > 0) {
-        for (int i = nh - 1; i >= 1; i--) st.push(ref[i]);
+      // hit children are sorted first (misses and empty slots carry +inf);
+      // the farther ones go on the stack far-to-near with their distances
+      const bool h1 = dist[1] < __builtin_inff() && ref[1] != kRefEmpty;
+      const bool h2 = dist[2] < __builtin_inff() && ref[2] != kRefEmpty;
+      const bool h3 = dist[3] < __builtin_inff() && ref[3] != kRefEmpty;
+      if (h3) st.push(ref[3], dist[3]);
+      if (h2) st.push(ref[2], dist[2]);
+      if (h1) st.push(ref[1], dist[1]);
+      if (dist[0] < __builtin_inff() && ref[0] != kRefEmpty) {
         cur = ref[0];
         continue;
       }
@@ -1297,7 +1305,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
       vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
       wc.insts++;
       if (in.geomKind == GK_MESH) {
-        st.push(kRefMarker);
+        st.push(kRefMarker, -__builtin_inff());
         obj = makeRay(oo, dd);
         rt = rayTriSetup(dd);
         curInst = ii;
@@ -1343,12 +1351,17 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
         }
       }
     }
-    // pop
+    // pop; an entry whose box starts beyond the current closest hit (the
+    // same widened comparison as the box test) cannot hold a closer hit
     while (true) {
       if (st.sp == base) return found;
-      cur = st.pop();
-      if (cur != kRefMarker) break;
-      inBlas = false;
+      uint2 e = st.pop();
+      cur = e.x;
+      if (cur == kRefMarker) {
+        inBlas = false;
+        continue;
+      }
+      if (__uint_as_float(e.y) <= hit.t * 1.0000004f) break;
     }
   }
 }
@@ -1383,7 +1396,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
       for (int i = 0; i < 4; i++) {
         if (dist[i] < __builtin_inff() && ref[i] != kRefEmpty) {
           if (nh == 0) next = ref[i];
-          else st.push(ref[i]);
+          else st.push(ref[i], 0.0f);
           nh++;
         }
       }
@@ -1399,7 +1412,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
         vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
         wc.insts++;
         if (in.geomKind == GK_MESH) {
-          st.push(kRefMarker);
+          st.push(kRefMarker, 0.0f);
           obj = makeRay(oo, dd);
           rt = rayTriSetup(dd);
           curInst = first;
@@ -1471,7 +1484,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
         cur = 0xFFFFFFFFu;
         break;
       }
-      cur = st.pop();
+      cur = st.pop().x;
       if (cur != kRefMarker) break;
       inBlas = false;
     }
@@ -1858,7 +1871,7 @@ DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned 
 // the kernel without spills; 3 and 4 trade scratch spills for occupancy).
 template <bool kCountWork, int kWaves>
 __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   int px, py;
   pixelOf(px, py);
   uint32_t rays = 0;
@@ -1866,7 +1879,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
   if (px < P.width && py < P.height) {
     Stack st;
     st.lds = ldsColumn(s_stack);
-    st.spill = P.spill + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+    st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
     st.stride = P.spillStride;
     st.sp = 0;
     renderPixel(P, px, py, st, wc, rays);
@@ -1913,11 +1926,11 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
                                                                const WorkItem* __restrict__ items, int numItems,
                                                                QueueState* q, int tileSize, uint32_t* spill,
                                                                int spillStride, unsigned long long* rayCounter) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   const int lane = threadIdx.x & 63;
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.spill = reinterpret_cast<uint64_t*>(spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = spillStride;
   st.sp = 0;
   WorkCount wc;
@@ -2173,10 +2186,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(LaunchParams P, WfParams W)
 // Closest-hit queries of the queue; each slot goes to the bucket of the
 // material it will shade (its instance's front material kind) or of misses.
 __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams W) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
   WorkCount wc;
@@ -2213,10 +2226,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams 
 // Shadow queries of the queue; each slot goes to the bucket of the material
 // its suspended bounce is shading.
 __global__ __launch_bounds__(kBlock) void k_wf_shadow(LaunchParams P, WfParams W) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = P.spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
   WorkCount wc;
@@ -2268,12 +2281,12 @@ __global__ void k_wf_clear(WfCounters* c, int what) {
 
 __global__ __launch_bounds__(kBlock) void k_trace(LaunchParams P, const float* rays, int n, float* tOut,
                                                  int* instOut, int* primOut, int* counters) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = P.spill + i;
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) + i;
   st.stride = P.spillStride;
   st.sp = 0;
   const float* r = rays + 8 * (size_t)i;
@@ -2346,12 +2359,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_brute(LaunchParams P, int numI
 }
 
 __global__ __launch_bounds__(kBlock) void k_shadow(LaunchParams P, const float* rays, int n, float* trOut) {
-  __shared__ uint32_t s_stack[kStackLds * kBlock];
+  __shared__ uint64_t s_stack[kStackLds * kBlock];
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = P.spill + i;
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) + i;
   st.stride = P.spillStride;
   st.sp = 0;
   const float* r = rays + 8 * (size_t)i;

@@ -152,7 +152,7 @@ struct sundog_server {
     blocksPerCU = persistentBlocksPerCU(waves);
     size_t threads = (size_t)numCUs * blocksPerCU * renderBlockThreads();
     int rows = std::max(0, spillRows);
-    spill.alloc(std::max<size_t>(16, (size_t)rows * threads * sizeof(uint32_t)));
+    spill.alloc(std::max<size_t>(16, (size_t)rows * threads * kStackEntryBytes));
   }
 
   RequestDev requestRecord(const ServerRequest& r, int sceneSlot) const {
@@ -564,7 +564,7 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     sundog_session& sc = *s->scenes[scene_id];
     size_t threads = (size_t)((width + 15) / 16) * ((height + 15) / 16) * renderBlockThreads();
     DevBuf sp;
-    sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * 4));
+    sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * kStackEntryBytes));
     P.spill = sp.as<uint32_t>();
     P.spillStride = (int)threads;
     HIP_OK(hipStreamSynchronize(s->stream));

@@ -42,6 +42,7 @@ int wfBlocksPerCU(int which);
 int stackLdsEntries();
 int renderBlockThreads();
 constexpr int kWfBlock = 256;  // threads per block of the wavefront kernels
+constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference + entry distance
 
 #define HIP_OK(expr)                                                                          \
   do {                                                                                        \
@@ -242,7 +243,7 @@ struct sundog_session {
     }
     if (threads > spillThreads) {
       HIP_OK(hipStreamSynchronize(stream));
-      spill.alloc((size_t)spillRows * threads * sizeof(uint32_t));
+      spill.alloc((size_t)spillRows * threads * kStackEntryBytes);
       spillThreads = threads;
     }
     P.spill = spill.as<uint32_t>();

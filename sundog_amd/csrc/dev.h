@@ -182,9 +182,7 @@ constexpr int kWfKeys = 16;
 constexpr int kWfPlanes = 16;
 
 struct WfCounters {
-  uint32_t closestN, shadowN;
-  uint32_t closestNext;  // next closest-hit query to hand to a lane
-  uint32_t pad;
+  uint32_t closestN, shadowN, pad0, pad1;
   uint32_t bucketN[kWfKeys];
 };
 

@@ -1255,148 +1255,115 @@ DEV RayState makeRay(vec3 o, vec3 d) {
 }
 
 // Closest hit with masking. Ties in t resolve to the smaller (instance,
-// primitive) so the answer does not depend on traversal order. The query is
-// resumable: closestBegin sets it up and each closestStep visits one node or
-// one leaf, so a kernel can interleave the queries of many rays in one lane.
-struct ClosestQuery {
-  RayState world, obj;
-  RayTri rt;
-  Hit hit;
-  float tmin;
-  uint32_t cur;
-  int curInst;
-  int base;  // stack depth at the start of the query
-  bool inBlas, found;
-  vec3 oObj, dObj;  // object-space ray of the instance of the current best hit
-};
-
-DEV void closestBegin(const LaunchParams& P, const Stack& st, ClosestQuery& q, vec3 o, vec3 d, float tmin,
-                      float tmax) {
-  q.hit.t = tmax;
-  q.hit.inst = 0x7fffffff;
-  q.hit.prim = 0x7fffffff;
-  q.hit.b1 = q.hit.b2 = 0.0f;
-  q.found = false;
-  q.world = makeRay(o, d);
-  q.obj = q.world;
-  q.rt = rayTriSetup(d);
-  q.curInst = -1;
-  q.inBlas = false;
-  q.base = st.sp;
-  q.tmin = tmin;
-  q.cur = P.tlasRoot;
-  q.oObj = q.dObj = v3(0.0f);
-}
-
-// Visits q.cur; returns false once the query is complete.
-DEV bool closestStep(const LaunchParams& P, Stack& st, ClosestQuery& q, WorkCount& wc) {
-  const uint32_t cur = q.cur;
-  if (!(cur & kRefLeaf)) {
-    const RayState& r = q.inBlas ? q.obj : q.world;
-    wc.nodes++;
-    float dist[4];
-    boxes4(P.nodes[cur], r.o, r.inv, q.tmin, q.hit.t, dist);
-    const int32_t* refs = &P.nodes[cur].child[0];
-    int4 rc = *reinterpret_cast<const int4*>(refs);
-    uint32_t ref[4] = {(uint32_t)rc.x, (uint32_t)rc.y, (uint32_t)rc.z, (uint32_t)rc.w};
-    // sort 4 (dist, ref) pairs ascending
+// primitive) so the answer does not depend on traversal order.
+DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
+                      vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
+  hit.t = tmax;
+  hit.inst = 0x7fffffff;
+  hit.prim = 0x7fffffff;
+  hit.b1 = hit.b2 = 0.0f;
+  bool found = false;
+  RayState world = makeRay(o, d);
+  RayState obj = world;
+  RayTri rt = rayTriSetup(d);
+  int curInst = -1;
+  bool inBlas = false;
+  const int base = st.sp;
+  uint32_t cur = P.tlasRoot;
+  while (true) {
+    if (!(cur & kRefLeaf)) {
+      const RayState& r = inBlas ? obj : world;
+      wc.nodes++;
+      float dist[4];
+      boxes4(P.nodes[cur], r.o, r.inv, tmin, hit.t, dist);
+      const int32_t* refs = &P.nodes[cur].child[0];
+      int4 rc = *reinterpret_cast<const int4*>(refs);
+      uint32_t ref[4] = {(uint32_t)rc.x, (uint32_t)rc.y, (uint32_t)rc.z, (uint32_t)rc.w};
+      // sort 4 (dist, ref) pairs ascending
 #define SD_CSWAP(a, b)                                   \
   if (dist[b] < dist[a]) {                               \
     float td = dist[a]; dist[a] = dist[b]; dist[b] = td; \
     uint32_t tr = ref[a]; ref[a] = ref[b]; ref[b] = tr;  \
   }
-    SD_CSWAP(0, 1) SD_CSWAP(2, 3) SD_CSWAP(0, 2) SD_CSWAP(1, 3) SD_CSWAP(1, 2)
+      SD_CSWAP(0, 1) SD_CSWAP(2, 3) SD_CSWAP(0, 2) SD_CSWAP(1, 3) SD_CSWAP(1, 2)
 #undef SD_CSWAP
-    // hit children are sorted first (misses and empty slots carry +inf);
-    // the farther ones go on the stack far-to-near with their distances
-    const bool h1 = dist[1] < __builtin_inff() && ref[1] != kRefEmpty;
-    const bool h2 = dist[2] < __builtin_inff() && ref[2] != kRefEmpty;
-    const bool h3 = dist[3] < __builtin_inff() && ref[3] != kRefEmpty;
-    if (h3) st.push(ref[3], dist[3]);
-    if (h2) st.push(ref[2], dist[2]);
-    if (h1) st.push(ref[1], dist[1]);
-    if (dist[0] < __builtin_inff() && ref[0] != kRefEmpty) {
-      q.cur = ref[0];
-      return true;
-    }
-  } else if (!q.inBlas) {
-    int ii = (int)(cur & 0x07FFFFFFu);
-    const DevInstance& in = P.instances[ii];
-    vec3 oo = xfPoint(in.inv, q.world.o), dd = xfVector(in.inv, q.world.d);
-    wc.insts++;
-    if (in.geomKind == GK_MESH) {
-      st.push(kRefMarker, -__builtin_inff());
-      q.obj = makeRay(oo, dd);
-      q.rt = rayTriSetup(dd);
-      q.curInst = ii;
-      q.inBlas = true;
-      q.cur = in.blasRoot;
-      return true;
-    }
-    wc.quads++;
-    float ts[2];
-    int n = quadricRoots(in.geomKind, oo, dd, q.tmin, candBound(q.hit.t), ts);
-    for (int k = 0; k < n; k++) {
-      if (closer(ts[k], ii, k, q.hit) && maskAccepts(P, in, k, 0.0f, 0.0f, oo, dd, ts[k])) {
-        q.hit.t = ts[k];
-        q.hit.inst = ii;
-        q.hit.prim = k;
-        q.oObj = oo;
-        q.dObj = dd;
-        q.found = true;
-        break;  // roots ascend: the first accepted one is the closest
+      // hit children are sorted first (misses and empty slots carry +inf);
+      // the farther ones go on the stack far-to-near with their distances
+      const bool h1 = dist[1] < __builtin_inff() && ref[1] != kRefEmpty;
+      const bool h2 = dist[2] < __builtin_inff() && ref[2] != kRefEmpty;
+      const bool h3 = dist[3] < __builtin_inff() && ref[3] != kRefEmpty;
+      if (h3) st.push(ref[3], dist[3]);
+      if (h2) st.push(ref[2], dist[2]);
+      if (h1) st.push(ref[1], dist[1]);
+      if (dist[0] < __builtin_inff() && ref[0] != kRefEmpty) {
+        cur = ref[0];
+        continue;
       }
-    }
-  } else {
-    int first = (int)(cur & 0x07FFFFFFu);
-    int cnt = (int)((cur >> 27) & 0xFu) + 1;
-    const DevInstance& in = P.instances[q.curInst];
-    for (int k = 0; k < cnt; k++) {
-      wc.tris++;
-      const float4* tv = P.tris + 3 * (size_t)(first + k);
-      float4 a = tv[0], b = tv[1], c = tv[2];
-      float t, b1, b2;
-      if (triHit(q.rt, q.obj.o, f4xyz(a), f4xyz(b), f4xyz(c), q.tmin, candBound(q.hit.t), t, b1, b2)) {
-        int prim = __float_as_int(a.w);
-        if (closer(t, q.curInst, prim, q.hit) && maskAccepts(P, in, prim, b1, b2, q.obj.o, q.obj.d, t)) {
-          q.hit.t = t;
-          q.hit.inst = q.curInst;
-          q.hit.prim = prim;
-          q.hit.b1 = b1;
-          q.hit.b2 = b2;
-          q.oObj = q.obj.o;
-          q.dObj = q.obj.d;
-          q.found = true;
+    } else if (!inBlas) {
+      int ii = (int)(cur & 0x07FFFFFFu);
+      const DevInstance& in = P.instances[ii];
+      vec3 oo = xfPoint(in.inv, o), dd = xfVector(in.inv, d);
+      wc.insts++;
+      if (in.geomKind == GK_MESH) {
+        st.push(kRefMarker, -__builtin_inff());
+        obj = makeRay(oo, dd);
+        rt = rayTriSetup(dd);
+        curInst = ii;
+        inBlas = true;
+        cur = in.blasRoot;
+        continue;
+      }
+      wc.quads++;
+      float ts[2];
+      int n = quadricRoots(in.geomKind, oo, dd, tmin, candBound(hit.t), ts);
+      for (int k = 0; k < n; k++) {
+        if (closer(ts[k], ii, k, hit) && maskAccepts(P, in, k, 0.0f, 0.0f, oo, dd, ts[k])) {
+          hit.t = ts[k];
+          hit.inst = ii;
+          hit.prim = k;
+          oObjOut = oo;
+          dObjOut = dd;
+          found = true;
+          break;  // roots ascend: the first accepted one is the closest
+        }
+      }
+    } else {
+      int first = (int)(cur & 0x07FFFFFFu);
+      int cnt = (int)((cur >> 27) & 0xFu) + 1;
+      const DevInstance& in = P.instances[curInst];
+      for (int k = 0; k < cnt; k++) {
+        wc.tris++;
+        const float4* tv = P.tris + 3 * (size_t)(first + k);
+        float4 a = tv[0], b = tv[1], c = tv[2];
+        float t, b1, b2;
+        if (triHit(rt, obj.o, f4xyz(a), f4xyz(b), f4xyz(c), tmin, candBound(hit.t), t, b1, b2)) {
+          int prim = __float_as_int(a.w);
+          if (closer(t, curInst, prim, hit) && maskAccepts(P, in, prim, b1, b2, obj.o, obj.d, t)) {
+            hit.t = t;
+            hit.inst = curInst;
+            hit.prim = prim;
+            hit.b1 = b1;
+            hit.b2 = b2;
+            oObjOut = obj.o;
+            dObjOut = obj.d;
+            found = true;
+          }
         }
       }
     }
-  }
-  // pop; an entry whose box starts beyond the current closest hit (the
-  // same widened comparison as the box test) cannot hold a closer hit
-  while (true) {
-    if (st.sp == q.base) return false;
-    uint2 e = st.pop();
-    q.cur = e.x;
-    if (q.cur == kRefMarker) {
-      q.inBlas = false;
-      continue;
+    // pop; an entry whose box starts beyond the current closest hit (the
+    // same widened comparison as the box test) cannot hold a closer hit
+    while (true) {
+      if (st.sp == base) return found;
+      uint2 e = st.pop();
+      cur = e.x;
+      if (cur == kRefMarker) {
+        inBlas = false;
+        continue;
+      }
+      if (__uint_as_float(e.y) <= hit.t * 1.0000004f) break;
     }
-    if (__uint_as_float(e.y) <= q.hit.t * 1.0000004f) return true;
   }
-}
-
-DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
-                      vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
-  ClosestQuery q;
-  closestBegin(P, st, q, o, d, tmin, tmax);
-  while (closestStep(P, st, q, wc)) {
-  }
-  hit = q.hit;
-  if (q.found) {
-    oObjOut = q.oObj;
-    dObjOut = q.dObj;
-  }
-  return q.found;
 }
 
 // Shadow transmittance along (tmin, tmax) (device/programs.cu:70-85,
@@ -2218,11 +2185,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(LaunchParams P, WfParams W)
 
 // Closest-hit queries of the queue; each slot goes to the bucket of the
 // material it will shade (its instance's front material kind) or of misses.
-// Persistent lanes: each lane advances its query a few nodes at a time and,
-// when the query completes, takes the next ray of the queue at once, so no
-// lane idles while the longest query of its wave finishes.
-constexpr int kWfStepsPerRefill = 8;
-
 __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams W) {
   __shared__ uint64_t s_stack[kStackLds * kBlock];
   Stack st;
@@ -2232,56 +2194,31 @@ __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams 
   st.sp = 0;
   WorkCount wc;
   uint32_t rays = 0;
-  const int lane = threadIdx.x & 63;
   const uint32_t total = W.cnt->closestN;
-  ClosestQuery q;
-  int slot = 0;
-  bool active = false, exhausted = false;
-  while (true) {
-    if (!exhausted) {
-      const uint64_t m = __ballot(!active);
-      if (m) {
-        const int leader = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&W.cnt->closestNext, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        exhausted = base + (uint32_t)__popcll(m) >= total;
-        if (!active) {
-          const uint32_t i = base + __popcll(m & ((1ull << lane) - 1ull));
-          if (i < total) {
-            slot = (int)W.closestQ[i];
-            closestBegin(P, st, q, f4xyz(wfLd(W, WP_O, slot)), f4xyz(wfLd(W, WP_D, slot)), 0.0f, 1e16f);
-            active = true;
-            rays++;
-          }
-        }
-      }
-    }
-    if (__ballot(active) == 0) break;
-    bool finished = false;
-    if (active) {
-      for (int k = 0; k < kWfStepsPerRefill; k++) {
-        if (!closestStep(P, st, q, wc)) {
-          finished = true;
-          break;
-        }
-      }
-    }
-    int key = kKeyMiss;
-    if (finished) {
-      active = false;
-      const int inst = q.found ? q.hit.inst : -1;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < total;
+    int slot = 0, key = kKeyMiss;
+    if (valid) {
+      slot = (int)W.closestQ[i];
+      vec3 o = f4xyz(wfLd(W, WP_O, slot)), d = f4xyz(wfLd(W, WP_D, slot));
+      Hit hit;
+      vec3 oo, dd;
+      const bool isHit = traceClosest(P, st, o, d, 0.0f, 1e16f, hit, oo, dd, wc);
+      rays++;
+      const int inst = isHit ? hit.inst : -1;
       wfSt(W, WP_RES0, slot,
-           make_float4(q.hit.t, __uint_as_float((uint32_t)inst), __uint_as_float((uint32_t)q.hit.prim), q.hit.b1));
-      wfSt(W, WP_RES1, slot, make_float4(q.hit.b2, 0.0f, 0.0f, 0.0f));
-      if (q.found) {
+           make_float4(hit.t, __uint_as_float((uint32_t)inst), __uint_as_float((uint32_t)hit.prim), hit.b1));
+      wfSt(W, WP_RES1, slot, make_float4(hit.b2, 0.0f, 0.0f, 0.0f));
+      if (isHit) {
         // sort key: the kind of the instance's front material (a face
         // without a material, 0xFFFF, sorts with the first kind)
         const uint32_t front = P.instances[inst].mats & 0xFFFFu;
         key = kKeyHit + (front == 0xFFFFu ? 0 : P.materials[front].kind);
       }
     }
-    wavePushKeyed(W.buckets, W.numSlots, W.cnt->bucketN, finished, key, (uint32_t)slot);
+    wavePushKeyed(W.buckets, W.numSlots, W.cnt->bucketN, valid, key, (uint32_t)slot);
   }
   flushCounters<false>(W.rayCounter != nullptr, W.rayCounter, nullptr, rays, wc);
 }
@@ -2335,7 +2272,6 @@ __global__ void k_wf_clear(WfCounters* c, int what) {
   } else if (t == 0) {
     c->closestN = 0;
     c->shadowN = 0;
-    c->closestNext = 0;
   }
 }
 

@@ -66,9 +66,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
-    ap.add_argument("--waves", type=int, default=2, choices=[2, 3, 4],
+    ap.add_argument("--waves", type=int, default=2, choices=[2, 3, 4, 5],
                     help="minimum waves per SIMD the megakernel's register budget allows")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
+                    help="minimum waves per SIMD of the wavefront query kernels")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
@@ -96,6 +98,7 @@ def main():
     sess = scene.session()
     sess.tune("waves", args.waves)
     sess.set_kernel(args.kernel)
+    sess.tune("wf_waves", args.wf_waves)
     setup_s = time.time() - t0
     k = args.spp_per_step
 

@@ -227,8 +227,9 @@ SUNDOG_API int sundog_session_reset(sundog_session*);
 SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int count);
 SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
-/* Launch-shape knobs picked by the autotuner. Keys: "waves" (2|3|4: the
- * minimum waves per SIMD the render kernel's register budget allows). */
+/* Launch-shape knobs picked by the autotuner. Keys: "waves" (2..5: the
+ * minimum waves per SIMD the megakernel's register budget allows),
+ * "wf_waves" (4|5: the same for the wavefront query kernels). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);
@@ -293,15 +294,34 @@ typedef struct {
   double kernel_ms, busy_ms;
   uint64_t used_bytes, budget_bytes;
   int32_t active, waiting, tile_size, waves, pass_spp, blocks_per_cu;
+  int32_t stack, graph_passes;
+  uint64_t graph_launches;
 } sundog_server_stats;
+
+/* Autotuner result: the launch shape it kept and the rates it measured on
+ * the calibration request (paths/s of the kernel time). */
+typedef struct {
+  int32_t tile, waves, stack, blocks_per_cu, graph_passes, trials;
+  double start_paths_per_s, best_paths_per_s;
+} sundog_tune_result;
 
 SUNDOG_API sundog_server* sundog_server_create(uint64_t hbm_budget_bytes, int pass_spp,
                                                int tile_size /* 8 | 16 | 32 */);
 SUNDOG_API void sundog_server_destroy(sundog_server*);
 /* Before the first submit. Returns the scene id, or -1. */
 SUNDOG_API int sundog_server_add_scene(sundog_server*, sundog_scene*, const sundog_render_options* opt);
-/* Keys: "tile" (8|16|32), "waves" (2|3|4), "pass_spp" (> 0). */
+/* Keys: "tile" (8|16|32: pixels per side of a work item), "waves" (2|3|4:
+ * occupancy variant of the persistent kernel), "stack" (4..24: LDS
+ * traversal-stack entries per lane), "blocks" (queue depth: resident
+ * persistent blocks per CU, 0 = as many as fit), "graph_passes" (1..64:
+ * progressive passes captured into one graph launch), "pass_spp" (> 0),
+ * "serial_waves" (2..5: occupancy variant of the per-request baseline). */
 SUNDOG_API int sundog_server_set_tuning(sundog_server*, const char* key, int value);
+/* Before the first submit: renders a calibration request of scene_id
+ * (width x height x spp) under candidate launch shapes by coordinate descent
+ * over waves, stack, tile, blocks and graph_passes, and keeps the fastest. */
+SUNDOG_API int sundog_server_autotune(sundog_server*, int scene_id, int width, int height, int spp,
+                                      sundog_tune_result* out);
 /* Returns the request id (>= 0) or -1; seed < 0 uses the scene's seed. */
 SUNDOG_API int64_t sundog_server_submit(sundog_server*, int scene_id, int width, int height, int spp,
                                         int64_t seed);

@@ -69,7 +69,14 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("paths", c_u64), ("rays_traced", c_u64), ("batches", c_u64), ("work_items", c_u64),
                 ("kernel_ms", c_d), ("busy_ms", c_d), ("used_bytes", c_u64), ("budget_bytes", c_u64),
                 ("active", c_i32), ("waiting", c_i32), ("tile_size", c_i32), ("waves", c_i32),
-                ("pass_spp", c_i32), ("blocks_per_cu", c_i32)]
+                ("pass_spp", c_i32), ("blocks_per_cu", c_i32), ("stack", c_i32), ("graph_passes", c_i32),
+                ("graph_launches", c_u64)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("tile", c_i32), ("waves", c_i32), ("stack", c_i32), ("blocks_per_cu", c_i32),
+                ("graph_passes", c_i32), ("trials", c_i32), ("start_paths_per_s", c_d),
+                ("best_paths_per_s", c_d)]
 
 
 def default_options(**kw):
@@ -154,6 +161,7 @@ SIGNATURES = {
     "sundog_server_release": (c_i, [c_p, c_i64]),
     "sundog_server_wait_idle": (c_i, [c_p, c_d]),
     "sundog_server_get_stats": (c_i, [c_p, ctypes.POINTER(ServerStats)]),
+    "sundog_server_autotune": (c_i, [c_p, c_i, c_i, c_i, c_i, ctypes.POINTER(TuneResult)]),
     "sundog_server_render_serial": (c_i, [c_p, c_i, c_i, c_i, c_i, c_i64, P_f, ctypes.POINTER(c_d)]),
     "sundog_scene_flatten": (c_p, [c_p, ctypes.POINTER(RenderOptions)]),
     "sundog_flat_free": (None, [c_p]),

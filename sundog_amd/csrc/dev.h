@@ -134,9 +134,10 @@ struct LaunchParams {
   const float* uvs;          // vec2 pool
   const float* lightPos;     // world-space mesh-light vertices (vec3 pool)
   const float* lightCdf;
-  uint32_t* spill;           // traversal-stack overflow, [entry][thread]
+  uint32_t* spill;           // traversal-stack overflow, 8-byte entries, [entry][thread]
   int32_t spillStride;       // threads covered by one spill row
   int32_t spillRows;
+  int32_t stackLds;          // LDS stack entries per lane of the megakernel
   uint32_t tlasRoot;         // packed reference of the top-level root
 };
 

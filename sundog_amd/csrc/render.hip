@@ -17,7 +17,8 @@ namespace sdk {
 #define DEV __device__ __forceinline__
 
 constexpr int kBlock = 256;
-constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) entries per lane in LDS
+constexpr int kStackLdsMax = 24;  // largest runtime LDS stack (megakernel, persistent kernel)
+constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) LDS entries per lane (fixed-size kernels)
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -1135,15 +1136,16 @@ struct Stack {
   uint64_t* spill;  // this lane's spill base: spill[i * stride]
   int stride;
   int sp;
+  int cap;  // LDS entries per lane; deeper entries go to the spill region
   DEV void push(uint32_t ref, float t) {
     uint64_t e = (uint64_t)ref | ((uint64_t)__float_as_uint(t) << 32);
-    if (sp < kStackLds) lds[sp * kBlock] = e;
-    else spill[(size_t)(sp - kStackLds) * stride] = e;
+    if (sp < cap) lds[sp * kBlock] = e;
+    else spill[(size_t)(sp - cap) * stride] = e;
     sp++;
   }
   DEV uint2 pop() {
     sp--;
-    uint64_t e = sp < kStackLds ? lds[sp * kBlock] : spill[(size_t)(sp - kStackLds) * stride];
+    uint64_t e = sp < cap ? lds[sp * kBlock] : spill[(size_t)(sp - cap) * stride];
     return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
   }
 };
@@ -1871,7 +1873,7 @@ DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned 
 // the kernel without spills; 3 and 4 trade scratch spills for occupancy).
 template <bool kCountWork, int kWaves>
 __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
-  __shared__ uint64_t s_stack[kStackLds * kBlock];
+  extern __shared__ uint64_t s_stack[];  // P.stackLds * kBlock entries
   int px, py;
   pixelOf(px, py);
   uint32_t rays = 0;
@@ -1882,6 +1884,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
     st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
     st.stride = P.spillStride;
     st.sp = 0;
+    st.cap = P.stackLds;
     renderPixel(P, px, py, st, wc, rays);
   }
   flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
@@ -1899,6 +1902,16 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
 // each request's sample offset for this pass (offset = first + pass * count,
 // clipped to the request's total). Runs as a kernel, never a memset, so it
 // can sit inside a captured graph.
+// Zeroes the pass and item counters of n queues (a kernel so that it can
+// open a captured graph).
+__global__ void k_queue_reset(QueueState* q, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    q[i].next = 0;
+    q[i].pass = 0;
+  }
+}
+
 __global__ void k_queue_prep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq) {
   __shared__ uint32_t s_pass;
   if (threadIdx.x == 0) s_pass = q->pass;
@@ -1925,14 +1938,16 @@ template <int kWaves>
 __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, const RequestDev* __restrict__ reqs,
                                                                const WorkItem* __restrict__ items, int numItems,
                                                                QueueState* q, int tileSize, uint32_t* spill,
-                                                               int spillStride, unsigned long long* rayCounter) {
-  __shared__ uint64_t s_stack[kStackLds * kBlock];
+                                                               int spillStride, int stackLds,
+                                                               unsigned long long* rayCounter) {
+  extern __shared__ uint64_t s_stack[];  // stackLds * kBlock entries
   const int lane = threadIdx.x & 63;
   Stack st;
   st.lds = ldsColumn(s_stack);
   st.spill = reinterpret_cast<uint64_t*>(spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = spillStride;
   st.sp = 0;
+  st.cap = stackLds;
   WorkCount wc;
   uint32_t rays = 0;
   const int pxPerTile = tileSize * tileSize;
@@ -2185,13 +2200,15 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(LaunchParams P, WfParams W)
 
 // Closest-hit queries of the queue; each slot goes to the bucket of the
 // material it will shade (its instance's front material kind) or of misses.
-__global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams W) {
+template <int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void k_wf_closest(LaunchParams P, WfParams W) {
   __shared__ uint64_t s_stack[kStackLds * kBlock];
   Stack st;
   st.lds = ldsColumn(s_stack);
   st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
+  st.cap = kStackLds;
   WorkCount wc;
   uint32_t rays = 0;
   const uint32_t total = W.cnt->closestN;
@@ -2225,13 +2242,15 @@ __global__ __launch_bounds__(kBlock) void k_wf_closest(LaunchParams P, WfParams 
 
 // Shadow queries of the queue; each slot goes to the bucket of the material
 // its suspended bounce is shading.
-__global__ __launch_bounds__(kBlock) void k_wf_shadow(LaunchParams P, WfParams W) {
+template <int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void k_wf_shadow(LaunchParams P, WfParams W) {
   __shared__ uint64_t s_stack[kStackLds * kBlock];
   Stack st;
   st.lds = ldsColumn(s_stack);
   st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
+  st.cap = kStackLds;
   WorkCount wc;
   uint32_t rays = 0;
   const uint32_t total = W.cnt->shadowN;
@@ -2289,6 +2308,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(LaunchParams P, const float* r
   st.spill = reinterpret_cast<uint64_t*>(P.spill) + i;
   st.stride = P.spillStride;
   st.sp = 0;
+  st.cap = kStackLds;
   const float* r = rays + 8 * (size_t)i;
   vec3 o = vec3{r[0], r[1], r[2]}, d = vec3{r[4], r[5], r[6]};
   Hit h;
@@ -2367,6 +2387,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(LaunchParams P, const float* 
   st.spill = reinterpret_cast<uint64_t*>(P.spill) + i;
   st.stride = P.spillStride;
   st.sp = 0;
+  st.cap = kStackLds;
   const float* r = rays + 8 * (size_t)i;
   WorkCount wc;
   vec3 tr = traceShadow(P, st, vec3{r[0], r[1], r[2]}, vec3{r[4], r[5], r[6]}, r[7], wc);
@@ -2396,14 +2417,17 @@ __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
 
 hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s) {
   dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+  const size_t lds = (size_t)P.stackLds * kBlock * sizeof(uint64_t);
   if (countWork)
-    hipLaunchKernelGGL((k_render<true, 2>), grid, dim3(kBlock), 0, s, P);
-  else if (waves >= 4)
-    hipLaunchKernelGGL((k_render<false, 4>), grid, dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL((k_render<true, 2>), grid, dim3(kBlock), lds, s, P);
+  else if (waves >= 5)
+    hipLaunchKernelGGL((k_render<false, 5>), grid, dim3(kBlock), lds, s, P);
+  else if (waves == 4)
+    hipLaunchKernelGGL((k_render<false, 4>), grid, dim3(kBlock), lds, s, P);
   else if (waves == 3)
-    hipLaunchKernelGGL((k_render<false, 3>), grid, dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL((k_render<false, 3>), grid, dim3(kBlock), lds, s, P);
   else
-    hipLaunchKernelGGL((k_render<false, 2>), grid, dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL((k_render<false, 2>), grid, dim3(kBlock), lds, s, P);
   return hipGetLastError();
 }
 
@@ -2433,6 +2457,11 @@ hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStrea
   return hipGetLastError();
 }
 
+hipError_t launchQueueReset(QueueState* q, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_queue_reset, dim3((n + 255) / 256), dim3(256), 0, s, q, n);
+  return hipGetLastError();
+}
+
 hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_queue_prep, dim3(1), dim3(256), 0, s, q, reqs, firstSample, numReq);
@@ -2441,57 +2470,73 @@ hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* first
 
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            unsigned long long* rayCounter, hipStream_t s) {
+                            int stackLds, unsigned long long* rayCounter, hipStream_t s) {
+  const size_t lds = (size_t)stackLds * kBlock * sizeof(uint64_t);
   if (waves >= 4)
-    hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
-                       spill, spillStride, rayCounter);
+    hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
+                       tileSize, spill, spillStride, stackLds, rayCounter);
   else if (waves == 3)
-    hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
-                       spill, spillStride, rayCounter);
+    hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
+                       tileSize, spill, spillStride, stackLds, rayCounter);
   else
-    hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), 0, s, S, reqs, items, numItems, q, tileSize,
-                       spill, spillStride, rayCounter);
+    hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
+                       tileSize, spill, spillStride, stackLds, rayCounter);
   return hipGetLastError();
 }
 
 // Resident blocks per CU of the persistent kernel at a given occupancy variant.
-int persistentBlocksPerCU(int waves) {
+int persistentBlocksPerCU(int waves, int stackLds) {
   int n = 0;
   hipError_t e;
+  const size_t lds = (size_t)stackLds * kBlock * sizeof(uint64_t);
   if (waves >= 4)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<4>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<4>, kBlock, lds);
   else if (waves == 3)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<3>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<3>, kBlock, lds);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<2>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<2>, kBlock, lds);
   return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 int stackLdsEntries() { return kStackLds; }
+int stackLdsMax() { return kStackLdsMax; }
 int renderBlockThreads() { return kBlock; }
 
 hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_wf_shade, dim3(blocks), dim3(kBlock), 0, s, P, W);
   return hipGetLastError();
 }
-hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_closest, dim3(blocks), dim3(kBlock), 0, s, P, W);
+hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s) {
+  if (waves >= 5)
+    hipLaunchKernelGGL(k_wf_closest<5>, dim3(blocks), dim3(kBlock), 0, s, P, W);
+  else
+    hipLaunchKernelGGL(k_wf_closest<4>, dim3(blocks), dim3(kBlock), 0, s, P, W);
   return hipGetLastError();
 }
-hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_wf_shadow, dim3(blocks), dim3(kBlock), 0, s, P, W);
+hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s) {
+  if (waves >= 5)
+    hipLaunchKernelGGL(k_wf_shadow<5>, dim3(blocks), dim3(kBlock), 0, s, P, W);
+  else
+    hipLaunchKernelGGL(k_wf_shadow<4>, dim3(blocks), dim3(kBlock), 0, s, P, W);
   return hipGetLastError();
 }
 hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s) {
   hipLaunchKernelGGL(k_wf_clear, dim3(1), dim3(64), 0, s, c, what);
   return hipGetLastError();
 }
-// Resident blocks per CU of each wavefront kernel: 0 shade, 1 closest, 2 shadow.
-int wfBlocksPerCU(int which) {
+// Resident blocks per CU of each wavefront kernel: 0 shade, 1 closest, 2
+// shadow (closest and shadow at the given waves-per-SIMD variant).
+int wfBlocksPerCU(int which, int waves) {
   int n = 0;
-  hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade, kBlock, 0)
-                 : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_closest, kBlock, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shadow, kBlock, 0);
+  hipError_t e;
+  if (which == 0)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade, kBlock, 0);
+  else if (which == 1)
+    e = waves >= 5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_closest<5>, kBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_closest<4>, kBlock, 0);
+  else
+    e = waves >= 5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shadow<5>, kBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shadow<4>, kBlock, 0);
   return (e == hipSuccess && n > 0) ? n : 1;
 }
 

@@ -66,13 +66,17 @@ struct sundog_server {
   size_t used = 0;
   int passSpp = 16;
   int tileSize = 16;
-  int waves = 2;
+  int waves = 4;         // occupancy variant of the persistent kernel
+  int stackLds = 16;     // LDS traversal-stack entries per lane
+  int blocksReq = 0;     // queue depth: resident persistent blocks per CU (0 = as many as fit)
+  int graphPasses = 4;   // progressive passes captured into one graph launch (1 = no graph)
+  int serialWaves = 2;   // occupancy variant of the per-request megakernel baseline
   int blocksPerCU = 1;
   int numCUs = 1;
-  int spillRows = 0;
+  int maxStackNeed = 0;  // deepest traversal stack over the resident scenes
   hipStream_t stream = nullptr;
-  DevBuf dReqs, dItems, dQueue, spill, rayCounter;
-  size_t reqCap = 0, itemCap = 0;
+  DevBuf dReqs, dItems, dQueue, dFirst, spill, rayCounter;
+  size_t reqCap = 0, itemCap = 0, queueCap = 0;
   Clock::time_point t0;
   std::mutex mu;
   std::condition_variable cv, idleCv;
@@ -86,7 +90,7 @@ struct sundog_server {
   std::string workerError;
   std::thread worker;
   // stats
-  uint64_t paths = 0, batches = 0, items = 0;
+  uint64_t paths = 0, batches = 0, items = 0, graphLaunches = 0;
   double kernelMs = 0, busyMs = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -142,16 +146,16 @@ struct sundog_server {
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     numCUs = prop.multiProcessorCount;
-    dQueue.alloc(sizeof(QueueState));
     rayCounter.alloc(8);
     HIP_OK(hipMemsetAsync(rayCounter.p, 0, 8, stream));
     applyTuning();
   }
 
   void applyTuning() {
-    blocksPerCU = persistentBlocksPerCU(waves);
+    const int fit = persistentBlocksPerCU(waves, stackLds);
+    blocksPerCU = blocksReq > 0 ? std::min(blocksReq, fit) : fit;
     size_t threads = (size_t)numCUs * blocksPerCU * renderBlockThreads();
-    int rows = std::max(0, spillRows);
+    int rows = std::max(0, maxStackNeed - stackLds);
     spill.alloc(std::max<size_t>(16, (size_t)rows * threads * kStackEntryBytes));
   }
 
@@ -173,11 +177,16 @@ struct sundog_server {
     return d;
   }
 
-  // One batch: the next pass of every active request. Requests are grouped
-  // so that one persistent launch references at most kMaxScenesPerLaunch
-  // scenes (their parameters travel as a kernel argument). Called by the
-  // worker with the lock released; only the worker touches `active`'s films.
-  void runBatch(std::vector<ServerRequest*>& batch) {
+  // One batch: the next `passes` progressive passes of every active request.
+  // Requests are grouped so that one persistent launch references at most
+  // kMaxScenesPerLaunch scenes (their parameters travel as a kernel
+  // argument). Each pass is a k_queue_prep (advances every request's sample
+  // window from its first sample and the group's pass counter) followed by
+  // one k_persistent launch per group; with passes > 1 the whole sequence
+  // is captured into a hipGraph and launched once. Returns the number of
+  // passes run. Called by the worker with the lock released; only the
+  // worker touches `active`'s films.
+  int runBatch(std::vector<ServerRequest*>& batch, int passes) {
     std::vector<ServerRequest*> order(batch);
     std::stable_sort(order.begin(), order.end(),
                      [](const ServerRequest* a, const ServerRequest* b) { return a->scene < b->scene; });
@@ -188,6 +197,7 @@ struct sundog_server {
     };
     std::vector<Group> groups;
     std::vector<RequestDev> reqs;
+    std::vector<int32_t> first;
     std::vector<WorkItem> work;
     int lastScene = -1;
     for (ServerRequest* rp : order) {
@@ -209,6 +219,7 @@ struct sundog_server {
       }
       int local = (int)(reqs.size() - g.reqBegin);
       reqs.push_back(requestRecord(r, g.numScenes - 1));
+      first.push_back(r.done);
       int tx = (r.width + tileSize - 1) / tileSize, ty = (r.height + tileSize - 1) / tileSize;
       for (int t = 0; t < tx * ty; t++) work.push_back(WorkItem{(int32_t)local, t});
       g.reqEnd = reqs.size();
@@ -217,21 +228,56 @@ struct sundog_server {
     if (reqs.size() > reqCap) {
       reqCap = reqs.size() * 2;
       dReqs.alloc(reqCap * sizeof(RequestDev));
+      dFirst.alloc(reqCap * sizeof(int32_t));
     }
     if (work.size() > itemCap) {
       itemCap = work.size() * 2;
       dItems.alloc(itemCap * sizeof(WorkItem));
     }
+    if (groups.size() > queueCap) {
+      queueCap = groups.size() * 2;
+      dQueue.alloc(queueCap * sizeof(QueueState));
+    }
     HIP_OK(hipMemcpyAsync(dReqs.p, reqs.data(), reqs.size() * sizeof(RequestDev), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(dFirst.p, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     HIP_OK(hipMemcpyAsync(dItems.p, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
-    int blocks = numCUs * blocksPerCU;
-    size_t threads = (size_t)blocks * renderBlockThreads();
+    const int blocks = numCUs * blocksPerCU;
+    const size_t threads = (size_t)blocks * renderBlockThreads();
+    auto enqueue = [&]() {
+      HIP_OK(launchQueueReset(dQueue.as<QueueState>(), (int)groups.size(), stream));
+      for (int p = 0; p < passes; p++) {
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+          const Group& g = groups[gi];
+          QueueState* q = dQueue.as<QueueState>() + gi;
+          RequestDev* rq = dReqs.as<RequestDev>() + g.reqBegin;
+          HIP_OK(launchQueuePrep(q, rq, dFirst.as<int32_t>() + g.reqBegin, (int)(g.reqEnd - g.reqBegin), stream));
+          HIP_OK(launchPersistent(g.table, rq, dItems.as<WorkItem>() + g.itemBegin, (int)(g.itemEnd - g.itemBegin),
+                                  q, tileSize, waves, blocks, spill.as<uint32_t>(), (int)threads, stackLds,
+                                  rayCounter.as<unsigned long long>(), stream));
+        }
+      }
+    };
     HIP_OK(hipEventRecord(ev0, stream));
-    for (const Group& g : groups) {
-      HIP_OK(hipMemsetAsync(dQueue.p, 0, sizeof(QueueState), stream));
-      HIP_OK(launchPersistent(g.table, dReqs.as<RequestDev>() + g.reqBegin, dItems.as<WorkItem>() + g.itemBegin,
-                              (int)(g.itemEnd - g.itemBegin), dQueue.as<QueueState>(), tileSize, waves, blocks,
-                              spill.as<uint32_t>(), (int)threads, rayCounter.as<unsigned long long>(), stream));
+    if (passes > 1) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HIP_OK(hipStreamEndCapture(stream, &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e == hipSuccess) e = hipGraphLaunch(exec, stream);
+      if (exec) (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      HIP_OK(e);
+      graphLaunches++;
+    } else {
+      enqueue();
     }
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
@@ -239,13 +285,22 @@ struct sundog_server {
     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
     kernelMs += ms;
     batches++;
-    items += work.size();
+    items += work.size() * passes;
     for (ServerRequest* rp : batch) {
       ServerRequest& r = *rp;
-      int n = std::min(passSpp, r.spp - r.done);
+      int n = std::min(passes * passSpp, r.spp - r.done);
       paths += (uint64_t)r.width * r.height * n;
       r.done += n;
     }
+    return passes;
+  }
+
+  // Passes for the next batch: as many as every active request still needs
+  // (so no request idles inside a graph), capped by graphPasses.
+  int passesFor(const std::vector<ServerRequest*>& batch) const {
+    int p = graphPasses;
+    for (const ServerRequest* r : batch) p = std::min(p, (r->spp - r->done + passSpp - 1) / passSpp);
+    return std::max(1, p);
   }
 
   void finish(ServerRequest& r) {
@@ -302,7 +357,7 @@ struct sundog_server {
           }
         }
         double tb = now();
-        runBatch(batch);
+        runBatch(batch, passesFor(batch));
         std::vector<ServerRequest*> finished;
         for (ServerRequest* r : batch)
           if (r->done >= r->spp) finished.push_back(r);
@@ -366,7 +421,8 @@ extern "C" int sundog_server_add_scene(sundog_server* s, sundog_scene* h, const 
     sundog_session* sc = sundog_session_create(h, opt);
     if (!sc) throw std::runtime_error(sundog_last_error());
     s->scenes.emplace_back(sc);
-    s->spillRows = std::max(s->spillRows, sc->spillRows);
+    s->maxStackNeed = std::max(s->maxStackNeed, sc->stackNeed);
+    if (s->stream) s->applyTuning();
     return (int)s->scenes.size() - 1;
   } catch (const std::exception& e) {
     setLastError(e.what());
@@ -386,6 +442,20 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
       if (value < 2 || value > 4) throw std::runtime_error("tuning: waves must be 2, 3 or 4");
       s->waves = value;
       if (s->stream) s->applyTuning();
+    } else if (k == "stack") {
+      if (value < 4 || value > stackLdsMax()) throw std::runtime_error("tuning: stack must be 4.." + std::to_string(stackLdsMax()));
+      s->stackLds = value;
+      if (s->stream) s->applyTuning();
+    } else if (k == "blocks") {
+      if (value < 0) throw std::runtime_error("tuning: blocks must be >= 0 (0 = as many as fit)");
+      s->blocksReq = value;
+      if (s->stream) s->applyTuning();
+    } else if (k == "graph_passes") {
+      if (value < 1 || value > 64) throw std::runtime_error("tuning: graph_passes must be 1..64");
+      s->graphPasses = value;
+    } else if (k == "serial_waves") {
+      if (value < 2 || value > 5) throw std::runtime_error("tuning: serial_waves must be 2..5");
+      s->serialWaves = value;
     } else if (k == "pass_spp") {
       if (value <= 0) throw std::runtime_error("tuning: pass_spp must be positive");
       s->passSpp = value;
@@ -529,6 +599,9 @@ extern "C" int sundog_server_get_stats(sundog_server* s, sundog_server_stats* ou
     out->waves = s->waves;
     out->pass_spp = s->passSpp;
     out->blocks_per_cu = s->blocksPerCU;
+    out->stack = s->stackLds;
+    out->graph_passes = s->graphPasses;
+    out->graph_launches = s->graphLaunches;
     return SUNDOG_OK;
   });
 }
@@ -572,12 +645,112 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     for (int done = 0; done < spp; done += 16) {
       P.sampleOffset = done;
       P.sppThisLaunch = std::min(16, spp - done);
-      HIP_OK(launchRender(P, false, sc.waves, s->stream));
+      HIP_OK(launchRender(P, false, s->serialWaves, s->stream));
       HIP_OK(hipStreamSynchronize(s->stream));
     }
     double ms = std::chrono::duration<double, std::milli>(Clock::now() - t).count();
     if (ms_out) *ms_out = ms;
     if (beauty) HIP_OK(hipMemcpy(beauty, r.accum.p, n, hipMemcpyDeviceToHost));
+    return SUNDOG_OK;
+  });
+}
+
+// Autotuner: coordinate descent over the persistent kernel's launch shape.
+// Each trial renders the calibration request from sample 0 through the same
+// batch path the worker uses and is timed with events on the server stream.
+extern "C" int sundog_server_autotune(sundog_server* s, int scene_id, int width, int height, int spp,
+                                      sundog_tune_result* out) {
+  return guardedS([&] {
+    if (!s) throw std::runtime_error("autotune: null handle");
+    if (scene_id < 0 || scene_id >= (int)s->scenes.size()) throw std::runtime_error("autotune: scene id out of range");
+    if (width <= 0 || height <= 0 || spp <= 0) throw std::runtime_error("autotune: width, height and spp must be positive");
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->started) throw std::runtime_error("autotune: call before the first submit");
+    }
+    s->ensureDevice();
+    ServerRequest r;
+    r.scene = scene_id;
+    r.width = width;
+    r.height = height;
+    r.spp = spp;
+    r.seed = s->scenes[scene_id]->P.seed;
+    size_t n = (size_t)width * height * 16;
+    r.accum.alloc(n);
+    r.albedo.alloc(n);
+    r.normal.alloc(n);
+    HIP_OK(hipMemsetAsync(r.accum.p, 0, n, s->stream));
+    HIP_OK(hipMemsetAsync(r.albedo.p, 0, n, s->stream));
+    HIP_OK(hipMemsetAsync(r.normal.p, 0, n, s->stream));
+    const uint64_t paths0 = s->paths, batches0 = s->batches, items0 = s->items, graphs0 = s->graphLaunches;
+    const double kms0 = s->kernelMs;
+    int trials = 0;
+    auto trial = [&]() {
+      s->applyTuning();
+      std::vector<ServerRequest*> batch{&r};
+      r.done = 0;
+      double ms = 0.0;
+      while (r.done < r.spp) {
+        double before = s->kernelMs;
+        s->runBatch(batch, s->passesFor(batch));
+        ms += s->kernelMs - before;
+      }
+      trials++;
+      return (double)width * height * spp / (ms * 1e-3);
+    };
+    trial();  // warm-up: first launches of each kernel
+    const double start = trial();
+    double best = start;
+    auto sweep = [&](int& knob, std::initializer_list<int> values) {
+      int keep = knob;
+      for (int v : values) {
+        if (v == keep) continue;
+        knob = v;
+        double rate = trial();
+        if (rate > best) {
+          best = rate;
+          keep = v;
+        }
+      }
+      knob = keep;
+    };
+    sweep(s->waves, {2, 3, 4});
+    sweep(s->stackLds, {8, 12, 16, 24});
+    sweep(s->tileSize, {8, 16, 32});
+    const int fit = persistentBlocksPerCU(s->waves, s->stackLds);
+    std::vector<int> depths;
+    for (int b = fit; b >= 1; b--) depths.push_back(b);
+    {
+      int keep = s->blocksReq;
+      for (int b : depths) {
+        s->blocksReq = b;
+        double rate = trial();
+        if (rate > best) {
+          best = rate;
+          keep = b;
+        }
+      }
+      s->blocksReq = keep;
+    }
+    sweep(s->graphPasses, {1, 2, 4, 8});
+    s->applyTuning();
+    // calibration work is not served work
+    s->paths = paths0;
+    s->batches = batches0;
+    s->items = items0;
+    s->graphLaunches = graphs0;
+    s->kernelMs = kms0;
+    if (out) {
+      *out = sundog_tune_result{};
+      out->tile = s->tileSize;
+      out->waves = s->waves;
+      out->stack = s->stackLds;
+      out->blocks_per_cu = s->blocksPerCU;
+      out->graph_passes = s->graphPasses;
+      out->trials = trials;
+      out->start_paths_per_s = start;
+      out->best_paths_per_s = best;
+    }
     return SUNDOG_OK;
   });
 }

@@ -28,18 +28,20 @@ hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool bru
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
 hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s);
+hipError_t launchQueueReset(QueueState* q, int n, hipStream_t s);
 hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq,
                            hipStream_t s);
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            unsigned long long* rayCounter, hipStream_t s);
-int persistentBlocksPerCU(int waves);
+                            int stackLds, unsigned long long* rayCounter, hipStream_t s);
+int persistentBlocksPerCU(int waves, int stackLds);
 hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
-hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
-hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
+hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s);
+hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s);
 hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s);
-int wfBlocksPerCU(int which);
+int wfBlocksPerCU(int which, int waves);
 int stackLdsEntries();
+int stackLdsMax();
 int renderBlockThreads();
 constexpr int kWfBlock = 256;  // threads per block of the wavefront kernels
 constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference + entry distance
@@ -195,7 +197,8 @@ struct sundog_session {
   LaunchParams P{};
   int kernel = SUNDOG_KERNEL_MEGA;
   bool countWork = false;
-  int waves = 2;  // occupancy variant of the render kernel (2, 3 or 4 waves per SIMD)
+  int waves = 4;    // occupancy variant of the render kernel (2 to 5 waves per SIMD)
+  int wfWaves = 4;  // occupancy variant of the wavefront query kernels (4 or 5)
   int numInstances = 0;
   int stackNeed = 0;
   int spillRows = 0;
@@ -233,7 +236,12 @@ struct sundog_session {
     return b;
   }
 
+  // Spill rows cover the deepest possible stack beyond the smallest LDS
+  // stack any kernel of this session uses.
+  int spillRowsFor(int ldsEntries) const { return std::max(0, stackNeed - std::min(ldsEntries, stackLdsEntries())); }
+
   void ensureSpill(size_t threads) {
+    spillRows = spillRowsFor(P.stackLds);
     if (spillRows <= 0) {
       if (!spill.p) spill.alloc(16);
       P.spill = spill.as<uint32_t>();
@@ -241,10 +249,10 @@ struct sundog_session {
       P.spillRows = 0;
       return;
     }
-    if (threads > spillThreads) {
+    const size_t need = (size_t)spillRows * threads * kStackEntryBytes;
+    if (need > spill.bytes) {
       HIP_OK(hipStreamSynchronize(stream));
-      spill.alloc((size_t)spillRows * threads * kStackEntryBytes);
-      spillThreads = threads;
+      spill.alloc(need);
     }
     P.spill = spill.as<uint32_t>();
     P.spillStride = (int)threads;
@@ -454,7 +462,8 @@ struct sundog_session {
     }
     blasNodes = (int)allNodes.size() - tlasNodes;
     stackNeed = tlasStack + 1 + blasStack;
-    spillRows = std::max(0, stackNeed - stackLdsEntries());
+    P.stackLds = stackLdsEntries();
+    spillRows = spillRowsFor(P.stackLds);
 
     std::vector<DevInstance> di;
     for (size_t i = 0; i < S.objects.size(); i++) {
@@ -620,7 +629,8 @@ struct sundog_session {
     HIP_OK(hipGetDevice(&dev));
     int cus = 0;
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int bShade = cus * wfBlocksPerCU(0), bClosest = cus * wfBlocksPerCU(1), bShadow = cus * wfBlocksPerCU(2);
+    const int bShade = cus * wfBlocksPerCU(0, 0), bClosest = cus * wfBlocksPerCU(1, wfWaves),
+              bShadow = cus * wfBlocksPerCU(2, wfWaves);
     ensureSpill((size_t)std::max(bClosest, bShadow) * renderBlockThreads());
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
@@ -641,8 +651,8 @@ struct sundog_session {
     wfIterations = 0;
     while (true) {
       HIP_OK(launchWfClear(W.cnt, 0, stream));
-      HIP_OK(launchWfClosest(P, W, bClosest, stream));
-      HIP_OK(launchWfShadow(P, W, bShadow, stream));
+      HIP_OK(launchWfClosest(P, W, bClosest, wfWaves, stream));
+      HIP_OK(launchWfShadow(P, W, bShadow, wfWaves, stream));
       HIP_OK(launchWfClear(W.cnt, 1, stream));
       HIP_OK(hipMemcpyAsync(wfHostCnt, W.cnt, sizeof(WfCounters), hipMemcpyDeviceToHost, stream));
       HIP_OK(hipStreamSynchronize(stream));

@@ -117,8 +117,15 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
     if (!s || !key) throw std::runtime_error("session: null argument");
     std::string k = key;
     if (k == "waves") {
-      if (value < 2 || value > 4) throw std::runtime_error("tuning: waves must be 2, 3 or 4");
+      if (value < 2 || value > 5) throw std::runtime_error("tuning: waves must be 2, 3, 4 or 5");
       s->waves = value;
+    } else if (k == "stack") {
+      if (value < 4 || value > stackLdsMax()) throw std::runtime_error("tuning: stack must be 4.." + std::to_string(stackLdsMax()));
+      HIP_OK(hipStreamSynchronize(s->stream));
+      s->P.stackLds = value;
+    } else if (k == "wf_waves") {
+      if (value < 4 || value > 5) throw std::runtime_error("tuning: wf_waves must be 4 or 5");
+      s->wfWaves = value;
     } else {
       throw std::runtime_error("tuning: unknown key '" + k + "'");
     }

@@ -87,6 +87,14 @@ class Server(object):
         _lib.check(self.lib.sundog_server_get_stats(self.ptr, ctypes.byref(st)), self.lib)
         return {k: getattr(st, k) for k, _ in st._fields_}
 
+    def autotune(self, scene, width=512, height=512, spp=32):
+        """Picks waves, stack, tile, queue depth and graph passes by measuring
+        a calibration request; call before the first submit."""
+        r = _lib.TuneResult()
+        _lib.check(self.lib.sundog_server_autotune(self.ptr, scene, width, height, spp, ctypes.byref(r)),
+                   self.lib)
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
     def render_serial(self, scene, width, height, spp, seed=-1, want_image=False):
         ms = ctypes.c_double(0)
         b = np.zeros(width * height * 4, np.float32) if want_image else None
@@ -156,7 +164,13 @@ def main(argv=None):
     ap.add_argument("--qps", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tile", type=int, default=16)
-    ap.add_argument("--waves", type=int, default=2)
+    ap.add_argument("--waves", type=int, default=4)
+    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--graph-passes", type=int, default=4)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shape with the autotuner on a 512x512, 32-spp request of scene 1")
+    ap.add_argument("--serial-waves", type=int, default=2,
+                    help="occupancy variant of the per-request megakernel baseline")
     ap.add_argument("--pass-spp", type=int, default=16)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--spp-max", type=int, default=1024)
@@ -170,6 +184,10 @@ def main(argv=None):
     srv = Server(budget_bytes=int(args.budget_gb * (1 << 30)), pass_spp=args.pass_spp, tile=args.tile)
     sids = [srv.add_scene(h) for h in handles]
     srv.tune("waves", args.waves)
+    srv.tune("stack", args.stack)
+    srv.tune("graph_passes", args.graph_passes)
+    srv.tune("serial_waves", args.serial_waves)
+    tuned = srv.autotune(sids[min(1, len(sids) - 1)]) if args.autotune else None
     reqs = make_requests(args.requests, args.qps, seed=args.seed, spp_hi=args.spp_max)
     total_paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
     print("serving: %d requests, %.3g paths, qps %.2f" % (len(reqs), total_paths, args.qps), flush=True)
@@ -183,7 +201,9 @@ def main(argv=None):
         "served_paths_per_s": total_paths / makespan,
         "served_kernel_paths_per_s": st["paths"] / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] else None,
         "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
-        "batches": st["batches"], "tile": st["tile_size"], "waves": st["waves"], "pass_spp": st["pass_spp"],
+        "batches": st["batches"], "graph_launches": st["graph_launches"], "tile": st["tile_size"],
+        "waves": st["waves"], "stack": st["stack"], "blocks_per_cu": st["blocks_per_cu"],
+        "graph_passes": st["graph_passes"], "pass_spp": st["pass_spp"], "autotune": tuned,
         "mrays_per_s": st["rays_traced"] / makespan / 1e6,
     }
     if not args.no_serial:
@@ -192,6 +212,7 @@ def main(argv=None):
             ms, _ = srv.render_serial(sids[r["scene"]], r["width"], r["height"], r["spp"], r["seed"])
             ser_ms += ms
         out["serial_paths_per_s"] = total_paths / (ser_ms * 1e-3)
+        out["serial_waves"] = args.serial_waves
         out["speedup_vs_serial"] = out["served_paths_per_s"] / out["serial_paths_per_s"]
     print(json.dumps(out), flush=True)
     return out

@@ -17,10 +17,15 @@ def _scenes(asset_dir):
             configs.instanced(asset_dir, width=64, height=36, spp=16, instances=32, subdiv=2).build()]
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
-def test_mix_equals_alone(asset_dir, tile):
+@pytest.mark.parametrize("tile,graph_passes,stack", [(8, 4, 16), (16, 4, 8), (32, 4, 24), (16, 1, 16)])
+def test_mix_equals_alone(asset_dir, tile, graph_passes, stack):
+    """Continuous batching, with passes captured into graphs (graph_passes >
+    1) or launched one by one, and with spill-heavy (8) or roomy (24) LDS
+    stacks: every film equals the request rendered alone."""
     hs = _scenes(asset_dir)
     srv = Server(budget_bytes=1 << 30, pass_spp=8, tile=tile)
+    srv.tune("graph_passes", graph_passes)
+    srv.tune("stack", stack)
     sids = [srv.add_scene(h) for h in hs]
     reqs = [(sids[0], 48, 40, 24, 5), (sids[1], 64, 36, 16, 9), (sids[0], 33, 17, 40, 2),
             (sids[1], 20, 70, 8, 1), (sids[0], 64, 64, 32, 1)]
@@ -66,3 +71,19 @@ def test_admission_by_footprint_and_rejection(asset_dir):
     assert rec[b]["t_admit_ms"] >= rec[a]["t_done_ms"]
     st = srv.stats()
     assert st["used_bytes"] == 0
+
+
+def test_autotune_then_serve(asset_dir):
+    hs = _scenes(asset_dir)
+    srv = Server(budget_bytes=1 << 30, pass_spp=8, tile=16)
+    sids = [srv.add_scene(h) for h in hs]
+    res = srv.autotune(sids[1], 64, 36, 16)
+    assert res["tile"] in (8, 16, 32) and res["waves"] in (2, 3, 4) and 4 <= res["stack"] <= 24
+    assert res["trials"] >= 10 and res["best_paths_per_s"] >= res["start_paths_per_s"] > 0
+    st = srv.stats()
+    assert st["paths"] == 0 and st["tile_size"] == res["tile"] and st["stack"] == res["stack"]
+    rid = srv.submit(sids[0], 40, 24, 20, 3)
+    srv.wait_idle(60.0)
+    out = srv.fetch(rid, 40, 24)
+    _, alone = srv.render_serial(sids[0], 40, 24, 20, 3, want_image=True)
+    assert np.array_equal(out.view(np.uint32), alone.view(np.uint32))

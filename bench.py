@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
-    ap.add_argument("--waves", type=int, default=2, choices=[2, 3, 4, 5],
+    ap.add_argument("--waves", type=int, default=4, choices=[2, 3, 4, 5],
                     help="minimum waves per SIMD the megakernel's register budget allows")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],

@@ -270,6 +270,32 @@ __global__ void k_pack_input(const float* __restrict__ src, uint16_t* __restrict
   dst[i] = f2bf(v);
 }
 
+// Network input from device films (float4 beauty / albedo / normal of a
+// w x h frame) at the padded size W x H (multiples of 16); padded pixels
+// repeat the nearest edge pixel.
+__global__ void k_pack_films(const float4* __restrict__ beauty, const float4* __restrict__ albedo,
+                             const float4* __restrict__ normal, int w, int h, int W, int H,
+                             float* __restrict__ dst) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)W * H) return;
+  int x = min((int)(i % W), w - 1), y = min((int)(i / W), h - 1);
+  size_t s = (size_t)y * w + x;
+  float4 b = beauty[s], a = albedo[s], n = normal[s];
+  float* d = dst + i * 9;
+  d[0] = b.x; d[1] = b.y; d[2] = b.z;
+  d[3] = a.x; d[4] = a.y; d[5] = a.z;
+  d[6] = n.x; d[7] = n.y; d[8] = n.z;
+}
+
+// Denoised linear HDR of the valid w x h region as float4 (alpha 1).
+__global__ void k_unpack_films(const float* __restrict__ fin, int w, int h, int W, float4* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)w * h) return;
+  int x = (int)(i % w), y = (int)(i / w);
+  const float* f = fin + ((size_t)y * W + x) * 3;
+  out[i] = make_float4(expm1f(f[0]), expm1f(f[1]), expm1f(f[2]), 1.0f);
+}
+
 // Final log-space output: the network predicts a residual on top of the
 // noisy input, out[p][c] = net[p][c] + log1p(max(beauty[p][c], 0)), c < 3.
 __global__ void k_final(const float* __restrict__ net, const float* __restrict__ src, float* __restrict__ out,
@@ -721,6 +747,44 @@ int guardedD(F&& f) {
   }
 }
 }  // namespace
+
+namespace sdn {
+
+// Denoises device films of a w x h frame with the weights in `weightsPath`;
+// writes float4 RGBA (w x h) to outHost. Used by sundog_render on request.
+double denoiseFilms(const char* weightsPath, const float4* beauty, const float4* albedo, const float4* normal,
+                    int w, int h, float* outHost) {
+  std::unique_ptr<sundog_denoiser> d(new sundog_denoiser());
+  d->init(1);
+  if (sundog_denoiser_load(d.get(), weightsPath) != SUNDOG_OK) throw std::runtime_error(sundog_last_error());
+  const int W = (w + 15) / 16 * 16, H = (h + 15) / 16 * 16;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  d->resize(1, H, W);
+  size_t P = (size_t)W * H;
+  HIP_OK(hipEventRecord(e0, d->stream));
+  hipLaunchKernelGGL(k_pack_films, dim3(blocks(P)), dim3(256), 0, d->stream, beauty, albedo, normal, w, h, W, H,
+                     d->devIn.as<float>());
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks(P * 16)), dim3(256), 0, d->stream, d->devIn.as<float>(),
+                     d->input.as<uint16_t>(), P);
+  d->forward();
+  DevBuf o;
+  o.alloc((size_t)w * h * 16);
+  hipLaunchKernelGGL(k_unpack_films, dim3(blocks((size_t)w * h)), dim3(256), 0, d->stream, d->fin.as<float>(), w, h,
+                     W, o.as<float4>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(e1, d->stream));
+  HIP_OK(hipMemcpyAsync(outHost, o.p, (size_t)w * h * 16, hipMemcpyDeviceToHost, d->stream));
+  HIP_OK(hipStreamSynchronize(d->stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+}  // namespace sdn
 
 extern "C" sundog_denoiser* sundog_denoiser_create(uint64_t seed) {
   try {

@@ -295,6 +295,11 @@ extern "C" int sundog_probe(sundog_device_info* out) {
   });
 }
 
+namespace sdn {
+double denoiseFilms(const char* weightsPath, const float4* beauty, const float4* albedo, const float4* normal,
+                    int w, int h, float* outHost);
+}
+
 extern "C" int sundog_render(sundog_scene* h, const sundog_render_options* opt) {
   return guarded([&] {
     if (!opt || !opt->out_path || !*opt->out_path) throw std::runtime_error("render: out_path is required");
@@ -330,11 +335,26 @@ extern "C" int sundog_render(sundog_scene* h, const sundog_render_options* opt) 
     }
     double renderMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tRender).count();
     if (!quiet) fprintf(stderr, "\n");
-    if (rs.denoise && !quiet)
-      fprintf(stderr, "denoise: no trained denoiser is loaded; writing the plain render\n");
     size_t npx = (size_t)rs.width * rs.height;
     std::vector<float> img(npx * 4);
-    sundog_session_download(s.get(), img.data(), nullptr, nullptr);
+    // Denoise on request (src/capi_render.cpp:247-256): the learned denoiser
+    // runs on the device films when trained weights are named by
+    // SUNDOG_DENOISER_WEIGHTS; otherwise the plain render is the output.
+    bool denoised = false;
+    double denoiseMs = 0.0;
+    if (rs.denoise) {
+      const char* weights = getenv("SUNDOG_DENOISER_WEIGHTS");
+      if (weights && *weights) {
+        HIP_OK(hipStreamSynchronize(s->stream));
+        denoiseMs = sdn::denoiseFilms(weights, s->accum.as<float4>(), s->albedo.as<float4>(),
+                                      s->normal.as<float4>(), rs.width, rs.height, img.data());
+        denoised = true;
+        if (!quiet) fprintf(stderr, "denoise: %.1f ms (%s)\n", denoiseMs, weights);
+      } else if (!quiet) {
+        fprintf(stderr, "denoise: SUNDOG_DENOISER_WEIGHTS names no trained weights; writing the plain render\n");
+      }
+    }
+    if (!denoised) sundog_session_download(s.get(), img.data(), nullptr, nullptr);
     std::string out = pfmPath(opt->out_path);
     writePfm(out, img, rs.width, rs.height);
     if (opt->aov_albedo_path) {
@@ -364,12 +384,12 @@ extern "C" int sundog_render(sundog_scene* h, const sundog_render_options* opt) 
       fprintf(f, "{\n  \"scene\": %s,\n", jstr(name).c_str());
       fprintf(f, "  \"width\": %d,\n  \"height\": %d,\n  \"spp\": %d,\n", rs.width, rs.height, rs.spp);
       fprintf(f, "  \"max_depth\": %d,\n  \"seed\": %u,\n", rs.maxDepth, rs.seed);
-      fprintf(f, "  \"denoised\": false,\n");
+      fprintf(f, "  \"denoised\": %s,\n", denoised ? "true" : "false");
       fprintf(f,
               "  \"timings_ms\": {\n    \"scene_load\": %.17g,\n    \"physics\": 0,\n"
-              "    \"gas_build\": %.17g,\n    \"render\": %.17g,\n    \"denoise\": 0,\n"
+              "    \"gas_build\": %.17g,\n    \"render\": %.17g,\n    \"denoise\": %.17g,\n"
               "    \"total\": %.17g\n  },\n",
-              s->prep.loadMs, s->buildMs - s->prep.loadMs, renderMs, totalMs);
+              s->prep.loadMs, s->buildMs - s->prep.loadMs, renderMs, denoiseMs, totalMs);
       fprintf(f, "  \"rays_traced\": %llu,\n  \"mrays_per_sec\": %.17g,\n", (unsigned long long)st.rays_traced,
               st.rays_traced / renderMs / 1000.0);
       fprintf(f, "  \"peak_vram_mb\": %zu,\n", freeBefore > freeAfter ? (freeBefore - freeAfter) >> 20 : (size_t)0);

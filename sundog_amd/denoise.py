@@ -221,6 +221,48 @@ def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0):
     return losses
 
 
+def measure(args):
+    """Trains on seed-randomised variants (inputs at 8 and 16 spp, targets at
+    --ref-spp), then renders a held-out Cornell variant at --width x --height:
+    --input-spp noisy + denoise against a plain --target-spp render."""
+    t0 = time.time()
+    xa, ya = dataset(args.asset_dir, args.variants // 2, args.size, args.size, 8, args.ref_spp)
+    xb, yb = dataset(args.asset_dir, args.variants - args.variants // 2, args.size, args.size, 16, args.ref_spp,
+                     first_variant=args.variants // 2)
+    xs, ys = np.concatenate([xa, xb]), np.concatenate([ya, yb])
+    data_s = time.time() - t0
+    den = Denoiser(seed=1)
+    t1 = time.time()
+    losses = train(den, xs, ys, args.steps, batch=8, crop=64, lr=1e-3, log_every=max(args.steps // 10, 1))
+    train_s = time.time() - t1
+    if args.out:
+        den.save(args.out)
+    w, h = args.width, args.height
+    v = 1000
+    noisy_sc = cornell_variant(v, w, h, args.input_spp, 5).build()
+    s = noisy_sc.session()
+    s.render_all()
+    b, a, n = s.download()
+    s.close()
+    t2 = time.time()
+    ref_sc = cornell_variant(v, w, h, args.target_spp, 77).build()
+    r = ref_sc.session()
+    r.render_all(chunk=256)
+    ref, _, _ = r.download()
+    r.close()
+    ref_s = time.time() - t2
+    out = den.infer_image(b, a, n)
+    ms, fl = den.time_forward(1, (h + 15) // 16 * 16, (w + 15) // 16 * 16, iters=5)
+    res = {"train_steps": args.steps, "train_s": train_s, "data_s": data_s,
+           "final_loss": float(np.mean(losses[-50:])),
+           "frame": [w, h], "input_spp": args.input_spp, "target_spp": args.target_spp, "target_render_s": ref_s,
+           "rel_rmse_noisy": rel_rmse(b, ref), "rel_rmse_denoised": rel_rmse(out, ref),
+           "denoise_forward_ms": ms, "denoise_gflop": fl / 1e9,
+           "mfma_frac_bf16_dense": fl / (ms * 1e-3) / 2.5e15}
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -235,6 +277,18 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--out", default="")
     t.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
+    ms = sub.add_parser("measure", help="train, then denoise a held-out 1080p frame and compare with a "
+                                        "plain high-spp render of it")
+    ms.add_argument("--steps", type=int, default=3000)
+    ms.add_argument("--variants", type=int, default=24)
+    ms.add_argument("--size", type=int, default=128)
+    ms.add_argument("--ref-spp", type=int, default=1024)
+    ms.add_argument("--width", type=int, default=1920)
+    ms.add_argument("--height", type=int, default=1080)
+    ms.add_argument("--input-spp", type=int, default=16)
+    ms.add_argument("--target-spp", type=int, default=4096)
+    ms.add_argument("--out", default="")
+    ms.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
     tm = sub.add_parser("time")
     tm.add_argument("--batch", type=int, default=1)
     tm.add_argument("--height", type=int, default=1088)
@@ -250,6 +304,9 @@ def main(argv=None):
                "shape": [args.batch, args.height, args.width]}
         print(json.dumps(out), flush=True)
         return out
+
+    if args.cmd == "measure":
+        return measure(args)
 
     t0 = time.time()
     xs, ys = dataset(args.asset_dir, args.variants, args.size, args.size, args.noisy_spp, args.ref_spp)

@@ -590,6 +590,18 @@ def probe(lib=None):
             "runtime": di.runtime_version}
 
 
+def read_pfm(path):
+    """Reads a colour PFM ('PF', little-endian scale < 0) as (h, w, 3)
+    float32, top row first."""
+    with open(path, "rb") as f:
+        if f.readline().strip() != b"PF":
+            raise ValueError("%s: not a colour PFM" % path)
+        w, h = (int(v) for v in f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(w * h * 12), "<f4" if scale < 0 else ">f4").astype(np.float32)
+    return data.reshape(h, w, 3)[::-1].copy()
+
+
 def rel_rmse(a, b):
     """||a - b||_2 / ||b||_2 over all pixels x RGB."""
     a = np.asarray(a, np.float64)[..., :3]

@@ -168,7 +168,7 @@ def main(argv=None):
     ap.add_argument("--stack", type=int, default=16)
     ap.add_argument("--graph-passes", type=int, default=4)
     ap.add_argument("--autotune", action="store_true",
-                    help="pick the launch shape with the autotuner on a 512x512, 32-spp request of scene 1")
+                    help="pick the launch shape with the autotuner on a 1024x576, 16-spp request of scene 1")
     ap.add_argument("--serial-waves", type=int, default=2,
                     help="occupancy variant of the per-request megakernel baseline")
     ap.add_argument("--pass-spp", type=int, default=16)
@@ -187,7 +187,7 @@ def main(argv=None):
     srv.tune("stack", args.stack)
     srv.tune("graph_passes", args.graph_passes)
     srv.tune("serial_waves", args.serial_waves)
-    tuned = srv.autotune(sids[min(1, len(sids) - 1)]) if args.autotune else None
+    tuned = srv.autotune(sids[min(1, len(sids) - 1)], 1024, 576, 16) if args.autotune else None
     reqs = make_requests(args.requests, args.qps, seed=args.seed, spp_hi=args.spp_max)
     total_paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
     print("serving: %d requests, %.3g paths, qps %.2f" % (len(reqs), total_paths, args.qps), flush=True)

@@ -130,3 +130,33 @@ def test_denoiser_gain_on_renders(asset_dir):
     den_err = np.mean([denoise.rel_rmse(d, y) for d, y in zip(out, yt)])
     print("held-out relRMSE noisy %.4f denoised %.4f" % (noisy, den_err))
     assert den_err < 0.8 * noisy, (noisy, den_err)
+
+
+def test_render_denoise_on_request(asset_dir, tmp_path, monkeypatch):
+    """sundog_render with denoise=1: the trained network's output when
+    SUNDOG_DENOISER_WEIGHTS names weights, the plain render otherwise."""
+    from sundog_amd import configs
+    from sundog_amd.scenelib import read_pfm
+    den = denoise.Denoiser(seed=2)
+    rng = np.random.default_rng(6)
+    x, y = _batch(rng, 2, 32, 32)
+    for _ in range(5):
+        den.train_step(x, y, 1e-3)
+    wpath = str(tmp_path / "w.sdn")
+    den.save(wpath)
+
+    def render(name, denoise_flag):
+        sc = configs.cornell(width=40, height=24, spp=8).build()
+        path = str(tmp_path / name)
+        sc.render_file(path, denoise=denoise_flag, aov_albedo_path=path + "_a", aov_normal_path=path + "_n")
+        return read_pfm(path + ".pfm"), read_pfm(path + "_a.pfm"), read_pfm(path + "_n.pfm")
+
+    monkeypatch.delenv("SUNDOG_DENOISER_WEIGHTS", raising=False)
+    plain, alb, nrm = render("plain", 0)
+    fallback, _, _ = render("fallback", 1)
+    assert np.array_equal(plain, fallback)
+    monkeypatch.setenv("SUNDOG_DENOISER_WEIGHTS", wpath)
+    dn, _, _ = render("dn", 1)
+    ref = den.infer_image(plain, alb, nrm)
+    assert not np.array_equal(dn, plain)
+    assert np.allclose(dn, ref, rtol=1e-5, atol=1e-6)

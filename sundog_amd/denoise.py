@@ -222,9 +222,10 @@ def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0):
 
 
 def measure(args):
-    """Trains on seed-randomised variants (inputs at 8 and 16 spp, targets at
-    --ref-spp), then renders a held-out Cornell variant at --width x --height:
-    --input-spp noisy + denoise against a plain --target-spp render."""
+    """Trains on seed-randomised variants (--size squared renders, inputs at 8
+    and 16 spp, targets at --ref-spp, random --crop crops), then renders a
+    held-out Cornell variant at --width x --height: --input-spp noisy +
+    denoise against a plain --target-spp render."""
     t0 = time.time()
     xa, ya = dataset(args.asset_dir, args.variants // 2, args.size, args.size, 8, args.ref_spp)
     xb, yb = dataset(args.asset_dir, args.variants - args.variants // 2, args.size, args.size, 16, args.ref_spp,
@@ -233,7 +234,7 @@ def measure(args):
     data_s = time.time() - t0
     den = Denoiser(seed=1)
     t1 = time.time()
-    losses = train(den, xs, ys, args.steps, batch=8, crop=64, lr=1e-3, log_every=max(args.steps // 10, 1))
+    losses = train(den, xs, ys, args.steps, batch=8, crop=args.crop, lr=1e-3, log_every=max(args.steps // 10, 1))
     train_s = time.time() - t1
     if args.out:
         den.save(args.out)
@@ -279,10 +280,11 @@ def main(argv=None):
     t.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
     ms = sub.add_parser("measure", help="train, then denoise a held-out 1080p frame and compare with a "
                                         "plain high-spp render of it")
-    ms.add_argument("--steps", type=int, default=3000)
+    ms.add_argument("--steps", type=int, default=6000)
     ms.add_argument("--variants", type=int, default=24)
-    ms.add_argument("--size", type=int, default=128)
-    ms.add_argument("--ref-spp", type=int, default=1024)
+    ms.add_argument("--size", type=int, default=256)
+    ms.add_argument("--ref-spp", type=int, default=4096)
+    ms.add_argument("--crop", type=int, default=128)
     ms.add_argument("--width", type=int, default=1920)
     ms.add_argument("--height", type=int, default=1080)
     ms.add_argument("--input-spp", type=int, default=16)

@@ -215,6 +215,9 @@ typedef struct {
    * 4-wide node visits (4 box tests each), triangle tests, quadric tests,
    * world->object instance transforms */
   uint64_t node_visits, tri_tests, quad_tests, inst_transforms;
+  int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH */
+  int32_t pad2;
+  double bvh_gpu_ms;        /* wall time of the last GPU LBVH rebuild */
 } sundog_session_stats;
 
 SUNDOG_API sundog_session* sundog_session_create(sundog_scene*,
@@ -229,7 +232,9 @@ SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
 /* Launch-shape knobs picked by the autotuner. Keys: "waves" (2..5: the
  * minimum waves per SIMD the megakernel's register budget allows),
- * "wf_waves" (4|5: the same for the wavefront query kernels). */
+ * "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
+ * (4..24: LDS traversal-stack entries per lane of the megakernel), and
+ * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);

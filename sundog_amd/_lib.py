@@ -56,7 +56,7 @@ class SessionStats(ctypes.Structure):
                 ("build_ms", c_d), ("device_bytes", c_u64), ("stack_depth", c_i32),
                 ("tlas_nodes", c_i32), ("blas_nodes", c_i32), ("pad", c_i32),
                 ("node_visits", c_u64), ("tri_tests", c_u64), ("quad_tests", c_u64),
-                ("inst_transforms", c_u64)]
+                ("inst_transforms", c_u64), ("bvh_builder", c_i32), ("pad2", c_i32), ("bvh_gpu_ms", c_d)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -20,7 +20,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libsundog.so")
 
 HOST_SOURCES = ["capi_scene.cpp", "assets.cpp", "bvh.cpp", "prepare.cpp"]
-HIP_SOURCES = ["render.hip", "session.hip", "server.hip", "denoise.hip"]
+HIP_SOURCES = ["render.hip", "session.hip", "server.hip", "denoise.hip", "lbvh.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

@@ -42,6 +42,12 @@ hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s);
 int wfBlocksPerCU(int which, int waves);
 int stackLdsEntries();
 int stackLdsMax();
+struct DevBuf;
+int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t nodeBase, uint32_t primBase,
+              DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s);
+hipError_t launchTriBoxes(const float* pos, const uint32_t* idx, int n, float4* boxes, hipStream_t s);
+hipError_t launchEmitTris(const uint64_t* sortedKeys, int n, const float* pos, const uint32_t* idx, float4* tris,
+                          hipStream_t s);
 int renderBlockThreads();
 constexpr int kWfBlock = 256;  // threads per block of the wavefront kernels
 constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference + entry distance
@@ -217,6 +223,11 @@ struct sundog_session {
   double buildMs = 0.0;
   uint64_t paths = 0;
   int tlasNodes = 0, blasNodes = 0;
+  int bvhBuilder = 0;        // 0: host binned SAH, 1: GPU LBVH
+  double bvhGpuMs = 0.0;     // wall time of the last GPU build
+  std::vector<DevInstance> instHost;
+  std::vector<int> meshPosOff, meshIdxOff;
+  std::vector<Aabb> meshBox;
 
   ~sundog_session() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -483,6 +494,14 @@ struct sundog_session {
     }
     numInstances = (int)di.size();
     instances.upload(di, stream);
+    instHost = di;
+    meshBox = meshBounds;
+    meshPosOff.clear();
+    meshIdxOff.clear();
+    for (const DevMesh& dm : dmesh) {
+      meshPosOff.push_back(dm.posOffset);
+      meshIdxOff.push_back(dm.idxOffset);
+    }
     nodes.upload(allNodes, stream);
     tris.upload(triPool, stream);
     meshes.upload(dmesh, stream);
@@ -586,6 +605,88 @@ struct sundog_session {
     HIP_OK(hipMemsetAsync(workCounters.p, 0, 32, stream));
     renderMs = 0.0;
     paths = 0;
+  }
+
+  // Rebuilds both levels with the GPU LBVH (lbvh.hip) from the resident
+  // mesh pools: the top level first (instance boxes from the meshes' bounds,
+  // one instance per leaf), then every used mesh's BLAS (up to 4 triangles
+  // per leaf, triangles re-emitted in leaf order). Closest hits do not depend
+  // on the tree, so films stay bit-identical wherever shadow rays are opaque.
+  void buildBvhGpu() {
+    HIP_OK(hipStreamSynchronize(stream));
+    auto t0 = std::chrono::steady_clock::now();
+    const Scene& S = prep.scene;
+    std::vector<float4> ib;
+    for (const Object& o : S.objects) {
+      Aabb ob = o.geomKind == GK_MESH ? meshBox[o.meshId] : quadricBox(o.geomKind);
+      Aabb w = transformBox(o.xform, ob);
+      ib.push_back(make_float4(w.lo.x, w.lo.y, w.lo.z, 0.0f));
+      ib.push_back(make_float4(w.hi.x, w.hi.y, w.hi.z, 0.0f));
+    }
+    DevBuf instBoxes, tlas, tlasKeys;
+    instBoxes.upload(ib, stream);
+    int tlasStack = 0;
+    const int nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack,
+                             stream);
+    struct Blas {
+      DevBuf nodes, keys;
+      int num = 0, stack = 0;
+      uint32_t nodeBase = 0, primBase = 0;
+      int mesh = -1, tris = 0;
+    };
+    std::vector<Blas> bl;
+    uint32_t nodeBase = (uint32_t)nT, primBase = 0;
+    int blasStack = 0;
+    std::vector<uint32_t> root(prep.meshes.size(), 0);
+    for (size_t mi = 0; mi < prep.meshes.size(); mi++) {
+      bool used = false;
+      for (const Object& o : S.objects) used = used || (o.geomKind == GK_MESH && o.meshId == (int)mi);
+      if (!used) continue;
+      const int n = (int)prep.meshes[mi].numTris();
+      DevBuf boxes;
+      boxes.alloc((size_t)n * 2 * sizeof(float4));
+      const float* pos = positions.as<float>() + 3 * (size_t)meshPosOff[mi];
+      const uint32_t* idx = indices.as<uint32_t>() + meshIdxOff[mi];
+      HIP_OK(launchTriBoxes(pos, idx, n, boxes.as<float4>(), stream));
+      bl.emplace_back();
+      Blas& b = bl.back();
+      b.mesh = (int)mi;
+      b.tris = n;
+      b.nodeBase = nodeBase;
+      b.primBase = primBase;
+      b.num = lbvhBuild(boxes.as<float4>(), n, 4, false, nodeBase, primBase, b.nodes, b.keys, b.stack, stream);
+      root[mi] = nodeBase;
+      nodeBase += (uint32_t)b.num;
+      primBase += (uint32_t)n;
+      blasStack = std::max(blasStack, b.stack);
+    }
+    DevBuf newNodes, newTris;
+    newNodes.alloc((size_t)nodeBase * sizeof(BvhNode4));
+    newTris.alloc(std::max<size_t>(16, (size_t)primBase * 3 * sizeof(float4)));
+    HIP_OK(hipMemcpyAsync(newNodes.p, tlas.p, (size_t)nT * sizeof(BvhNode4), hipMemcpyDeviceToDevice, stream));
+    for (Blas& b : bl) {
+      HIP_OK(hipMemcpyAsync(newNodes.as<BvhNode4>() + b.nodeBase, b.nodes.p, (size_t)b.num * sizeof(BvhNode4),
+                            hipMemcpyDeviceToDevice, stream));
+      const float* pos = positions.as<float>() + 3 * (size_t)meshPosOff[b.mesh];
+      const uint32_t* idx = indices.as<uint32_t>() + meshIdxOff[b.mesh];
+      HIP_OK(launchEmitTris(b.keys.as<uint64_t>(), b.tris, pos, idx, newTris.as<float4>() + 3 * (size_t)b.primBase,
+                            stream));
+    }
+    for (size_t i = 0; i < S.objects.size(); i++)
+      if (S.objects[i].geomKind == GK_MESH) instHost[i].blasRoot = root[S.objects[i].meshId];
+    instances.upload(instHost, stream);
+    HIP_OK(hipStreamSynchronize(stream));
+    nodes = std::move(newNodes);
+    tris = std::move(newTris);
+    P.nodes = nodes.as<BvhNode4>();
+    P.tris = tris.as<float4>();
+    P.instances = instances.as<DevInstance>();
+    tlasNodes = nT;
+    blasNodes = (int)nodeBase - nT;
+    stackNeed = tlasStack + 1 + blasStack;
+    spillRows = spillRowsFor(P.stackLds);
+    bvhBuilder = 1;
+    bvhGpuMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
   void renderPass(int offset, int count) {

@@ -123,6 +123,9 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
       if (value < 4 || value > stackLdsMax()) throw std::runtime_error("tuning: stack must be 4.." + std::to_string(stackLdsMax()));
       HIP_OK(hipStreamSynchronize(s->stream));
       s->P.stackLds = value;
+    } else if (k == "bvh") {
+      if (value != 1) throw std::runtime_error("tuning: bvh must be 1 (rebuild both levels with the GPU LBVH)");
+      s->buildBvhGpu();
     } else if (k == "wf_waves") {
       if (value < 4 || value > 5) throw std::runtime_error("tuning: wf_waves must be 4 or 5");
       s->wfWaves = value;
@@ -180,6 +183,8 @@ extern "C" int sundog_session_get_stats(sundog_session* s, sundog_session_stats*
     out->device_bytes = s->deviceBytes();
     out->stack_depth = s->stackNeed;
     out->tlas_nodes = s->tlasNodes;
+    out->bvh_builder = s->bvhBuilder;
+    out->bvh_gpu_ms = s->bvhGpuMs;
     out->blas_nodes = s->blasNodes;
     return SUNDOG_OK;
   });

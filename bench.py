@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
                     help="minimum waves per SIMD of the wavefront query kernels")
+    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh"],
+                    help="host binned-SAH build (default) or the GPU LBVH rebuild")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
@@ -99,9 +101,12 @@ def main():
     sess.tune("waves", args.waves)
     sess.set_kernel(args.kernel)
     sess.tune("wf_waves", args.wf_waves)
+    if args.bvh == "lbvh":
+        sess.tune("bvh", 1)
     setup_s = time.time() - t0
     k = args.spp_per_step
 
+    st0 = sess.stats()
     # instrumented pass: traversal work per ray query (the megakernel's
     # counting variant; the wavefront variant runs the same traversal code)
     sess.set_kernel("megakernel")
@@ -161,6 +166,7 @@ def main():
             "step": "one %d-spp progressive pass over the full frame" % k,
             "kernel": args.kernel,
             "waves_per_simd": args.waves,
+            "bvh": args.bvh,
         },
         "mrays_per_s": rays / wall / 1e6,
         "rays_per_path": rays / max(paths, 1),
@@ -178,6 +184,7 @@ def main():
             "work_per_query": {k2: v / queries for k2, v in work.items()},
         },
         "setup_s": setup_s,
+        "bvh_build_ms": {"host_sah_total_build": st0["build_ms"], "gpu_lbvh": st0["bvh_gpu_ms"]},
         "finite": finite,
     }
 

@@ -358,6 +358,17 @@ SUNDOG_API struct sundog_flat_scene* sundog_scene_flatten(sundog_scene*,
                                                           const sundog_render_options* opt);
 SUNDOG_API void sundog_flat_free(struct sundog_flat_scene*);
 
+/* ---- HDR output transfer ------------------------------------------------------
+ * The chain of the reference's output writer (src/film.cpp:96-115,
+ * src/transfer.h): exposure, BT.709 -> BT.2020 gamut, 1.0 -> 203 nits,
+ * SMPTE ST 2084 (PQ), 12-bit code values. sundog_render writes these codes
+ * as a 16-bit PPM (maxval 4095) beside the linear PFM. */
+SUNDOG_API float sundog_pq_oetf(float linear);  /* 1.0 = 10,000 nits; clamps, NaN -> 0 */
+SUNDOG_API float sundog_pq_eotf(float code);
+SUNDOG_API void sundog_bt709_to_2020(const float in[3], float out[3]);
+/* width*height RGBA float film -> width*height*3 12-bit codes. */
+SUNDOG_API int sundog_encode_pq12(const float* rgba, int width, int height, float exposure, uint16_t* out);
+
 /* ---- learned denoiser -------------------------------------------------------
  * Replaces the vendor AI denoiser behind src/denoise.cpp:18-53 (HDR beauty +
  * albedo + normal guides in, denoised HDR out) with a BF16 U-Net whose

@@ -165,6 +165,10 @@ SIGNATURES = {
     "sundog_server_render_serial": (c_i, [c_p, c_i, c_i, c_i, c_i, c_i64, P_f, ctypes.POINTER(c_d)]),
     "sundog_scene_flatten": (c_p, [c_p, ctypes.POINTER(RenderOptions)]),
     "sundog_flat_free": (None, [c_p]),
+    "sundog_pq_oetf": (c_f, [c_f]),
+    "sundog_pq_eotf": (c_f, [c_f]),
+    "sundog_bt709_to_2020": (None, [P_f, P_f]),
+    "sundog_encode_pq12": (c_i, [P_f, c_i, c_i, c_f, ctypes.POINTER(ctypes.c_uint16)]),
     "sundog_denoiser_create": (c_p, [c_u64]),
     "sundog_denoiser_destroy": (None, [c_p]),
     "sundog_denoiser_param_count": (c_i64, [c_p]),

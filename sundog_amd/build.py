@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libsundog.so")
 
-HOST_SOURCES = ["capi_scene.cpp", "assets.cpp", "bvh.cpp", "prepare.cpp"]
+HOST_SOURCES = ["capi_scene.cpp", "assets.cpp", "bvh.cpp", "prepare.cpp", "transfer.cpp"]
 HIP_SOURCES = ["render.hip", "session.hip", "server.hip", "denoise.hip", "lbvh.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

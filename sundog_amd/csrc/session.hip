@@ -7,6 +7,7 @@
 // are owned by a session, so the same code path also serves callers that
 // keep a scene resident across many passes (sundog_session_*).
 #include "session.h"
+#include "transfer.h"
 
 using namespace sdk;
 
@@ -362,6 +363,10 @@ extern "C" int sundog_render(sundog_scene* h, const sundog_render_options* opt) 
     if (!denoised) sundog_session_download(s.get(), img.data(), nullptr, nullptr);
     std::string out = pfmPath(opt->out_path);
     writePfm(out, img, rs.width, rs.height);
+    // display-referred codes of the reference's AVIF (PQ / BT.2020, 12-bit)
+    std::vector<uint16_t> pq(npx * 3);
+    encodePq12(img.data(), rs.width, rs.height, rs.exposure, pq.data());
+    writePpm12(out.substr(0, out.size() - 4) + ".pq.ppm", pq.data(), rs.width, rs.height);
     if (opt->aov_albedo_path) {
       sundog_session_download(s.get(), nullptr, img.data(), nullptr);
       writePfm(pfmPath(opt->aov_albedo_path), img, rs.width, rs.height);

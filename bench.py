@@ -13,7 +13,7 @@ sides). The roofline entry prices the render kernel's algorithmic traversal
 FLOPs (12 per box test, 40 per triangle test, 30 per quadric test, 30 per
 instance transform; counted by an instrumented pass) against the FP32
 vector peak; the kernel time comes from HIP events on the session stream.
-cpu_baseline times the CPU checker on a bounded sample of the same frame.
+cpu_baseline times the CPU oracle on a bounded sample of the same frame.
 """
 
 import argparse
@@ -86,7 +86,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the render path shards by pixel tiles, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(the render path would shard by pixel tiles, one set of tiles per GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -19,7 +19,7 @@
  * Part 2 holds additive entry points with no counterpart in the reference:
  * resident render sessions (scene data and film kept in HBM across passes),
  * ray-query and sampling probes used by parity tests, and the flat scene
- * view that the CPU checker consumes. They change no Part-1 symbol.
+ * view that the CPU oracle consumes. They change no Part-1 symbol.
  */
 #ifndef SUNDOG_API_H
 #define SUNDOG_API_H

@@ -1,7 +1,7 @@
 """CPU restatement of the learned denoiser (test infrastructure only).
 
 Only tests/, the package smoke check and bench.py's CPU-baseline leg use
-this module, and only as the checker of the HIP kernels in
+this module, and only as the oracle of the HIP kernels in
 sundog_amd/csrc/denoise.hip; nothing shipped imports it.
 
 The reference denoises through the vendor AI denoiser

@@ -1,4 +1,4 @@
-// CPU checker for the sundog render path (TEST INFRASTRUCTURE ONLY; see
+// CPU oracle for the sundog render path (TEST INFRASTRUCTURE ONLY; see
 // oracle.h). Scalar C++17 + OpenMP. Every function cites the reference
 // source it restates. Compiled with -ffp-contract=off so that the
 // geometry functions reproduce the GPU's bits (the GPU disables contraction
@@ -1885,7 +1885,7 @@ extern "C" float oracle_flame_transmittance(const sundog_flat_scene* f, const fl
 }
 
 // ---------------------------------------------------------------------------
-// probes used by the checker's own tests
+// probes used by the oracle's own tests
 // ---------------------------------------------------------------------------
 
 extern "C" int oracle_env_sample(const sundog_flat_scene* f, float u0, float u1, float wi_out[3],

@@ -1,8 +1,8 @@
-/* oracle.h — CPU checker for the sundog render path.
+/* oracle.h — CPU oracle for the sundog render path.
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (sundog_amd/, libsundog.so)
  * includes, links or calls this code; only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg use it, as the checker.
+ * and bench.py's cpu_baseline leg use it, as the oracle.
  *
  * The oracle is a scalar C++17 + OpenMP restatement of the reference render
  * path (device/programs.cu:311-572 and the device helper headers), consuming

@@ -1,4 +1,4 @@
-"""ctypes wrapper of the CPU checker (oracle/_build/liboracle.so).
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product.

@@ -447,7 +447,7 @@ class SceneHandle(object):
 
 
 class FlatScene(object):
-    """Owns a sundog_flat_scene* (read-only view for the CPU checker)."""
+    """Owns a sundog_flat_scene* (read-only view for the CPU oracle)."""
 
     def __init__(self, ptr, lib):
         self.ptr = ptr

@@ -1,4 +1,4 @@
-"""GPU parity: the HIP path through the C ABI against the CPU checker.
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
 
 Bit-exact: PCG32 streams, closest-hit (t, instance, primitive) from the BVH
 against brute force on both sides, environment-CDF bins. Within tolerance:
@@ -46,7 +46,7 @@ def _rays_looking_at(n, seed, eye_lo, eye_hi, target_lo, target_hi):
     return r
 
 
-def test_device_pcg32_matches_checker_and_published_vectors(orc):
+def test_device_pcg32_matches_oracle_and_published_vectors(orc):
     dev = sd.device_pcg32(42, 54, 6)
     ref = orc.pcg32(42, 54, 6)
     gold = np.load(os.path.join(GOLDEN, "pcg32_demo.npz"))["round1"]

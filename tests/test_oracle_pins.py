@@ -1,5 +1,5 @@
-"""Pins of the CPU checker against fixed points that do not come from the
-checker itself: published generator vectors, closed forms, invariants and
+"""Pins of the CPU oracle against fixed points that do not come from the
+oracle itself: published generator vectors, closed forms, invariants and
 brute force. Each test names what would fail if a term were dropped, a sign
 flipped or an index transposed. CPU only.
 """

@@ -68,8 +68,10 @@ constexpr int kTile = 32;  // MFMA 32x32x16
 // in: [P][Cin] (BF16, or FP32 when kInF32), w: [Cout][9][Cin] BF16.
 // out[p][co] = bias[co] + sum_t sum_ci in[p + tapOff(t)][ci] * w[co][t][ci];
 // tapSign = +1 for forward, -1 for backward-data (the transposed conv reads
-// the neighbour on the opposite side). Cin % 16 == 0, Cout % 32 == 0.
-template <bool kInF32, bool kOutF32>
+// the neighbour on the opposite side). Cin % 16 == 0, Cout % (32 * kNT) == 0.
+// Each wave owns 32 pixels x (32 * kNT) output channels: one activation
+// fragment per K step feeds kNT MFMAs.
+template <bool kInF32, bool kOutF32, int kNT>
 __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, const uint16_t* __restrict__ w,
                                               const float* __restrict__ bias, void* __restrict__ outV, int N,
                                               int H, int W, int Cin, int Cout, int tapSign, int relu) {
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
   const int wave = threadIdx.x >> 6;
   const int P = N * H * W;
   const int m0 = (blockIdx.x * 4 + wave) * kTile;
-  const int co0 = blockIdx.y * kTile;
+  const int co0 = blockIdx.y * kTile * kNT;
   if (m0 >= P) return;
   const int row = lane & 31, half = lane >> 5;
   const int p = m0 + row;
@@ -85,7 +87,10 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
   const int n = pv ? p / (H * W) : 0;
   const int yx = pv ? p - n * H * W : 0;
   const int y = yx / W, x = yx - (yx / W) * W;
-  f32x16 acc = {};
+  f32x16 acc[kNT];
+#pragma unroll
+  for (int j = 0; j < kNT; j++) acc[j] = f32x16{};
+  const size_t wStride = (size_t)kTile * 9 * Cin;  // between output-channel tiles
   const uint16_t* wrow = w + (size_t)(co0 + row) * 9 * Cin;
   for (int t = 0; t < 9; t++) {
     const int sy = y + tapSign * (t / 3 - 1), sx = x + tapSign * (t % 3 - 1);
@@ -102,23 +107,55 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
           a = *(const u16x8*)((const uint16_t*)inV + src * Cin + c + 8 * half);
         }
       }
-      u16x8 b = *(const u16x8*)(wrow + (size_t)t * Cin + c + 8 * half);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc, 0, 0, 0);
-    }
-  }
-  const int co = co0 + row;  // D column = lane % 32
-  const float bv = bias ? bias[co] : 0.0f;
+      const uint16_t* wk = wrow + (size_t)t * Cin + c + 8 * half;
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    int i = 8 * (r / 4) + 4 * half + (r % 4);
-    int pp = m0 + i;
-    if (pp < P) {
-      float v = acc[r] + bv;
-      if (relu) v = fmaxf(v, 0.0f);
-      if (kOutF32) ((float*)outV)[(size_t)pp * Cout + co] = v;
-      else ((uint16_t*)outV)[(size_t)pp * Cout + co] = f2bf(v);
+      for (int j = 0; j < kNT; j++) {
+        u16x8 b = *(const u16x8*)(wk + j * wStride);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[j], 0, 0, 0);
+      }
     }
   }
+#pragma unroll
+  for (int j = 0; j < kNT; j++) {
+    const int co = co0 + j * kTile + row;  // D column = lane % 32
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      int i = 8 * (r / 4) + 4 * half + (r % 4);
+      int pp = m0 + i;
+      if (pp < P) {
+        float v = acc[j][r] + bv;
+        if (relu) v = fmaxf(v, 0.0f);
+        if (kOutF32) ((float*)outV)[(size_t)pp * Cout + co] = v;
+        else ((uint16_t*)outV)[(size_t)pp * Cout + co] = f2bf(v);
+      }
+    }
+  }
+}
+
+// Output-channel tiles per wave for a convolution with `tiles` 32-wide tiles:
+// the largest divisor up to 5 (more accumulators cost occupancy: 5 tiles take
+// ~172 registers, 2 waves per SIMD).
+static int convTilesPerWave(int tiles) {
+  for (int k : {5, 4, 3, 2})
+    if (tiles % k == 0) return k;
+  return 1;
+}
+
+template <bool kInF32, bool kOutF32>
+static void launchConv(int nt, dim3 grid, hipStream_t s, const void* in, const uint16_t* w, const float* b, void* out,
+                       int N, int H, int W, int Cin, int Cout, int tapSign, int relu) {
+#define SD_CONV(K)                                                                                             \
+  case K:                                                                                                      \
+    hipLaunchKernelGGL((k_conv<kInF32, kOutF32, K>), grid, dim3(256), 0, s, in, w, b, out, N, H, W, Cin, Cout, \
+                       tapSign, relu);                                                                         \
+    break;
+  switch (nt) {
+    SD_CONV(1) SD_CONV(2) SD_CONV(3) SD_CONV(4) SD_CONV(5)
+    default:
+      throw std::runtime_error("denoiser: unsupported tile count");
+  }
+#undef SD_CONV
 }
 
 // ---------------------------------------------------------------------------
@@ -539,14 +576,15 @@ struct sundog_denoiser {
   void conv(int k, const void* in, bool inF32, void* out, bool outF32, int h, int w) {
     const Conv& c = net.convs[k];
     int P = N * h * w;
-    dim3 grid((P + 127) / 128, c.cout / 32);
+    const int nt = convTilesPerWave(c.cout / 32);
+    dim3 grid((P + 127) / 128, c.cout / (32 * nt));
     const float* b = net.params.as<float>() + c.bOff;
     if (outF32)
-      hipLaunchKernelGGL((k_conv<false, true>), grid, dim3(256), 0, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w,
-                         c.cin, c.cout, 1, c.relu ? 1 : 0);
+      launchConv<false, true>(nt, grid, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w, c.cin, c.cout, 1,
+                              c.relu ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_conv<false, false>), grid, dim3(256), 0, stream, in, c.wf.as<uint16_t>(), b, out, N, h,
-                         w, c.cin, c.cout, 1, c.relu ? 1 : 0);
+      launchConv<false, false>(nt, grid, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w, c.cin, c.cout, 1,
+                               c.relu ? 1 : 0);
     (void)inF32;
     HIP_OK(hipGetLastError());
   }
@@ -617,10 +655,10 @@ struct sundog_denoiser {
   void convGradX(int k, const float* dy, float* dx, int h, int w) {
     const Conv& c = net.convs[k];
     int P = N * h * w;
-    dim3 grid((P + 127) / 128, c.cin / 32 > 0 ? c.cin / 32 : 1);
     if (c.cin % 32) throw std::runtime_error("denoiser: backward-data needs Cin % 32 == 0");
-    hipLaunchKernelGGL((k_conv<true, true>), grid, dim3(256), 0, stream, dy, c.wt.as<uint16_t>(), nullptr, dx, N,
-                       h, w, c.cout, c.cin, -1, 0);
+    const int nt = convTilesPerWave(c.cin / 32);
+    dim3 grid((P + 127) / 128, c.cin / (32 * nt));
+    launchConv<true, true>(nt, grid, stream, dy, c.wt.as<uint16_t>(), nullptr, dx, N, h, w, c.cout, c.cin, -1, 0);
     HIP_OK(hipGetLastError());
   }
 

@@ -46,3 +46,22 @@ def test_wavefront_progressive_passes(asset_dir):
     (one, _, _), _ = _render(sc, "wavefront", [(0, 24)])
     (many, _, _), _ = _render(sc, "wavefront", [(0, 5), (5, 7), (12, 1), (13, 11)])
     assert np.array_equal(one.view(np.uint32), many.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["cornell", "instanced_mini", "volumes_mini"])
+def test_occupancy_variants_bit_identical(name, asset_dir):
+    """The megakernel's occupancy variants (film running means in registers
+    at 2 waves per SIMD, folded in memory at 3 to 5) give the same bits."""
+    sc = _scene(name, asset_dir)
+    spp = sc.session().spp
+    ref = None
+    for waves in (2, 3, 4, 5):
+        s = sc.session()
+        s.tune("waves", waves)
+        s.render(0, spp)
+        films = s.download()
+        if ref is None:
+            ref = films
+        else:
+            for a, b in zip(ref, films):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), waves

@@ -1809,25 +1809,17 @@ DEV void pixelOf(int& px, int& py) {
 // +sppThisLaunch) of one pixel in order, tracing inline. A lane whose path
 // ends starts its next sample at once instead of idling until the slowest
 // path of its wave finishes; per-pixel sample order (and so the running
-// mean) is unchanged. With kFilmInRegisters the pixel's three running means
-// stay in registers for the launch; otherwise each finished sample is folded
-// into the film in memory. Both apply the same operations in the same
-// order; the memory form frees 12 registers, which keeps the high-occupancy
-// variants from spilling live state to scratch.
-template <bool kFilmInRegisters>
+// mean) is unchanged. The pixel's running means stay in registers for the
+// launch (the same operations in the same order as folding each sample in
+// memory; folding in memory instead was measured slower at every
+// occupancy and did not reduce the spill traffic of the 4-wave variant).
 DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount& wc, uint32_t& rays) {
   const int pixel = py * P.width + px;
-  float4 fAcc, fAlb, fNrm;
-  if (kFilmInRegisters) {
-    fAcc = P.accum[pixel];
-    fAlb = P.albedo[pixel];
-    fNrm = P.normal[pixel];
-  }
+  float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
   PathState ps;
   ps.si = 0;
   ps.medLo = ps.medHi = 0;
   bool alive = pathBegin(P, ps, px, py);
-  bool firstHit = true;
   while (alive) {
     Hit hit;
     vec3 oObj = v3(0.0f), dObj = v3(0.0f);
@@ -1835,12 +1827,6 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
     rays++;
     SurfState sf;
     int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
-    if (!kFilmInRegisters && firstHit) {
-      // the sample's first-hit AOVs are final after its first query
-      P.albedo[pixel] = runningMean(P.albedo[pixel], ps.aovA, ps.s);
-      P.normal[pixel] = runningMean(P.normal[pixel], ps.aovN, ps.s);
-      firstHit = false;
-    }
     if (r == PN_SHADOW) {
       vec3 so, wi;
       float tmaxS;
@@ -1850,22 +1836,15 @@ DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount
       r = pathOnShadow(P, ps, sf, tr);
     }
     if (r == PN_END) {
-      if (kFilmInRegisters) {
-        fAcc = runningMean(fAcc, sanitize(ps.L), ps.s);
-        fAlb = runningMean(fAlb, ps.aovA, ps.s);
-        fNrm = runningMean(fNrm, ps.aovN, ps.s);
-      } else {
-        P.accum[pixel] = runningMean(P.accum[pixel], sanitize(ps.L), ps.s);
-      }
+      fAcc = runningMean(fAcc, sanitize(ps.L), ps.s);
+      fAlb = runningMean(fAlb, ps.aovA, ps.s);
+      fNrm = runningMean(fNrm, ps.aovN, ps.s);
       alive = pathBegin(P, ps, px, py);
-      firstHit = true;
     }
   }
-  if (kFilmInRegisters) {
-    P.accum[pixel] = fAcc;
-    P.albedo[pixel] = fAlb;
-    P.normal[pixel] = fNrm;
-  }
+  P.accum[pixel] = fAcc;
+  P.albedo[pixel] = fAlb;
+  P.normal[pixel] = fNrm;
 }
 
 // Wave-level sums of the ray / work counters, one 64-bit atomic per wave.
@@ -1907,7 +1886,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
     st.stride = P.spillStride;
     st.sp = 0;
     st.cap = P.stackLds;
-    renderPixel<(kWaves <= 2)>(P, px, py, st, wc, rays);
+    renderPixel(P, px, py, st, wc, rays);
   }
   flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
 }
@@ -1998,7 +1977,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
     for (int base = 0; base < pxPerTile; base += 64) {
       int k = base + lane;
       int px = tx + (k % tileSize), py = ty + (k / tileSize);
-      if (k < pxPerTile && px < P.width && py < P.height) renderPixel<(kWaves <= 2)>(P, px, py, st, wc, rays);
+      if (k < pxPerTile && px < P.width && py < P.height) renderPixel(P, px, py, st, wc, rays);
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);

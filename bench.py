@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--workload", default="instanced", choices=["instanced", "cornell", "volumes"],
+                    help="the headline instanced-icosphere frame (default), or the Cornell-box and "
+                         "volumes-and-environment frames")
     ap.add_argument("--waves", type=int, default=4, choices=[2, 3, 4, 5],
                     help="minimum waves per SIMD the megakernel's register budget allows")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
@@ -93,10 +96,23 @@ def main():
     import sundog_amd as sd
     from sundog_amd import configs
 
-    W, H, SPP, DEPTH = 1920, 1080, 256, 12
     t0 = time.time()
-    scene = configs.instanced(args.asset_dir, width=W, height=H, spp=SPP, max_depth=DEPTH,
-                              instances=args.instances).build()
+    if args.workload == "instanced":
+        W, H, SPP, DEPTH = 1920, 1080, 256, 12
+        scene = configs.instanced(args.asset_dir, width=W, height=H, spp=SPP, max_depth=DEPTH,
+                                  instances=args.instances).build()
+        workload = ("instanced icospheres: %d x 20,480-tri instances, %dx%d, %d spp, depth %d"
+                    % (args.instances, W, H, SPP, DEPTH))
+    elif args.workload == "cornell":
+        W, H, SPP, DEPTH = 256, 256, 64, 8
+        scene = configs.cornell(width=W, height=H, spp=SPP, max_depth=DEPTH).build()
+        workload = "Cornell box (5 rects, lambert + glass spheres, rect light): %dx%d, %d spp, depth %d" % (
+            W, H, SPP, DEPTH)
+    else:
+        W, H, SPP, DEPTH = 1280, 720, 512, 16
+        scene = configs.volumes(args.asset_dir, width=W, height=H, spp=SPP, max_depth=DEPTH).build()
+        workload = ("volumes + environment (2048x1024 sky, 5-octave flame box, water, rough glass, transparent "
+                    "shadows): %dx%d, %d spp, depth %d" % (W, H, SPP, DEPTH))
     sess = scene.session()
     sess.tune("waves", args.waves)
     sess.set_kernel(args.kernel)
@@ -161,8 +177,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seed-generated scene: icosphere OBJ + PCG32 layout)",
         "config": {
-            "workload": "instanced icospheres: %d x 20,480-tri instances, %dx%d, %d spp, depth %d"
-                        % (args.instances, W, H, SPP, DEPTH),
+            "workload": workload,
             "step": "one %d-spp progressive pass over the full frame" % k,
             "kernel": args.kernel,
             "waves_per_simd": args.waves,
@@ -176,7 +191,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": measured_traffic(kname),
+            "traffic": measured_traffic(kname) if args.workload == "instanced" else None,
             "peak_packed_fp32": FP32_PEAK_PACKED_TFLOPS,
             "kernel": kname,
             "kernel_ms_avg": kernel_ms_avg,
@@ -202,8 +217,8 @@ def main():
             "unit": "paths/s",
             "cores": cst["threads"],
             "kind": "port",
-            "sample": "every %dth 16x16 tile of the same 1080p frame (%d of %d tiles), samples 0..%d; "
-                      "%.1f s; %.2f Mrays/s" % (args.cpu_tile_stride, (tiles + args.cpu_tile_stride - 1)
+            "sample": "every %dth 16x16 tile of the same %dx%d frame (%d of %d tiles), samples 0..%d; "
+                      "%.1f s; %.2f Mrays/s" % (args.cpu_tile_stride, W, H, (tiles + args.cpu_tile_stride - 1)
                                                // args.cpu_tile_stride, tiles, args.cpu_spp - 1,
                                                cst["seconds"],
                                                (cst["radiance_rays"] + cst["shadow_rays"]) / cst["seconds"] / 1e6),

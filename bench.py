@@ -103,16 +103,19 @@ def main():
                                   instances=args.instances).build()
         workload = ("instanced icospheres: %d x 20,480-tri instances, %dx%d, %d spp, depth %d"
                     % (args.instances, W, H, SPP, DEPTH))
+        data = "synthetic (seed-generated scene: icosphere OBJ + PCG32 layout)"
     elif args.workload == "cornell":
         W, H, SPP, DEPTH = 256, 256, 64, 8
         scene = configs.cornell(width=W, height=H, spp=SPP, max_depth=DEPTH).build()
         workload = "Cornell box (5 rects, lambert + glass spheres, rect light): %dx%d, %d spp, depth %d" % (
             W, H, SPP, DEPTH)
+        data = "synthetic (the fixed Cornell layout, no assets)"
     else:
         W, H, SPP, DEPTH = 1280, 720, 512, 16
         scene = configs.volumes(args.asset_dir, width=W, height=H, spp=SPP, max_depth=DEPTH).build()
         workload = ("volumes + environment (2048x1024 sky, 5-octave flame box, water, rough glass, transparent "
                     "shadows): %dx%d, %d spp, depth %d" % (W, H, SPP, DEPTH))
+        data = "synthetic (procedural 2048x1024 sky image written as .hdr, fixed layout)"
     sess = scene.session()
     sess.tune("waves", args.waves)
     sess.set_kernel(args.kernel)
@@ -175,7 +178,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seed-generated scene: icosphere OBJ + PCG32 layout)",
+        "data": data,
         "config": {
             "workload": workload,
             "step": "one %d-spp progressive pass over the full frame" % k,

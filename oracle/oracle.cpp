@@ -824,6 +824,62 @@ BsdfSample bsdfSample(const sundog_flat_material& m, V albedo, V rayDir, V n, bo
 // ---------------------------------------------------------------------------
 // lights (device/light_sample.cuh:24-226)
 // ---------------------------------------------------------------------------
+// Mesh-light sample for fixed uniforms (src/capi_render.cpp:119-172 builds
+// the area CDF; device/light_sample.cuh:94-154 samples it): uc picks a
+// triangle with probability area/total by the CDF, (r0, r1) a uniform point in
+// it by the square-root warp; the solid-angle pdf is d^2 / (|cos_l| * total).
+LightSample sampleMeshLightU(const Scene& S, int li_, V x, float uc, float r0, float r1, int* triOut) {
+  const sundog_flat_light& lt = S.f->lights[li_];
+  LightSample s;
+  const MeshLightData& ml = S.meshLight[li_];
+  const sundog_flat_mesh& mesh = S.f->meshes[ml.mesh];
+  int lo = 0, hi = mesh.num_tris;
+  while (lo + 1 < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ml.cdf[mid] <= uc) lo = mid;
+    else hi = mid;
+  }
+  if (triOut) *triOut = lo;
+  uint32_t i0 = mesh.indices[3 * lo], i1 = mesh.indices[3 * lo + 1], i2 = mesh.indices[3 * lo + 2];
+  V p0 = ml.wp[i0], p1 = ml.wp[i1], p2 = ml.wp[i2];
+  float su = sqrtf(r0);
+  float bx = su * (1.0f - r1);
+  float by = su * r1;
+  V p = p0 * (1.0f - bx - by) + p1 * bx + p2 * by;
+  V ngU = cross(p1 - p0, p2 - p0);
+  float ngL2 = len2(ngU);
+  if (ngL2 < 1e-20f) return s;
+  V ng = ngU * (ml.ngSign / sqrtf(ngL2));
+  V to = p - x;
+  float d2 = len2(to);
+  if (d2 < 1e-10f) return s;
+  float dist = sqrtf(d2);
+  V wi = to / dist;
+  float cosL = -dot(wi, ng);
+  if (!lt.two_sided && cosL <= 0.0f) return s;
+  cosL = fabsf(cosL);
+  if (cosL < 1e-6f) return s;
+  s.wi = wi;
+  s.dist = dist;
+  s.Li = ld(lt.L);
+  if (lt.tex_id >= 0) {
+    float tu, tv;
+    if (mesh.uvs) {
+      tu = mesh.uvs[2 * i0] * (1.0f - bx - by) + mesh.uvs[2 * i1] * bx + mesh.uvs[2 * i2] * by;
+      tv = mesh.uvs[2 * i0 + 1] * (1.0f - bx - by) + mesh.uvs[2 * i1 + 1] * bx + mesh.uvs[2 * i2 + 1] * by;
+    } else {
+      tu = bx;
+      tv = by;
+    }
+    float c[4];
+    evalTexture(S.f->textures[lt.tex_id], tu, tv, c);
+    s.Li = ld(lt.L) * mk(c[0], c[1], c[2]);
+  }
+  s.pdf = d2 / (cosL * ml.area);
+  s.valid = true;
+  return s;
+}
+
 LightSample sampleLight(const Scene& S, int li_, V x, Rng& rng) {
   const sundog_flat_light& lt = S.f->lights[li_];
   LightSample s;
@@ -893,55 +949,11 @@ LightSample sampleLight(const Scene& S, int li_, V x, Rng& rng) {
       return s;
     }
     case 5: {
-      const MeshLightData& ml = S.meshLight[li_];
-      const sundog_flat_mesh& mesh = S.f->meshes[ml.mesh];
+      // three draws in this order: triangle pick, then the two barycentric
       float uc = rng.rnd();
-      int lo = 0, hi = mesh.num_tris;
-      while (lo + 1 < hi) {
-        int mid = (lo + hi) >> 1;
-        if (ml.cdf[mid] <= uc) lo = mid;
-        else hi = mid;
-      }
-      uint32_t i0 = mesh.indices[3 * lo], i1 = mesh.indices[3 * lo + 1], i2 = mesh.indices[3 * lo + 2];
-      V p0 = ml.wp[i0], p1 = ml.wp[i1], p2 = ml.wp[i2];
       float r0 = rng.rnd();
       float r1 = rng.rnd();
-      float su = sqrtf(r0);
-      float bx = su * (1.0f - r1);
-      float by = su * r1;
-      V p = p0 * (1.0f - bx - by) + p1 * bx + p2 * by;
-      V ngU = cross(p1 - p0, p2 - p0);
-      float ngL2 = len2(ngU);
-      if (ngL2 < 1e-20f) return s;
-      V ng = ngU * (ml.ngSign / sqrtf(ngL2));
-      V to = p - x;
-      float d2 = len2(to);
-      if (d2 < 1e-10f) return s;
-      float dist = sqrtf(d2);
-      V wi = to / dist;
-      float cosL = -dot(wi, ng);
-      if (!lt.two_sided && cosL <= 0.0f) return s;
-      cosL = fabsf(cosL);
-      if (cosL < 1e-6f) return s;
-      s.wi = wi;
-      s.dist = dist;
-      s.Li = ld(lt.L);
-      if (lt.tex_id >= 0) {
-        float tu, tv;
-        if (mesh.uvs) {
-          tu = mesh.uvs[2 * i0] * (1.0f - bx - by) + mesh.uvs[2 * i1] * bx + mesh.uvs[2 * i2] * by;
-          tv = mesh.uvs[2 * i0 + 1] * (1.0f - bx - by) + mesh.uvs[2 * i1 + 1] * bx + mesh.uvs[2 * i2 + 1] * by;
-        } else {
-          tu = bx;
-          tv = by;
-        }
-        float c[4];
-        evalTexture(S.f->textures[lt.tex_id], tu, tv, c);
-        s.Li = ld(lt.L) * mk(c[0], c[1], c[2]);
-      }
-      s.pdf = d2 / (cosL * ml.area);
-      s.valid = true;
-      return s;
+      return sampleMeshLightU(S, li_, x, uc, r0, r1, nullptr);
     }
     case 3: {
       s.isDelta = true;
@@ -1920,4 +1932,68 @@ extern "C" float oracle_march_flames(const sundog_flat_scene* f, const float o[3
   L_out[2] = L.z;
   if (draws_out) *draws_out = (int)rng.draws;
   return trans.x;
+}
+
+extern "C" int oracle_texture(const sundog_flat_scene* f, int tex, const float* uv, int n, float* out) {
+  return guard([&] {
+    if (tex < 0 || tex >= f->num_textures) throw std::runtime_error("oracle_texture: texture id out of range");
+    for (int i = 0; i < n; i++) evalTexture(f->textures[tex], uv[2 * i], uv[2 * i + 1], out + 4 * (size_t)i);
+  });
+}
+
+extern "C" void oracle_bilinear(const float* img, int w, int h, const float* uv, int n, int wrap_v, float* out) {
+  for (int i = 0; i < n; i++) bilinear(img, w, h, uv[2 * i], uv[2 * i + 1], wrap_v != 0, out + 4 * (size_t)i);
+}
+
+extern "C" void oracle_noise(const float* p, int n, uint32_t seed, int octaves, float* out) {
+  for (int i = 0; i < n; i++) {
+    V q = ld(p + 3 * (size_t)i);
+    out[i] = octaves > 0 ? fbmN(q, seed, octaves) : valueNoise(q, seed);
+  }
+}
+
+extern "C" void oracle_water_normal(const float x[3], const float ns[3], const float wo[3], float amp, float freq,
+                                    int octaves, float out[3]) {
+  V n = waterNormal(ld(x), ld(ns), ld(wo), amp, freq, octaves);
+  out[0] = n.x;
+  out[1] = n.y;
+  out[2] = n.z;
+}
+
+extern "C" int oracle_flame_field(const sundog_flat_scene* f, int flame, const float* p, int n, float* sigma_out,
+                                  float* emission_out) {
+  return guard([&] {
+    if (flame < 0 || flame >= f->num_flames) throw std::runtime_error("oracle_flame_field: flame id out of range");
+    for (int i = 0; i < n; i++) {
+      float sg;
+      V em;
+      flameField(f->flames[flame], ld(p + 3 * (size_t)i), sg, em);
+      sigma_out[i] = sg;
+      emission_out[3 * i] = em.x;
+      emission_out[3 * i + 1] = em.y;
+      emission_out[3 * i + 2] = em.z;
+    }
+  });
+}
+
+extern "C" int oracle_mesh_light_sample(const sundog_flat_scene* f, int light, const float x[3], const float* u,
+                                        int n, float* wi_out, float* dist_out, float* pdf_out, int32_t* tri_out,
+                                        float* area_out) {
+  return guard([&] {
+    if (light < 0 || light >= f->num_lights) throw std::runtime_error("oracle_mesh_light_sample: light id out of range");
+    Scene S;
+    setupScene(f, S);
+    if (S.meshLight[light].mesh < 0) throw std::runtime_error("oracle_mesh_light_sample: not a mesh light");
+    *area_out = S.meshLight[light].area;
+    for (int i = 0; i < n; i++) {
+      int tri = -1;
+      LightSample s = sampleMeshLightU(S, light, ld(x), u[3 * i], u[3 * i + 1], u[3 * i + 2], &tri);
+      wi_out[3 * i] = s.wi.x;
+      wi_out[3 * i + 1] = s.wi.y;
+      wi_out[3 * i + 2] = s.wi.z;
+      dist_out[i] = s.dist;
+      pdf_out[i] = s.valid ? s.pdf : 0.0f;
+      tri_out[i] = tri;
+    }
+  });
 }

@@ -68,6 +68,23 @@ int oracle_env_sample(const sundog_flat_scene* s, float u0, float u1, float wi_o
 float oracle_march_flames(const sundog_flat_scene* s, const float o[3], const float d[3], float tend,
                           uint64_t seed, uint64_t seq, float trans_out[3], float L_out[3],
                           int* draws_out);
+/* Texture `tex` of the flat scene at n (u, v) pairs: n x 4 floats. */
+int oracle_texture(const sundog_flat_scene* s, int tex, const float* uv, int n, float* out);
+/* The bilinear image filter alone (wrap in u; wrap or clamp in v). */
+void oracle_bilinear(const float* img, int w, int h, const float* uv, int n, int wrap_v, float* out);
+/* Value noise (octaves = 0) or fbm of `octaves` octaves at n points. */
+void oracle_noise(const float* p, int n, uint32_t seed, int octaves, float* out);
+/* The water material's perturbed shading normal. */
+void oracle_water_normal(const float x[3], const float ns[3], const float wo[3], float amp, float freq,
+                         int octaves, float out[3]);
+/* Flame density and emission at n points (sigma n, emission n x 3). */
+int oracle_flame_field(const sundog_flat_scene* s, int flame, const float* p, int n, float* sigma_out,
+                       float* emission_out);
+/* Mesh-light samples for n fixed uniform triples (pick, r0, r1) seen from x:
+ * direction, distance, solid-angle pdf (0 when invalid), picked triangle, and
+ * the light's total world-space area. */
+int oracle_mesh_light_sample(const sundog_flat_scene* s, int light, const float x[3], const float* u, int n,
+                             float* wi_out, float* dist_out, float* pdf_out, int32_t* tri_out, float* area_out);
 /* Last error message. */
 const char* oracle_last_error(void);
 

@@ -79,6 +79,16 @@ def load():
     lib.oracle_march_flames.restype = ctypes.c_float
     lib.oracle_march_flames.argtypes = [ctypes.c_void_p, P_f, P_f, ctypes.c_float, ctypes.c_uint64,
                                         ctypes.c_uint64, P_f, P_f, P_i32]
+    lib.oracle_texture.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, ctypes.c_int, P_f]
+    lib.oracle_bilinear.argtypes = [P_f, ctypes.c_int, ctypes.c_int, P_f, ctypes.c_int, ctypes.c_int, P_f]
+    lib.oracle_bilinear.restype = None
+    lib.oracle_noise.argtypes = [P_f, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, P_f]
+    lib.oracle_noise.restype = None
+    lib.oracle_water_normal.argtypes = [P_f, P_f, P_f, ctypes.c_float, ctypes.c_float, ctypes.c_int, P_f]
+    lib.oracle_water_normal.restype = None
+    lib.oracle_flame_field.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, ctypes.c_int, P_f, P_f]
+    lib.oracle_mesh_light_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, P_f, ctypes.c_int, P_f, P_f,
+                                             P_f, P_i32, P_f]
     lib.oracle_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -221,3 +231,69 @@ def march_flames(flat, o, d, tend, seed, seq):
     lib.oracle_march_flames(flat.ptr, _fp(np.asarray(o, np.float32)), _fp(np.asarray(d, np.float32)), tend,
                             seed, seq, _fp(tr), _fp(L), draws.ctypes.data_as(P_i32))
     return tr, L, int(draws[0])
+
+
+def texture(flat, tex, uv):
+    """Texture `tex` of the flat scene at (n, 2) uv pairs -> (n, 4)."""
+    lib = load()
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 4), np.float32)
+    if lib.oracle_texture(flat.ptr, tex, _fp(uv), len(uv), _fp(out)) != 0:
+        raise RuntimeError(_err(lib))
+    return out
+
+
+def bilinear(img, uv, wrap_v):
+    """The bilinear filter over an (h, w, 4) float32 image at (n, 2) uv."""
+    lib = load()
+    img = np.ascontiguousarray(img, np.float32)
+    h, w = img.shape[:2]
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 4), np.float32)
+    lib.oracle_bilinear(_fp(img), w, h, _fp(uv), len(uv), int(bool(wrap_v)), _fp(out))
+    return out
+
+
+def noise(p, seed, octaves=0):
+    """Value noise (octaves=0) or fbm at (n, 3) points."""
+    lib = load()
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    out = np.zeros(len(p), np.float32)
+    lib.oracle_noise(_fp(p), len(p), seed, octaves, _fp(out))
+    return out
+
+
+def water_normal(x, ns, wo, amp, freq, octaves):
+    lib = load()
+    out = np.zeros(3, np.float32)
+    lib.oracle_water_normal(_fp(np.asarray(x, np.float32)), _fp(np.asarray(ns, np.float32)),
+                            _fp(np.asarray(wo, np.float32)), amp, freq, octaves, _fp(out))
+    return out
+
+
+def flame_field(flat, flame, p):
+    """(sigma (n,), emission (n, 3)) of flame `flame` at (n, 3) points."""
+    lib = load()
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    sg = np.zeros(len(p), np.float32)
+    em = np.zeros((len(p), 3), np.float32)
+    if lib.oracle_flame_field(flat.ptr, flame, _fp(p), len(p), _fp(sg), _fp(em)) != 0:
+        raise RuntimeError(_err(lib))
+    return sg, em
+
+
+def mesh_light_sample(flat, light, x, u):
+    """Mesh-light samples for (n, 3) uniform triples seen from x: dict of
+    wi (n, 3), dist, pdf, tri and the light's total area."""
+    lib = load()
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+    n = len(u)
+    wi = np.zeros((n, 3), np.float32)
+    dist = np.zeros(n, np.float32)
+    pdf = np.zeros(n, np.float32)
+    tri = np.zeros(n, np.int32)
+    area = np.zeros(1, np.float32)
+    if lib.oracle_mesh_light_sample(flat.ptr, light, _fp(np.asarray(x, np.float32)), _fp(u), n, _fp(wi),
+                                    _fp(dist), _fp(pdf), tri.ctypes.data_as(P_i32), _fp(area)) != 0:
+        raise RuntimeError(_err(lib))
+    return {"wi": wi, "dist": dist, "pdf": pdf, "tri": tri, "area": float(area[0])}

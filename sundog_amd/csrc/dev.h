@@ -106,6 +106,11 @@ struct LaunchParams {
   float4* normal;
   unsigned long long* rayCounter;
   unsigned long long* workCounters;  // node visits, triangle tests, quadric tests, instance transforms
+  // Per-sample results of the shared-sample megakernel: kPoolPlanes x
+  // kPoolSamples x 64 floats per wave of the render grid; sampleBufWaves
+  // regions (launchRender refuses a grid with more waves).
+  float* sampleBuf;
+  int64_t sampleBufWaves;
 
   CameraDev cam;
   int32_t bgKind;
@@ -151,6 +156,12 @@ struct WorkItem {
 // kernel argument, so every scene pointer is known to be global memory) and
 // reads only these per-request records from device memory.
 constexpr int kMaxScenesPerLaunch = 4;
+
+// Shared-sample megakernel: a wave draws the paths of its 64 pixels from one
+// pool of kPoolSamples samples per pixel at a time; each finished path leaves
+// kPoolPlanes floats (radiance, first-hit albedo, first-hit normal).
+constexpr int kPoolSamples = 16;
+constexpr int kPoolPlanes = 9;
 
 struct SceneTable {
   LaunchParams scene[kMaxScenesPerLaunch];

@@ -1584,12 +1584,11 @@ DEV int neeStrategies(const LaunchParams& P) { return P.numLights + (P.hasEnv ? 
 // Starts the next sample of the pixel: stratified jitter over the total
 // spp, thin-lens or pinhole camera ray (device/programs.cu:316-350). Returns
 // false when the launch's samples are all begun.
-DEV bool pathBegin(const LaunchParams& P, PathState& ps, int px, int py) {
-  if (ps.si == P.sppThisLaunch) return false;
+// Camera ray and fresh path state of sample s of pixel (px, py).
+DEV void pathStart(const LaunchParams& P, PathState& ps, int px, int py, int s) {
   const int pixel = py * P.width + px;
   const int nStrata = numStrata(P);
-  ps.s = P.sampleOffset + ps.si;
-  ps.si++;
+  ps.s = s;
   Pcg rng = Pcg::make(((uint64_t)pixel << 32) ^ (uint64_t)ps.s, (uint64_t)P.seed);
   float jx, jy;
   if (ps.s < nStrata * nStrata) {
@@ -1618,6 +1617,14 @@ DEV bool pathBegin(const LaunchParams& P, PathState& ps, int px, int py) {
   ps.aovA = v3(0.0f);
   ps.aovN = v3(0.0f);
   ps.rngState = rng.state;
+}
+
+// The next of the launch's samples of pixel (px, py), in order; false once
+// all sppThisLaunch have begun.
+DEV bool pathBegin(const LaunchParams& P, PathState& ps, int px, int py) {
+  if (ps.si == P.sppThisLaunch) return false;
+  pathStart(P, ps, px, py, P.sampleOffset + ps.si);
+  ps.si++;
   return true;
 }
 
@@ -1798,21 +1805,10 @@ DEV int pathOnShadow(const LaunchParams& P, PathState& ps, const SurfState& sf, 
   return r;
 }
 
-// 16x16-pixel tiles; each wave64 covers an 8x8 quarter.
-DEV void pixelOf(int& px, int& py) {
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-}
-
-// Megakernel driver: one work-item renders samples [sampleOffset,
-// +sppThisLaunch) of one pixel in order, tracing inline. A lane whose path
-// ends starts its next sample at once instead of idling until the slowest
-// path of its wave finishes; per-pixel sample order (and so the running
-// mean) is unchanged. The pixel's running means stay in registers for the
-// launch (the same operations in the same order as folding each sample in
-// memory; folding in memory instead was measured slower at every
-// occupancy and did not reduce the spill traffic of the 4-wave variant).
+// Per-pixel driver (the persistent tile-queue kernel): one work-item renders
+// samples [sampleOffset, +sppThisLaunch) of one pixel in order, tracing
+// inline. A lane whose path ends starts its pixel's next sample at once; the
+// pixel's running means stay in registers for the launch.
 DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount& wc, uint32_t& rays) {
   const int pixel = py * P.width + px;
   float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
@@ -1870,24 +1866,113 @@ DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned 
   }
 }
 
+// Shared-sample driver for one wave of the render grid: the wave's 64 pixels
+// x kPoolSamples samples form one pool of paths, and a lane whose path ends
+// takes the next path of the pool from an LDS counter, whatever its pixel.
+// The wave then idles only for its last paths, not for the pixel whose 16
+// paths happened to be longest. Each path's sanitized radiance and first-hit
+// AOVs go to the wave's region of P.sampleBuf; once the pool is drained each
+// lane folds its own pixel's samples in sample order, the same running-mean
+// operations in the same order as a pixel rendered alone, so films are
+// bit-identical to per-pixel rendering. Launches of more than kPoolSamples
+// samples drain one pool per kPoolSamples.
+DEV void renderWavePooled(const LaunchParams& P, uint32_t* poolNext, float* waveBuf, Stack& st, WorkCount& wc,
+                          uint32_t& rays) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 16 + (wave & 1) * 8, y0 = blockIdx.y * 16 + (wave >> 1) * 8;
+  // slot j of the wave is pixel (x0 + j % 8, y0 + j / 8)
+  const int myPx = x0 + (lane & 7), myPy = y0 + (lane >> 3);
+  const bool mine = myPx < P.width && myPy < P.height;
+  for (int c0 = 0; c0 < P.sppThisLaunch; c0 += kPoolSamples) {
+    const int n = min(kPoolSamples, P.sppThisLaunch - c0);
+    const int items = n * 64;  // item = sample-in-pool * 64 + slot
+    if (lane == 0) *poolNext = 64;  // item `lane` is each lane's first
+    int item = lane;
+    PathState ps;
+    ps.si = 0;
+    ps.medLo = ps.medHi = 0;
+    // claim items until one lies inside the image (or the pool is empty)
+    auto claim = [&]() -> bool {
+      while (item < items) {
+        const int j = item & 63;
+        const int px = x0 + (j & 7), py = y0 + (j >> 3);
+        if (px < P.width && py < P.height) {
+          pathStart(P, ps, px, py, P.sampleOffset + c0 + (item >> 6));
+          return true;
+        }
+        item = (int)atomicAdd(poolNext, 1u);
+      }
+      return false;
+    };
+    bool alive = claim();
+    while (alive) {
+      Hit hit;
+      vec3 oObj = v3(0.0f), dObj = v3(0.0f);
+      bool isHit = traceClosest(P, st, ps.o, ps.d, 0.0f, 1e16f, hit, oObj, dObj, wc);
+      rays++;
+      SurfState sf;
+      int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
+      if (r == PN_SHADOW) {
+        vec3 so, wi;
+        float tmaxS;
+        shadowRay(sf, so, wi, tmaxS);
+        rays++;
+        vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
+        r = pathOnShadow(P, ps, sf, tr);
+      }
+      if (r == PN_END) {
+        const vec3 L = sanitize(ps.L);
+        const float v[kPoolPlanes] = {L.x, L.y, L.z, ps.aovA.x, ps.aovA.y, ps.aovA.z, ps.aovN.x, ps.aovN.y, ps.aovN.z};
+#pragma unroll
+        for (int k = 0; k < kPoolPlanes; k++) waveBuf[k * (kPoolSamples * 64) + item] = v[k];
+        item = (int)atomicAdd(poolNext, 1u);
+        alive = claim();
+      }
+    }
+    // every lane of the wave is past its last path: make the wave's
+    // per-sample stores visible to the lane that folds each pixel
+    __threadfence_block();
+    // (the film is only touched here, so no film value is live while paths
+    // are traced)
+    if (mine) {
+      const int pixel = myPy * P.width + myPx;
+      float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
+      for (int si = 0; si < n; si++) {
+        float x[kPoolPlanes];
+#pragma unroll
+        for (int k = 0; k < kPoolPlanes; k++) x[k] = waveBuf[k * (kPoolSamples * 64) + si * 64 + lane];
+        const int s = P.sampleOffset + c0 + si;
+        fAcc = runningMean(fAcc, vec3{x[0], x[1], x[2]}, s);
+        fAlb = runningMean(fAlb, vec3{x[3], x[4], x[5]}, s);
+        fNrm = runningMean(fNrm, vec3{x[6], x[7], x[8]}, s);
+      }
+      P.accum[pixel] = fAcc;
+      P.albedo[pixel] = fAlb;
+      P.normal[pixel] = fNrm;
+    }
+    // the next pool reuses the counter and the buffer region
+    __threadfence_block();
+  }
+}
+
 // kWaves: minimum waves per SIMD the register allocation must allow (2 fits
 // the kernel without spills; 3 and 4 trade scratch spills for occupancy).
 template <bool kCountWork, int kWaves>
 __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
   extern __shared__ uint64_t s_stack[];  // P.stackLds * kBlock entries
-  int px, py;
-  pixelOf(px, py);
+  __shared__ uint32_t s_poolNext[kBlock / 64];
   uint32_t rays = 0;
   WorkCount wc;
-  if (px < P.width && py < P.height) {
-    Stack st;
-    st.lds = ldsColumn(s_stack);
-    st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
-    st.stride = P.spillStride;
-    st.sp = 0;
-    st.cap = P.stackLds;
-    renderPixel(P, px, py, st, wc, rays);
-  }
+  Stack st;
+  st.lds = ldsColumn(s_stack);
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+  st.stride = P.spillStride;
+  st.sp = 0;
+  st.cap = P.stackLds;
+  const int wave = threadIdx.x >> 6;
+  const size_t waveGlobal = (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kBlock / 64) + wave;
+  renderWavePooled(P, &s_poolNext[wave], P.sampleBuf + waveGlobal * (kPoolPlanes * kPoolSamples * 64), st, wc,
+                   rays);
   flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
 }
 
@@ -2419,6 +2504,8 @@ __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
 hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s) {
   dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
   const size_t lds = (size_t)P.stackLds * kBlock * sizeof(uint64_t);
+  // every wave of the grid owns one per-sample result region
+  if (!P.sampleBuf || P.sampleBufWaves < (int64_t)grid.x * grid.y * (kBlock / 64)) return hipErrorInvalidValue;
   if (countWork)
     hipLaunchKernelGGL((k_render<true, 2>), grid, dim3(kBlock), lds, s, P);
   else if (waves >= 5)

@@ -640,6 +640,11 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * kStackEntryBytes));
     P.spill = sp.as<uint32_t>();
     P.spillStride = (int)threads;
+    const size_t waves = threads / 64;
+    DevBuf sb;
+    sb.alloc(waves * (size_t)kPoolPlanes * kPoolSamples * 64 * sizeof(float));
+    P.sampleBuf = sb.as<float>();
+    P.sampleBufWaves = (int64_t)waves;
     HIP_OK(hipStreamSynchronize(s->stream));
     auto t = Clock::now();
     for (int done = 0; done < spp; done += 16) {

@@ -211,7 +211,7 @@ struct sundog_session {
   DevBuf textures, texels, materials, lights, flames, instances, nodes, tris, meshes;
   DevBuf positions, indices, normals, uvs, lightPos, lightCdf;
   DevBuf envTex, margCdf, condCdf;
-  DevBuf accum, albedo, normal, rayCounter, workCounters, spill;
+  DevBuf accum, albedo, normal, rayCounter, workCounters, spill, sampleBuf;
   size_t spillThreads = 0;
   // wavefront variant
   DevBuf wfState, wfClosest, wfShadow, wfBuckets, wfCnt;
@@ -241,7 +241,7 @@ struct sundog_session {
     const DevBuf* all[] = {&textures, &texels, &materials, &lights, &flames, &instances, &nodes, &tris,
                            &meshes, &positions, &indices, &normals, &uvs, &lightPos, &lightCdf, &envTex,
                            &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill,
-                           &wfState, &wfClosest, &wfShadow, &wfBuckets, &wfCnt};
+                           &sampleBuf, &wfState, &wfClosest, &wfShadow, &wfBuckets, &wfCnt};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->bytes;
     return b;
@@ -250,6 +250,18 @@ struct sundog_session {
   // Spill rows cover the deepest possible stack beyond the smallest LDS
   // stack any kernel of this session uses.
   int spillRowsFor(int ldsEntries) const { return std::max(0, stackNeed - std::min(ldsEntries, stackLdsEntries())); }
+
+  // Per-sample result region of every wave of a render grid (the
+  // shared-sample megakernel), grown on demand.
+  void ensureSampleBuf(size_t waves) {
+    const size_t need = waves * (size_t)kPoolPlanes * kPoolSamples * 64 * sizeof(float);
+    if (need > sampleBuf.bytes) {
+      HIP_OK(hipStreamSynchronize(stream));
+      sampleBuf.alloc(need);
+    }
+    P.sampleBuf = sampleBuf.as<float>();
+    P.sampleBufWaves = (int64_t)waves;
+  }
 
   void ensureSpill(size_t threads) {
     spillRows = spillRowsFor(P.stackLds);
@@ -698,6 +710,7 @@ struct sundog_session {
     }
     size_t gx = (P.width + 15) / 16, gy = (P.height + 15) / 16;
     ensureSpill(gx * gy * renderBlockThreads());
+    ensureSampleBuf(gx * gy * (renderBlockThreads() / 64));
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
     HIP_OK(hipEventRecord(ev0, stream));

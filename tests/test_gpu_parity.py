@@ -79,7 +79,7 @@ def test_closest_hit_bit_exact(scene_name, orc, asset_dir):
         assert np.array_equal(t_bvh.view(np.uint32), t.view(np.uint32))
 
 
-def test_shadow_transmittance_matches_checker(orc, asset_dir):
+def test_shadow_transmittance_matches_oracle(orc, asset_dir):
     sc = configs.volumes(asset_dir, width=32, height=18, spp=1, env_w=64, env_h=32).build()
     sess = sc.session()
     flat = sc.flatten()

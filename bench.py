@@ -69,8 +69,10 @@ def main():
     ap.add_argument("--workload", default="instanced", choices=["instanced", "cornell", "volumes"],
                     help="the headline instanced-icosphere frame (default), or the Cornell-box and "
                          "volumes-and-environment frames")
-    ap.add_argument("--waves", type=int, default=4, choices=[2, 3, 4, 5],
+    ap.add_argument("--waves", type=int, default=5, choices=[2, 3, 4, 5],
                     help="minimum waves per SIMD the megakernel's register budget allows")
+    ap.add_argument("--stack", type=int, default=12,
+                    help="traversal stack entries per lane kept in LDS (deeper entries spill to HBM)")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
                     help="minimum waves per SIMD of the wavefront query kernels")
@@ -118,6 +120,7 @@ def main():
         data = "synthetic (procedural 2048x1024 sky image written as .hdr, fixed layout)"
     sess = scene.session()
     sess.tune("waves", args.waves)
+    sess.tune("stack", args.stack)
     sess.set_kernel(args.kernel)
     sess.tune("wf_waves", args.wf_waves)
     if args.bvh == "lbvh":
@@ -184,6 +187,7 @@ def main():
             "step": "one %d-spp progressive pass over the full frame" % k,
             "kernel": args.kernel,
             "waves_per_simd": args.waves,
+            "stack_lds": args.stack,
             "bvh": args.bvh,
         },
         "mrays_per_s": rays / wall / 1e6,

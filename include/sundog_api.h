@@ -231,9 +231,10 @@ SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int cou
 SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
 /* Launch-shape knobs picked by the autotuner. Keys: "waves" (2..5: the
- * minimum waves per SIMD the megakernel's register budget allows),
- * "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
- * (4..24: LDS traversal-stack entries per lane of the megakernel), and
+ * minimum waves per SIMD the megakernel's register budget allows; default
+ * 5), "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
+ * (4..24: LDS traversal-stack entries per lane of the megakernel; default
+ * 12), and
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Runs the instrumented kernel variant that counts traversal work. */

@@ -19,6 +19,10 @@ namespace sdk {
 constexpr int kBlock = 256;
 constexpr int kStackLdsMax = 24;  // largest runtime LDS stack (megakernel, persistent kernel)
 constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) LDS entries per lane (fixed-size kernels)
+// Default LDS stack of the megakernel: with 5 waves per SIMD, 12 entries
+// (24 KiB per block) leave LDS for all 20 waves of a CU; measured fastest at
+// 1080p on the instanced scene (deeper entries spill to HBM).
+constexpr int kStackLdsMegakernel = 12;
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -2587,6 +2591,7 @@ int persistentBlocksPerCU(int waves, int stackLds) {
 }
 
 int stackLdsEntries() { return kStackLds; }
+int stackLdsDefault() { return kStackLdsMegakernel; }
 int stackLdsMax() { return kStackLdsMax; }
 int renderBlockThreads() { return kBlock; }
 

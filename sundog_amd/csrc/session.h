@@ -41,6 +41,7 @@ hipError_t launchWfShadow(const LaunchParams& P, const WfParams& W, int blocks, 
 hipError_t launchWfClear(WfCounters* c, int what, hipStream_t s);
 int wfBlocksPerCU(int which, int waves);
 int stackLdsEntries();
+int stackLdsDefault();
 int stackLdsMax();
 struct DevBuf;
 int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t nodeBase, uint32_t primBase,
@@ -203,7 +204,7 @@ struct sundog_session {
   LaunchParams P{};
   int kernel = SUNDOG_KERNEL_MEGA;
   bool countWork = false;
-  int waves = 4;    // occupancy variant of the render kernel (2 to 5 waves per SIMD)
+  int waves = 5;    // occupancy variant of the render kernel (2 to 5 waves per SIMD)
   int wfWaves = 4;  // occupancy variant of the wavefront query kernels (4 or 5)
   int numInstances = 0;
   int stackNeed = 0;
@@ -485,7 +486,7 @@ struct sundog_session {
     }
     blasNodes = (int)allNodes.size() - tlasNodes;
     stackNeed = tlasStack + 1 + blasStack;
-    P.stackLds = stackLdsEntries();
+    P.stackLds = stackLdsDefault();
     spillRows = spillRowsFor(P.stackLds);
 
     std::vector<DevInstance> di;

@@ -69,8 +69,9 @@ def main():
     ap.add_argument("--workload", default="instanced", choices=["instanced", "cornell", "volumes"],
                     help="the headline instanced-icosphere frame (default), or the Cornell-box and "
                          "volumes-and-environment frames")
-    ap.add_argument("--waves", type=int, default=5, choices=[2, 3, 4, 5],
-                    help="minimum waves per SIMD the megakernel's register budget allows")
+    ap.add_argument("--waves", type=int, default=0, choices=[0, 2, 3, 4, 5],
+                    help="minimum waves per SIMD the megakernel's register budget allows; 0 (default) "
+                         "lets the session time the variants on real passes before the timed region")
     ap.add_argument("--stack", type=int, default=12,
                     help="traversal stack entries per lane kept in LDS (deeper entries spill to HBM)")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
@@ -146,6 +147,12 @@ def main():
     flops_per_query = flops_instr / queries
     sess.count_work(False)
 
+    # occupancy variant: fixed, or timed on real passes before the warm-up
+    # (sundog_session_autotune; the film is cleared afterwards)
+    waves = args.waves
+    if args.kernel == "megakernel" and waves == 0:
+        waves = sess.autotune(k)
+
     # warmup, then the timed passes
     for i in range(args.warmup):
         sess.render((i * k) % SPP, k)
@@ -164,7 +171,7 @@ def main():
     kernel_ms_avg = st["render_ms"] / args.steps
     flops_per_launch = flops_per_query * rays / args.steps
     achieved_tflops = flops_per_launch / (kernel_ms_avg * 1e-3) / 1e12
-    kname = ("sdk::k_render<false, %d>" % args.waves if args.kernel == "megakernel"
+    kname = ("sdk::k_render<false, %d>" % waves if args.kernel == "megakernel"
              else "wavefront pass (k_wf_shade + k_wf_closest + k_wf_shadow)")
     beauty, _, _ = sess.download()
     finite = bool(np.isfinite(beauty).all())
@@ -186,7 +193,8 @@ def main():
             "workload": workload,
             "step": "one %d-spp progressive pass over the full frame" % k,
             "kernel": args.kernel,
-            "waves_per_simd": args.waves,
+            "waves_per_simd": waves,
+            "waves_chosen_by": "autotune" if args.waves == 0 and args.kernel == "megakernel" else "flag",
             "stack_lds": args.stack,
             "bvh": args.bvh,
         },

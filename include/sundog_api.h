@@ -218,6 +218,8 @@ typedef struct {
   int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH */
   int32_t pad2;
   double bvh_gpu_ms;        /* wall time of the last GPU LBVH rebuild */
+  int32_t waves;            /* megakernel occupancy variant in use (0: auto mode still timing) */
+  int32_t stack_lds;        /* LDS traversal-stack entries per lane */
 } sundog_session_stats;
 
 SUNDOG_API sundog_session* sundog_session_create(sundog_scene*,
@@ -231,12 +233,17 @@ SUNDOG_API int sundog_session_render(sundog_session*, int sample_offset, int cou
 SUNDOG_API int sundog_session_set_kernel(sundog_session*, int kernel);
 SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
 /* Launch-shape knobs picked by the autotuner. Keys: "waves" (2..5: the
- * minimum waves per SIMD the megakernel's register budget allows; default
- * 5), "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
+ * minimum waves per SIMD the megakernel's register budget allows; 0, the
+ * default, picks one per scene by timing the first passes), "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
  * (4..24: LDS traversal-stack entries per lane of the megakernel; default
  * 12), and
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
+/* Times the megakernel's occupancy variants on passes of `count` samples
+ * from sample 0 (one untimed warm-up pass, then 5, 4, 3, 2 waves per SIMD
+ * until one is more than 5% slower than the best), keeps the fastest for
+ * later passes and clears the film. *waves_out receives the choice. */
+SUNDOG_API int sundog_session_autotune(sundog_session*, int count, int* waves_out);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);
 /* Waits for outstanding work. */

@@ -56,7 +56,8 @@ class SessionStats(ctypes.Structure):
                 ("build_ms", c_d), ("device_bytes", c_u64), ("stack_depth", c_i32),
                 ("tlas_nodes", c_i32), ("blas_nodes", c_i32), ("pad", c_i32),
                 ("node_visits", c_u64), ("tri_tests", c_u64), ("quad_tests", c_u64),
-                ("inst_transforms", c_u64), ("bvh_builder", c_i32), ("pad2", c_i32), ("bvh_gpu_ms", c_d)]
+                ("inst_transforms", c_u64), ("bvh_builder", c_i32), ("pad2", c_i32), ("bvh_gpu_ms", c_d),
+                ("waves", c_i32), ("stack_lds", c_i32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -142,6 +143,7 @@ SIGNATURES = {
     "sundog_session_set_count_rays": (c_i, [c_p, c_i]),
     "sundog_session_set_count_work": (c_i, [c_p, c_i]),
     "sundog_session_set_tuning": (c_i, [c_p, c_s, c_i]),
+    "sundog_session_autotune": (c_i, [c_p, c_i, ctypes.POINTER(c_i)]),
     "sundog_session_sync": (c_i, [c_p]),
     "sundog_session_download": (c_i, [c_p, P_f, P_f, P_f]),
     "sundog_session_get_stats": (c_i, [c_p, ctypes.POINTER(SessionStats)]),

@@ -204,7 +204,13 @@ struct sundog_session {
   LaunchParams P{};
   int kernel = SUNDOG_KERNEL_MEGA;
   bool countWork = false;
-  int waves = 5;    // occupancy variant of the render kernel (2 to 5 waves per SIMD)
+  // Occupancy variant of the render kernel: 2 to 5 waves per SIMD, or 0 to
+  // choose per scene from timed passes (see pickWaves).
+  int waves = 0;
+  int autoWaves = 0;                 // the variant auto mode settled on (0: still trying)
+  int autoNext = 0;                  // index of the next candidate to time
+  bool autoWarm = false;             // the untimed warm-up pass has run
+  double autoMsPerPath[6] = {};      // timed cost per path of each variant tried
   int wfWaves = 4;  // occupancy variant of the wavefront query kernels (4 or 5)
   int numInstances = 0;
   int stackNeed = 0;
@@ -251,6 +257,42 @@ struct sundog_session {
   // Spill rows cover the deepest possible stack beyond the smallest LDS
   // stack any kernel of this session uses.
   int spillRowsFor(int ldsEntries) const { return std::max(0, stackNeed - std::min(ldsEntries, stackLdsEntries())); }
+
+  // Auto occupancy: the variants give bit-identical films, so the first
+  // passes of a scene can time them on real work. After one untimed warm-up
+  // pass, candidates are timed in the order 5, 4, 3, 2 waves per SIMD (cost
+  // per path); the trials stop at the first variant more than 5% slower than
+  // the best so far, and later passes use the best. Large frames favour many
+  // waves (latency hiding); small frames, whose grids cannot fill the chip,
+  // favour the spill-free low-occupancy variants.
+  static constexpr int kAutoCandidates[4] = {5, 4, 3, 2};
+  int pickWaves(bool& trial) {
+    trial = false;
+    if (waves != 0) return waves;
+    if (autoWaves != 0) return autoWaves;
+    if (!autoWarm) {
+      autoWarm = true;
+      return kAutoCandidates[0];
+    }
+    trial = true;
+    return kAutoCandidates[autoNext];
+  }
+  void recordTrial(int w, double msPerPath) {
+    autoMsPerPath[w] = msPerPath;
+    int best = kAutoCandidates[0];
+    for (int i = 0; i <= autoNext; i++)
+      if (autoMsPerPath[kAutoCandidates[i]] < autoMsPerPath[best]) best = kAutoCandidates[i];
+    autoNext++;
+    if (autoNext == 4 || msPerPath > 1.05 * autoMsPerPath[best]) autoWaves = best;
+  }
+  void resetAuto() {
+    autoWaves = autoNext = 0;
+    autoWarm = false;
+    for (double& m : autoMsPerPath) m = 0.0;
+  }
+  // The variant renders use now (the fixed one, the auto choice, or 0 while
+  // auto mode is still timing candidates).
+  int currentWaves() const { return waves != 0 ? waves : autoWaves; }
 
   // Per-sample result region of every wave of a render grid (the
   // shared-sample megakernel), grown on demand.
@@ -714,12 +756,15 @@ struct sundog_session {
     ensureSampleBuf(gx * gy * (renderBlockThreads() / 64));
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
+    bool trial = false;
+    const int w = countWork ? 2 : pickWaves(trial);
     HIP_OK(hipEventRecord(ev0, stream));
-    HIP_OK(launchRender(P, countWork, waves, stream));
+    HIP_OK(launchRender(P, countWork, w, stream));
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    if (trial) recordTrial(w, ms / ((double)P.width * P.height * count));
     renderMs += ms;
     paths += (uint64_t)P.width * P.height * count;
   }

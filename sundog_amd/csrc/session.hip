@@ -118,8 +118,10 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
     if (!s || !key) throw std::runtime_error("session: null argument");
     std::string k = key;
     if (k == "waves") {
-      if (value < 2 || value > 5) throw std::runtime_error("tuning: waves must be 2, 3, 4 or 5");
+      if (value != 0 && (value < 2 || value > 5))
+        throw std::runtime_error("tuning: waves must be 0 (auto), 2, 3, 4 or 5");
       s->waves = value;
+      if (value == 0) s->resetAuto();
     } else if (k == "stack") {
       if (value < 4 || value > stackLdsMax()) throw std::runtime_error("tuning: stack must be 4.." + std::to_string(stackLdsMax()));
       HIP_OK(hipStreamSynchronize(s->stream));
@@ -133,6 +135,24 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
     } else {
       throw std::runtime_error("tuning: unknown key '" + k + "'");
     }
+    return SUNDOG_OK;
+  });
+}
+
+extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_out) {
+  return guarded([&] {
+    if (!s) throw std::runtime_error("session: null handle");
+    if (count <= 0) throw std::runtime_error("autotune: count must be positive");
+    if (s->kernel == SUNDOG_KERNEL_WAVEFRONT) throw std::runtime_error("autotune: times the megakernel");
+    const bool counting = s->countWork;
+    s->countWork = false;
+    s->waves = 0;
+    s->resetAuto();
+    // one untimed warm-up pass, then at most four timed candidates
+    for (int i = 0; i < 5 && s->autoWaves == 0; i++) s->renderPass(0, count);
+    s->countWork = counting;
+    s->reset();
+    if (waves_out) *waves_out = s->autoWaves;
     return SUNDOG_OK;
   });
 }
@@ -187,6 +207,8 @@ extern "C" int sundog_session_get_stats(sundog_session* s, sundog_session_stats*
     out->bvh_builder = s->bvhBuilder;
     out->bvh_gpu_ms = s->bvhGpuMs;
     out->blas_nodes = s->blasNodes;
+    out->waves = s->currentWaves();
+    out->stack_lds = s->P.stackLds;
     return SUNDOG_OK;
   });
 }

@@ -506,6 +506,13 @@ class Session(object):
     def tune(self, key, value):
         _lib.check(self.lib.sundog_session_set_tuning(self.ptr, key.encode(), int(value)), self.lib)
 
+    def autotune(self, count=16):
+        """Times the megakernel's occupancy variants on real passes, keeps
+        the fastest, clears the film; returns the chosen waves per SIMD."""
+        w = ctypes.c_int(0)
+        _lib.check(self.lib.sundog_session_autotune(self.ptr, int(count), ctypes.byref(w)), self.lib)
+        return w.value
+
     KERNELS = {"megakernel": 0, "wavefront": 1}
 
     def set_kernel(self, name):

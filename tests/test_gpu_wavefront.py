@@ -50,8 +50,8 @@ def test_wavefront_progressive_passes(asset_dir):
 
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini", "volumes_mini"])
 def test_occupancy_variants_bit_identical(name, asset_dir):
-    """The megakernel's occupancy variants (film running means in registers
-    at 2 waves per SIMD, folded in memory at 3 to 5) give the same bits."""
+    """The megakernel's occupancy variants (2 to 5 waves per SIMD, different
+    register budgets and spills) give the same bits."""
     sc = _scene(name, asset_dir)
     spp = sc.session().spp
     ref = None
@@ -65,3 +65,31 @@ def test_occupancy_variants_bit_identical(name, asset_dir):
         else:
             for a, b in zip(ref, films):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), waves
+
+
+@pytest.mark.parametrize("name", ["cornell", "instanced_mini"])
+def test_autotune_picks_a_variant_and_keeps_the_bits(name, asset_dir):
+    """sundog_session_autotune times the variants on real passes, keeps one
+    and clears the film; renders afterwards (and renders in auto mode, whose
+    first passes are the timed trials) equal a fixed-variant render."""
+    sc = _scene(name, asset_dir)
+    spp = sc.session().spp
+    fixed = sc.session()
+    fixed.tune("waves", 2)
+    fixed.render(0, spp)
+    ref = fixed.download()
+    s = sc.session()
+    w = s.autotune(4)
+    assert w in (2, 3, 4, 5)
+    st = s.stats()
+    assert st["waves"] == w and st["paths"] == 0
+    assert not any(f.any() for f in s.download())      # film cleared
+    s.render(0, spp)
+    for a, b in zip(ref, s.download()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    auto = sc.session()                                   # auto mode, no explicit call
+    for off in range(0, spp, 2):
+        auto.render(off, min(2, spp - off))
+    assert auto.stats()["waves"] in (0, 2, 3, 4, 5)
+    for a, b in zip(ref, auto.download()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -323,12 +323,13 @@ SUNDOG_API sundog_server* sundog_server_create(uint64_t hbm_budget_bytes, int pa
 SUNDOG_API void sundog_server_destroy(sundog_server*);
 /* Before the first submit. Returns the scene id, or -1. */
 SUNDOG_API int sundog_server_add_scene(sundog_server*, sundog_scene*, const sundog_render_options* opt);
-/* Keys: "tile" (8|16|32: pixels per side of a work item), "waves" (2|3|4:
+/* Keys: "tile" (8|16|32: pixels per side of a work item), "waves" (2..5:
  * occupancy variant of the persistent kernel), "stack" (4..24: LDS
  * traversal-stack entries per lane), "blocks" (queue depth: resident
  * persistent blocks per CU, 0 = as many as fit), "graph_passes" (1..64:
  * progressive passes captured into one graph launch), "pass_spp" (> 0),
- * "serial_waves" (2..5: occupancy variant of the per-request baseline). */
+ * "serial_waves" (0: picked per request by timing its first passes, the
+ * default; or 2..5: occupancy variant of the per-request baseline). */
 SUNDOG_API int sundog_server_set_tuning(sundog_server*, const char* key, int value);
 /* Before the first submit: renders a calibration request of scene_id
  * (width x height x spp) under candidate launch shapes by coordinate descent

@@ -1809,44 +1809,6 @@ DEV int pathOnShadow(const LaunchParams& P, PathState& ps, const SurfState& sf, 
   return r;
 }
 
-// Per-pixel driver (the persistent tile-queue kernel): one work-item renders
-// samples [sampleOffset, +sppThisLaunch) of one pixel in order, tracing
-// inline. A lane whose path ends starts its pixel's next sample at once; the
-// pixel's running means stay in registers for the launch.
-DEV void renderPixel(const LaunchParams& P, int px, int py, Stack& st, WorkCount& wc, uint32_t& rays) {
-  const int pixel = py * P.width + px;
-  float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
-  PathState ps;
-  ps.si = 0;
-  ps.medLo = ps.medHi = 0;
-  bool alive = pathBegin(P, ps, px, py);
-  while (alive) {
-    Hit hit;
-    vec3 oObj = v3(0.0f), dObj = v3(0.0f);
-    bool isHit = traceClosest(P, st, ps.o, ps.d, 0.0f, 1e16f, hit, oObj, dObj, wc);
-    rays++;
-    SurfState sf;
-    int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
-    if (r == PN_SHADOW) {
-      vec3 so, wi;
-      float tmaxS;
-      shadowRay(sf, so, wi, tmaxS);
-      rays++;
-      vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
-      r = pathOnShadow(P, ps, sf, tr);
-    }
-    if (r == PN_END) {
-      fAcc = runningMean(fAcc, sanitize(ps.L), ps.s);
-      fAlb = runningMean(fAlb, ps.aovA, ps.s);
-      fNrm = runningMean(fNrm, ps.aovN, ps.s);
-      alive = pathBegin(P, ps, px, py);
-    }
-  }
-  P.accum[pixel] = fAcc;
-  P.albedo[pixel] = fAlb;
-  P.normal[pixel] = fNrm;
-}
-
 // Wave-level sums of the ray / work counters, one 64-bit atomic per wave.
 // Every lane of the wave must reach this call.
 template <bool kCountWork>
@@ -1879,14 +1841,14 @@ DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned 
 // lane folds its own pixel's samples in sample order, the same running-mean
 // operations in the same order as a pixel rendered alone, so films are
 // bit-identical to per-pixel rendering. Launches of more than kPoolSamples
-// samples drain one pool per kPoolSamples.
-DEV void renderWavePooled(const LaunchParams& P, uint32_t* poolNext, float* waveBuf, Stack& st, WorkCount& wc,
-                          uint32_t& rays) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x0 = blockIdx.x * 16 + (wave & 1) * 8, y0 = blockIdx.y * 16 + (wave >> 1) * 8;
-  // slot j of the wave is pixel (x0 + j % 8, y0 + j / 8)
-  const int myPx = x0 + (lane & 7), myPy = y0 + (lane >> 3);
-  const bool mine = myPx < P.width && myPy < P.height;
+// samples drain one pool per kPoolSamples. slotPixel(j, px, py) names the
+// pixel of slot j (0..63) and returns false for a slot outside the image.
+template <class SlotPixel>
+DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* poolNext, float* waveBuf, Stack& st,
+                          WorkCount& wc, uint32_t& rays) {
+  const int lane = threadIdx.x & 63;
+  int myPx = 0, myPy = 0;
+  const bool mine = slotPixel(lane, myPx, myPy);
   for (int c0 = 0; c0 < P.sppThisLaunch; c0 += kPoolSamples) {
     const int n = min(kPoolSamples, P.sppThisLaunch - c0);
     const int items = n * 64;  // item = sample-in-pool * 64 + slot
@@ -1898,9 +1860,8 @@ DEV void renderWavePooled(const LaunchParams& P, uint32_t* poolNext, float* wave
     // claim items until one lies inside the image (or the pool is empty)
     auto claim = [&]() -> bool {
       while (item < items) {
-        const int j = item & 63;
-        const int px = x0 + (j & 7), py = y0 + (j >> 3);
-        if (px < P.width && py < P.height) {
+        int px = 0, py = 0;
+        if (slotPixel(item & 63, px, py)) {
           pathStart(P, ps, px, py, P.sampleOffset + c0 + (item >> 6));
           return true;
         }
@@ -1975,8 +1936,15 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
   st.cap = P.stackLds;
   const int wave = threadIdx.x >> 6;
   const size_t waveGlobal = (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kBlock / 64) + wave;
-  renderWavePooled(P, &s_poolNext[wave], P.sampleBuf + waveGlobal * (kPoolPlanes * kPoolSamples * 64), st, wc,
-                   rays);
+  // 16x16-pixel blocks; each wave covers an 8x8 quarter, slot j at (j % 8, j / 8)
+  const int x0 = blockIdx.x * 16 + (wave & 1) * 8, y0 = blockIdx.y * 16 + (wave >> 1) * 8;
+  auto slotPixel = [&](int j, int& px, int& py) -> bool {
+    px = x0 + (j & 7);
+    py = y0 + (j >> 3);
+    return px < P.width && py < P.height;
+  };
+  renderWavePooled(P, slotPixel, &s_poolNext[wave], P.sampleBuf + waveGlobal * (kPoolPlanes * kPoolSamples * 64), st,
+                   wc, rays);
   flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
 }
 
@@ -2028,10 +1996,12 @@ template <int kWaves>
 __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, const RequestDev* __restrict__ reqs,
                                                                const WorkItem* __restrict__ items, int numItems,
                                                                QueueState* q, int tileSize, uint32_t* spill,
-                                                               int spillStride, int stackLds,
+                                                               int spillStride, int stackLds, float* sampleBuf,
                                                                unsigned long long* rayCounter) {
   extern __shared__ uint64_t s_stack[];  // stackLds * kBlock entries
-  const int lane = threadIdx.x & 63;
+  __shared__ uint32_t s_poolNext[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* waveBuf = sampleBuf + ((size_t)blockIdx.x * (kBlock / 64) + wave) * (kPoolPlanes * kPoolSamples * 64);
   Stack st;
   st.lds = ldsColumn(s_stack);
   st.spill = reinterpret_cast<uint64_t*>(spill) + (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -2063,10 +2033,15 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
     P.cam = R.cam;
     const int tilesX = (P.width + tileSize - 1) / tileSize;
     const int tx = (tile % tilesX) * tileSize, ty = (tile / tilesX) * tileSize;
+    // each 64-pixel group of the tile drains one shared sample pool
     for (int base = 0; base < pxPerTile; base += 64) {
-      int k = base + lane;
-      int px = tx + (k % tileSize), py = ty + (k / tileSize);
-      if (k < pxPerTile && px < P.width && py < P.height) renderPixel(P, px, py, st, wc, rays);
+      auto slotPixel = [&](int j, int& px, int& py) -> bool {
+        const int k = base + j;
+        px = tx + (k % tileSize);
+        py = ty + (k / tileSize);
+        return k < pxPerTile && px < P.width && py < P.height;
+      };
+      renderWavePooled(P, slotPixel, &s_poolNext[wave], waveBuf, st, wc, rays);
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
@@ -2562,17 +2537,23 @@ hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* first
 
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            int stackLds, unsigned long long* rayCounter, hipStream_t s) {
+                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, unsigned long long* rayCounter,
+                            hipStream_t s) {
   const size_t lds = (size_t)stackLds * kBlock * sizeof(uint64_t);
-  if (waves >= 4)
+  // every wave of the grid owns one per-sample result region
+  if (!sampleBuf || sampleBufWaves < (int64_t)blocks * (kBlock / 64)) return hipErrorInvalidValue;
+  if (waves >= 5)
+    hipLaunchKernelGGL((k_persistent<5>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
+                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
+  else if (waves == 4)
     hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
   else if (waves == 3)
     hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
   else
     hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
   return hipGetLastError();
 }
 
@@ -2581,7 +2562,9 @@ int persistentBlocksPerCU(int waves, int stackLds) {
   int n = 0;
   hipError_t e;
   const size_t lds = (size_t)stackLds * kBlock * sizeof(uint64_t);
-  if (waves >= 4)
+  if (waves >= 5)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<5>, kBlock, lds);
+  else if (waves == 4)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<4>, kBlock, lds);
   else if (waves == 3)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<3>, kBlock, lds);

@@ -70,12 +70,13 @@ struct sundog_server {
   int stackLds = 16;     // LDS traversal-stack entries per lane
   int blocksReq = 0;     // queue depth: resident persistent blocks per CU (0 = as many as fit)
   int graphPasses = 4;   // progressive passes captured into one graph launch (1 = no graph)
-  int serialWaves = 2;   // occupancy variant of the per-request megakernel baseline
+  int serialWaves = 0;   // occupancy variant of the per-request megakernel baseline (0: picked per request)
   int blocksPerCU = 1;
   int numCUs = 1;
   int maxStackNeed = 0;  // deepest traversal stack over the resident scenes
   hipStream_t stream = nullptr;
-  DevBuf dReqs, dItems, dQueue, dFirst, spill, rayCounter;
+  DevBuf dReqs, dItems, dQueue, dFirst, spill, sampleBuf, rayCounter;
+  int64_t sampleBufWaves = 0;  // per-sample result regions (one per wave of the persistent grid)
   size_t reqCap = 0, itemCap = 0, queueCap = 0;
   Clock::time_point t0;
   std::mutex mu;
@@ -151,12 +152,21 @@ struct sundog_server {
     applyTuning();
   }
 
+  // Knobs that reallocate the launch buffers (spill, per-sample results)
+  // may only change while no request is in flight; the caller holds mu.
+  void requireIdle(const char* what) const {
+    if (!active.empty() || !waiting.empty())
+      throw std::runtime_error(std::string("tuning: ") + what + " cannot change while requests are in flight");
+  }
+
   void applyTuning() {
     const int fit = persistentBlocksPerCU(waves, stackLds);
     blocksPerCU = blocksReq > 0 ? std::min(blocksReq, fit) : fit;
     size_t threads = (size_t)numCUs * blocksPerCU * renderBlockThreads();
     int rows = std::max(0, maxStackNeed - stackLds);
     spill.alloc(std::max<size_t>(16, (size_t)rows * threads * kStackEntryBytes));
+    sampleBufWaves = (int64_t)(threads / 64);
+    sampleBuf.alloc((size_t)sampleBufWaves * kPoolPlanes * kPoolSamples * 64 * sizeof(float));
   }
 
   RequestDev requestRecord(const ServerRequest& r, int sceneSlot) const {
@@ -253,7 +263,8 @@ struct sundog_server {
           HIP_OK(launchQueuePrep(q, rq, dFirst.as<int32_t>() + g.reqBegin, (int)(g.reqEnd - g.reqBegin), stream));
           HIP_OK(launchPersistent(g.table, rq, dItems.as<WorkItem>() + g.itemBegin, (int)(g.itemEnd - g.itemBegin),
                                   q, tileSize, waves, blocks, spill.as<uint32_t>(), (int)threads, stackLds,
-                                  rayCounter.as<unsigned long long>(), stream));
+                                  sampleBuf.as<float>(), sampleBufWaves, rayCounter.as<unsigned long long>(),
+                                  stream));
         }
       }
     };
@@ -439,22 +450,26 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
       if (value != 8 && value != 16 && value != 32) throw std::runtime_error("tuning: tile must be 8, 16 or 32");
       s->tileSize = value;
     } else if (k == "waves") {
-      if (value < 2 || value > 4) throw std::runtime_error("tuning: waves must be 2, 3 or 4");
+      if (value < 2 || value > 5) throw std::runtime_error("tuning: waves must be 2, 3, 4 or 5");
+      s->requireIdle("waves");
       s->waves = value;
       if (s->stream) s->applyTuning();
     } else if (k == "stack") {
       if (value < 4 || value > stackLdsMax()) throw std::runtime_error("tuning: stack must be 4.." + std::to_string(stackLdsMax()));
+      s->requireIdle("stack");
       s->stackLds = value;
       if (s->stream) s->applyTuning();
     } else if (k == "blocks") {
       if (value < 0) throw std::runtime_error("tuning: blocks must be >= 0 (0 = as many as fit)");
+      s->requireIdle("blocks");
       s->blocksReq = value;
       if (s->stream) s->applyTuning();
     } else if (k == "graph_passes") {
       if (value < 1 || value > 64) throw std::runtime_error("tuning: graph_passes must be 1..64");
       s->graphPasses = value;
     } else if (k == "serial_waves") {
-      if (value < 2 || value > 5) throw std::runtime_error("tuning: serial_waves must be 2..5");
+      if (value != 0 && (value < 2 || value > 5))
+        throw std::runtime_error("tuning: serial_waves must be 0 (auto) or 2..5");
       s->serialWaves = value;
     } else if (k == "pass_spp") {
       if (value <= 0) throw std::runtime_error("tuning: pass_spp must be positive");
@@ -646,12 +661,19 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     P.sampleBuf = sb.as<float>();
     P.sampleBufWaves = (int64_t)waves;
     HIP_OK(hipStreamSynchronize(s->stream));
+    OccupancyPicker pick;  // serialWaves 0: the per-request choice a session makes
     auto t = Clock::now();
     for (int done = 0; done < spp; done += 16) {
       P.sampleOffset = done;
       P.sppThisLaunch = std::min(16, spp - done);
-      HIP_OK(launchRender(P, false, s->serialWaves, s->stream));
+      bool trial = false;
+      const int w = s->serialWaves != 0 ? s->serialWaves : pick.pick(trial);
+      auto t0 = Clock::now();
+      HIP_OK(launchRender(P, false, w, s->stream));
       HIP_OK(hipStreamSynchronize(s->stream));
+      if (trial)
+        pick.record(w, std::chrono::duration<double, std::milli>(Clock::now() - t0).count() /
+                           ((double)width * height * P.sppThisLaunch));
     }
     double ms = std::chrono::duration<double, std::milli>(Clock::now() - t).count();
     if (ms_out) *ms_out = ms;
@@ -719,7 +741,7 @@ extern "C" int sundog_server_autotune(sundog_server* s, int scene_id, int width,
       }
       knob = keep;
     };
-    sweep(s->waves, {2, 3, 4});
+    sweep(s->waves, {2, 3, 4, 5});
     sweep(s->stackLds, {8, 12, 16, 24});
     sweep(s->tileSize, {8, 16, 32});
     const int fit = persistentBlocksPerCU(s->waves, s->stackLds);

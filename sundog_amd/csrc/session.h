@@ -33,7 +33,8 @@ hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* first
                            hipStream_t s);
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            int stackLds, unsigned long long* rayCounter, hipStream_t s);
+                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, unsigned long long* rayCounter,
+                            hipStream_t s);
 int persistentBlocksPerCU(int waves, int stackLds);
 hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
 hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s);
@@ -52,6 +53,41 @@ hipError_t launchEmitTris(const uint64_t* sortedKeys, int n, const float* pos, c
 int renderBlockThreads();
 constexpr int kWfBlock = 256;  // threads per block of the wavefront kernels
 constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference + entry distance
+
+// Auto occupancy for the megakernel: its variants (2 to 5 waves per SIMD)
+// give bit-identical films, so the first passes of a render can time them
+// on real work. After one untimed warm-up pass, candidates are timed in the
+// order 5, 4, 3, 2 (cost per path); the trials stop at the first variant
+// more than 5% slower than the best so far, and later passes use the best.
+// Large frames favour many waves (latency hiding); small frames, whose grids
+// cannot fill the chip, favour the spill-free low-occupancy variants.
+struct OccupancyPicker {
+  static constexpr int kCandidates[4] = {5, 4, 3, 2};
+  int chosen = 0;           // the variant settled on (0: still timing)
+  int next = 0;             // index of the next candidate to time
+  bool warm = false;        // the untimed warm-up pass has run
+  double msPerPath[6] = {};
+  // The variant for the next pass; trial is set when that pass is timed.
+  int pick(bool& trial) {
+    trial = false;
+    if (chosen != 0) return chosen;
+    if (!warm) {
+      warm = true;
+      return kCandidates[0];
+    }
+    trial = true;
+    return kCandidates[next];
+  }
+  void record(int w, double ms) {
+    msPerPath[w] = ms;
+    int best = kCandidates[0];
+    for (int i = 0; i <= next; i++)
+      if (msPerPath[kCandidates[i]] < msPerPath[best]) best = kCandidates[i];
+    next++;
+    if (next == 4 || ms > 1.05 * msPerPath[best]) chosen = best;
+  }
+  void reset() { *this = OccupancyPicker{}; }
+};
 
 #define HIP_OK(expr)                                                                          \
   do {                                                                                        \
@@ -205,12 +241,9 @@ struct sundog_session {
   int kernel = SUNDOG_KERNEL_MEGA;
   bool countWork = false;
   // Occupancy variant of the render kernel: 2 to 5 waves per SIMD, or 0 to
-  // choose per scene from timed passes (see pickWaves).
+  // choose per scene from timed passes (autoPick).
   int waves = 0;
-  int autoWaves = 0;                 // the variant auto mode settled on (0: still trying)
-  int autoNext = 0;                  // index of the next candidate to time
-  bool autoWarm = false;             // the untimed warm-up pass has run
-  double autoMsPerPath[6] = {};      // timed cost per path of each variant tried
+  OccupancyPicker autoPick;
   int wfWaves = 4;  // occupancy variant of the wavefront query kernels (4 or 5)
   int numInstances = 0;
   int stackNeed = 0;
@@ -258,41 +291,15 @@ struct sundog_session {
   // stack any kernel of this session uses.
   int spillRowsFor(int ldsEntries) const { return std::max(0, stackNeed - std::min(ldsEntries, stackLdsEntries())); }
 
-  // Auto occupancy: the variants give bit-identical films, so the first
-  // passes of a scene can time them on real work. After one untimed warm-up
-  // pass, candidates are timed in the order 5, 4, 3, 2 waves per SIMD (cost
-  // per path); the trials stop at the first variant more than 5% slower than
-  // the best so far, and later passes use the best. Large frames favour many
-  // waves (latency hiding); small frames, whose grids cannot fill the chip,
-  // favour the spill-free low-occupancy variants.
-  static constexpr int kAutoCandidates[4] = {5, 4, 3, 2};
   int pickWaves(bool& trial) {
     trial = false;
-    if (waves != 0) return waves;
-    if (autoWaves != 0) return autoWaves;
-    if (!autoWarm) {
-      autoWarm = true;
-      return kAutoCandidates[0];
-    }
-    trial = true;
-    return kAutoCandidates[autoNext];
+    return waves != 0 ? waves : autoPick.pick(trial);
   }
-  void recordTrial(int w, double msPerPath) {
-    autoMsPerPath[w] = msPerPath;
-    int best = kAutoCandidates[0];
-    for (int i = 0; i <= autoNext; i++)
-      if (autoMsPerPath[kAutoCandidates[i]] < autoMsPerPath[best]) best = kAutoCandidates[i];
-    autoNext++;
-    if (autoNext == 4 || msPerPath > 1.05 * autoMsPerPath[best]) autoWaves = best;
-  }
-  void resetAuto() {
-    autoWaves = autoNext = 0;
-    autoWarm = false;
-    for (double& m : autoMsPerPath) m = 0.0;
-  }
+  void recordTrial(int w, double msPerPath) { autoPick.record(w, msPerPath); }
+  void resetAuto() { autoPick.reset(); }
   // The variant renders use now (the fixed one, the auto choice, or 0 while
   // auto mode is still timing candidates).
-  int currentWaves() const { return waves != 0 ? waves : autoWaves; }
+  int currentWaves() const { return waves != 0 ? waves : autoPick.chosen; }
 
   // Per-sample result region of every wave of a render grid (the
   // shared-sample megakernel), grown on demand.

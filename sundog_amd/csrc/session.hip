@@ -149,10 +149,10 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     s->waves = 0;
     s->resetAuto();
     // one untimed warm-up pass, then at most four timed candidates
-    for (int i = 0; i < 5 && s->autoWaves == 0; i++) s->renderPass(0, count);
+    for (int i = 0; i < 5 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
     s->countWork = counting;
     s->reset();
-    if (waves_out) *waves_out = s->autoWaves;
+    if (waves_out) *waves_out = s->autoPick.chosen;
     return SUNDOG_OK;
   });
 }

@@ -168,9 +168,12 @@ def main(argv=None):
     ap.add_argument("--stack", type=int, default=16)
     ap.add_argument("--graph-passes", type=int, default=4)
     ap.add_argument("--autotune", action="store_true",
-                    help="pick the launch shape with the autotuner on a 1024x576, 16-spp request of scene 1")
-    ap.add_argument("--serial-waves", type=int, default=2,
-                    help="occupancy variant of the per-request megakernel baseline")
+                    help="pick the launch shape with the autotuner on a calibration request of scene 1")
+    ap.add_argument("--autotune-size", default="1920x1080x32",
+                    help="calibration request WIDTHxHEIGHTxSPP (large enough that tails do not dominate)")
+    ap.add_argument("--serial-waves", type=int, default=0,
+                    help="occupancy variant of the per-request megakernel baseline (0: picked per "
+                         "request by timing its first passes, as a session does)")
     ap.add_argument("--pass-spp", type=int, default=16)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--spp-max", type=int, default=1024)
@@ -187,7 +190,8 @@ def main(argv=None):
     srv.tune("stack", args.stack)
     srv.tune("graph_passes", args.graph_passes)
     srv.tune("serial_waves", args.serial_waves)
-    tuned = srv.autotune(sids[min(1, len(sids) - 1)], 1024, 576, 16) if args.autotune else None
+    cw, ch, cs = (int(v) for v in args.autotune_size.split("x"))
+    tuned = srv.autotune(sids[min(1, len(sids) - 1)], cw, ch, cs) if args.autotune else None
     reqs = make_requests(args.requests, args.qps, seed=args.seed, spp_hi=args.spp_max)
     total_paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
     print("serving: %d requests, %.3g paths, qps %.2f" % (len(reqs), total_paths, args.qps), flush=True)

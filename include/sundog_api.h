@@ -240,9 +240,9 @@ SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Times the megakernel's occupancy variants on passes of `count` samples
- * from sample 0 (one untimed warm-up pass, then 5, 4, 3, 2 waves per SIMD
- * until one is more than 5% slower than the best), keeps the fastest for
- * later passes and clears the film. *waves_out receives the choice. */
+ * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per
+ * SIMD twice each, the faster pass counting), keeps the fastest for later
+ * passes and clears the film. *waves_out receives the choice. */
 SUNDOG_API int sundog_session_autotune(sundog_session*, int count, int* waves_out);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);

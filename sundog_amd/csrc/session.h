@@ -13,6 +13,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sundog_api.h"
@@ -63,8 +64,11 @@ constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference
 // cannot fill the chip, favour the spill-free low-occupancy variants.
 struct OccupancyPicker {
   static constexpr int kCandidates[4] = {5, 4, 3, 2};
+  int rounds = 1;           // timed passes per candidate (the fastest counts)
+  bool exhaustive = false;  // time every candidate instead of stopping early
   int chosen = 0;           // the variant settled on (0: still timing)
-  int next = 0;             // index of the next candidate to time
+  int next = 0;             // index of the candidate being timed
+  int round = 0;            // timed passes of that candidate so far
   bool warm = false;        // the untimed warm-up pass has run
   double msPerPath[6] = {};
   // The variant for the next pass; trial is set when that pass is timed.
@@ -79,14 +83,22 @@ struct OccupancyPicker {
     return kCandidates[next];
   }
   void record(int w, double ms) {
-    msPerPath[w] = ms;
+    msPerPath[w] = round == 0 ? ms : std::min(msPerPath[w], ms);
+    if (++round < rounds) return;
+    round = 0;
     int best = kCandidates[0];
     for (int i = 0; i <= next; i++)
       if (msPerPath[kCandidates[i]] < msPerPath[best]) best = kCandidates[i];
     next++;
-    if (next == 4 || ms > 1.05 * msPerPath[best]) chosen = best;
+    if (next == 4 || (!exhaustive && msPerPath[w] > 1.05 * msPerPath[best])) chosen = best;
   }
-  void reset() { *this = OccupancyPicker{}; }
+  void reset() {
+    const int r = rounds;
+    const bool ex = exhaustive;
+    *this = OccupancyPicker{};
+    rounds = r;
+    exhaustive = ex;
+  }
 };
 
 #define HIP_OK(expr)                                                                          \
@@ -197,19 +209,63 @@ inline Aabb quadricBox(int kind) {
   return b;
 }
 
+// Widen a transformed box by a few ulps of its magnitude so that rounding
+// (of the forward transform here, of the float inverse on the ray side)
+// never shrinks it below the surface it bounds.
+inline Aabb padTransformed(Aabb r) {
+  vec3 mag = vmax(vec3{fabsf(r.lo.x), fabsf(r.lo.y), fabsf(r.lo.z)}, vec3{fabsf(r.hi.x), fabsf(r.hi.y), fabsf(r.hi.z)});
+  vec3 pad = mag * 2e-6f + v3(1e-7f);
+  r.lo = r.lo - pad;
+  r.hi = r.hi + pad;
+  return r;
+}
+
 inline Aabb transformBox(const Affine& m, const Aabb& b) {
   Aabb r;
   for (int i = 0; i < 8; i++) {
     vec3 c = vec3{(i & 1) ? b.hi.x : b.lo.x, (i & 2) ? b.hi.y : b.lo.y, (i & 4) ? b.hi.z : b.lo.z};
     r.grow(m.point(c));
   }
-  // widen by a few ulps of the magnitude so rounding in the transform never
-  // shrinks the box below the surface it bounds
-  vec3 mag = vmax(vec3{fabsf(r.lo.x), fabsf(r.lo.y), fabsf(r.lo.z)}, vec3{fabsf(r.hi.x), fabsf(r.hi.y), fabsf(r.hi.z)});
-  vec3 pad = mag * 2e-6f + v3(1e-7f);
-  r.lo = r.lo - pad;
-  r.hi = r.hi + pad;
-  return r;
+  return padTransformed(r);
+}
+
+// World box of every object. A mesh instance gets the bounds of its
+// transformed vertices (those of its triangles' convex hull): transformed
+// object-box corners grow with rotation and non-uniform scale (38% more
+// surface area on average on the instanced-icosphere scene), and every
+// extra unit of area is extra rays entering the instance for nothing.
+// Quadrics keep their transformed boxes. Split over host threads.
+inline std::vector<Aabb> objectWorldBoxes(const Scene& S, const std::vector<HostMesh>& meshes,
+                                          const std::vector<Aabb>& meshBounds) {
+  std::vector<Aabb> out(S.objects.size());
+  auto work = [&](size_t begin, size_t end) {
+    for (size_t i = begin; i < end; i++) {
+      const Object& o = S.objects[i];
+      if (o.geomKind != GK_MESH) {
+        out[i] = transformBox(o.xform, quadricBox(o.geomKind));
+        continue;
+      }
+      const std::vector<vec3>& pts = meshes[o.meshId].positions;
+      if (pts.empty()) {
+        out[i] = transformBox(o.xform, meshBounds[o.meshId]);
+        continue;
+      }
+      Aabb r;
+      for (const vec3& p : pts) r.grow(o.xform.point(p));
+      out[i] = padTransformed(r);
+    }
+  };
+  const size_t n = out.size();
+  const size_t threads = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  if (n < 64 || threads == 1) {
+    work(0, n);
+    return out;
+  }
+  std::vector<std::thread> pool;
+  const size_t chunk = (n + threads - 1) / threads;
+  for (size_t b = 0; b < n; b += chunk) pool.emplace_back(work, b, std::min(n, b + chunk));
+  for (std::thread& t : pool) t.join();
+  return out;
 }
 
 inline void relocate(std::vector<BvhNode4>& nodes, size_t from, size_t count, uint32_t nodeBase,
@@ -491,12 +547,7 @@ struct sundog_session {
     }
 
     // top level over all objects
-    std::vector<Aabb> instBoxes(S.objects.size());
-    for (size_t i = 0; i < S.objects.size(); i++) {
-      const Object& o = S.objects[i];
-      Aabb ob = o.geomKind == GK_MESH ? meshBounds[o.meshId] : quadricBox(o.geomKind);
-      instBoxes[i] = transformBox(o.xform, ob);
-    }
+    std::vector<Aabb> instBoxes = objectWorldBoxes(S, prep.meshes, meshBounds);
     int tlasStack = 0;
     if (!instBoxes.empty()) {
       Bvh4 tlas = buildBvh4(instBoxes, 1);
@@ -679,9 +730,7 @@ struct sundog_session {
     auto t0 = std::chrono::steady_clock::now();
     const Scene& S = prep.scene;
     std::vector<float4> ib;
-    for (const Object& o : S.objects) {
-      Aabb ob = o.geomKind == GK_MESH ? meshBox[o.meshId] : quadricBox(o.geomKind);
-      Aabb w = transformBox(o.xform, ob);
+    for (const Aabb& w : objectWorldBoxes(S, prep.meshes, meshBox)) {
       ib.push_back(make_float4(w.lo.x, w.lo.y, w.lo.z, 0.0f));
       ib.push_back(make_float4(w.hi.x, w.hi.y, w.hi.z, 0.0f));
     }

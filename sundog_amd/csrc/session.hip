@@ -147,9 +147,13 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     const bool counting = s->countWork;
     s->countWork = false;
     s->waves = 0;
-    s->resetAuto();
-    // one untimed warm-up pass, then at most four timed candidates
-    for (int i = 0; i < 5 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
+    // every candidate, twice (the faster pass counts): one untimed warm-up
+    // pass and eight timed ones; later passes of this session then use the
+    // winner
+    s->autoPick = OccupancyPicker{};
+    s->autoPick.rounds = 2;
+    s->autoPick.exhaustive = true;
+    for (int i = 0; i < 9 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
     s->countWork = counting;
     s->reset();
     if (waves_out) *waves_out = s->autoPick.chosen;

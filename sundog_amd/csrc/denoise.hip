@@ -63,24 +63,70 @@ __device__ inline bf16x8 asBf(u16x8 v) { return __builtin_bit_cast(bf16x8, v); }
 constexpr int kTile = 32;  // MFMA 32x32x16
 
 // ---------------------------------------------------------------------------
-// forward / backward-data convolution (implicit GEMM, one wave = 32x32 tile)
+// forward / backward-data convolution (implicit GEMM)
 // ---------------------------------------------------------------------------
 // in: [P][Cin] (BF16, or FP32 when kInF32), w: [Cout][9][Cin] BF16.
 // out[p][co] = bias[co] + sum_t sum_ci in[p + tapOff(t)][ci] * w[co][t][ci];
 // tapSign = +1 for forward, -1 for backward-data (the transposed conv reads
 // the neighbour on the opposite side). Cin % 16 == 0, Cout % (32 * kNT) == 0.
-// Each wave owns 32 pixels x (32 * kNT) output channels: one activation
-// fragment per K step feeds kNT MFMAs.
+//
+// A block is 4 waves x 32 pixels (M = 128) by 32 * kNT output channels (N).
+// K = 9 * Cin runs tap-major (k = tap * Cin + ci, the weights' own row
+// order) in chunks of ConvChunk<kNT>::kK elements: each chunk's weight slab
+// (N x chunk, the B operand of all four waves) is staged once per block in
+// LDS, double-buffered: the next chunk is prefetched into registers while
+// the current one feeds the MFMAs, then stored to the other buffer before
+// the chunk's single barrier. Chunks are longer for fewer output tiles (the
+// shallow full-resolution layers stage their whole weight matrix once and
+// run without a barrier). Each wave's activation fragment (its 32 pixels x
+// 16 channels of one tap, the A operand) is loaded from global memory once
+// per K step and reused by its kNT MFMAs. Vector v of a slab row r is stored
+// at v ^ (r & 7) (rows hold a multiple of 8 16-byte vectors) so the 32 rows
+// a half-wave reads land in 8 distinct 16-byte bank groups.
+template <int kNT>
+struct ConvChunk {
+  static constexpr int kK = kNT == 1 ? 320 : (kNT == 2 ? 128 : 64);  // K elements per chunk
+  static constexpr int kVecs = kK / 8;                                 // 16-byte vectors per slab row
+  static constexpr int kRegs = kNT * 32 * kVecs / 256;                 // slab vectors per thread
+};
+
+template <int kNT>
+__device__ __forceinline__ void convLoadChunk(const uint16_t* __restrict__ w, int co0, int k0, int kn, int Ktot,
+                                              u16x8 (&reg)[ConvChunk<kNT>::kRegs]) {
+  constexpr int kVecs = ConvChunk<kNT>::kVecs;
+#pragma unroll
+  for (int k = 0; k < ConvChunk<kNT>::kRegs; k++) {
+    const int idx = (int)threadIdx.x + k * 256;
+    const int r = idx / kVecs, v = idx - r * kVecs;
+    reg[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (v * 8 < kn) reg[k] = *(const u16x8*)(w + (size_t)(co0 + r) * Ktot + k0 + v * 8);
+  }
+}
+
+template <int kNT>
+__device__ __forceinline__ void convStoreChunk(u16x8* sB, const u16x8 (&reg)[ConvChunk<kNT>::kRegs]) {
+  constexpr int kVecs = ConvChunk<kNT>::kVecs;
+#pragma unroll
+  for (int k = 0; k < ConvChunk<kNT>::kRegs; k++) {
+    const int idx = (int)threadIdx.x + k * 256;
+    const int r = idx / kVecs, v = idx - r * kVecs;
+    sB[r * kVecs + (v ^ (r & 7))] = reg[k];
+  }
+}
+
 template <bool kInF32, bool kOutF32, int kNT>
 __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, const uint16_t* __restrict__ w,
                                               const float* __restrict__ bias, void* __restrict__ outV, int N,
                                               int H, int W, int Cin, int Cout, int tapSign, int relu) {
+  constexpr int kK = ConvChunk<kNT>::kK, kVecs = ConvChunk<kNT>::kVecs;
+  __shared__ u16x8 sB[2][kNT * 32 * kVecs];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int P = N * H * W;
   const int m0 = (blockIdx.x * 4 + wave) * kTile;
   const int co0 = blockIdx.y * kTile * kNT;
-  if (m0 >= P) return;
+  // a wave past the last pixel still stages weights and meets every barrier
+  const bool waveActive = m0 < P;
   const int row = lane & 31, half = lane >> 5;
   const int p = m0 + row;
   const bool pv = p < P;
@@ -90,31 +136,51 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
   f32x16 acc[kNT];
 #pragma unroll
   for (int j = 0; j < kNT; j++) acc[j] = f32x16{};
-  const size_t wStride = (size_t)kTile * 9 * Cin;  // between output-channel tiles
-  const uint16_t* wrow = w + (size_t)(co0 + row) * 9 * Cin;
-  for (int t = 0; t < 9; t++) {
-    const int sy = y + tapSign * (t / 3 - 1), sx = x + tapSign * (t % 3 - 1);
-    const bool valid = pv && sy >= 0 && sy < H && sx >= 0 && sx < W;
-    const size_t src = ((size_t)n * H + (valid ? sy : 0)) * W + (valid ? sx : 0);
-    for (int c = 0; c < Cin; c += 16) {
-      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (valid) {
-        if (kInF32) {
-          const float* ip = (const float*)inV + src * Cin + c + 8 * half;
-          float4 f0 = *(const float4*)ip, f1 = *(const float4*)(ip + 4);
-          a = u16x8{f2bf(f0.x), f2bf(f0.y), f2bf(f0.z), f2bf(f0.w), f2bf(f1.x), f2bf(f1.y), f2bf(f1.z), f2bf(f1.w)};
-        } else {
-          a = *(const u16x8*)((const uint16_t*)inV + src * Cin + c + 8 * half);
+  const int Ktot = 9 * Cin;
+  const int numChunks = (Ktot + kK - 1) / kK;
+  u16x8 reg[ConvChunk<kNT>::kRegs];
+  convLoadChunk<kNT>(w, co0, 0, min(kK, Ktot), Ktot, reg);
+  convStoreChunk<kNT>(sB[0], reg);
+  __syncthreads();
+  int tap = 0, ci = 0;  // (tap, channel) of the next K step
+  for (int q = 0; q < numChunks; q++) {
+    const int k0 = q * kK, kn = min(kK, Ktot - k0);
+    if (q + 1 < numChunks) convLoadChunk<kNT>(w, co0, k0 + kK, min(kK, Ktot - k0 - kK), Ktot, reg);
+    if (waveActive) {
+      const u16x8* slab = sB[q & 1];
+      for (int kk = 0; kk < kn; kk += 16) {
+        const int sy = y + tapSign * (tap / 3 - 1), sx = x + tapSign * (tap % 3 - 1);
+        const bool valid = pv && sy >= 0 && sy < H && sx >= 0 && sx < W;
+        const size_t src = ((size_t)n * H + sy) * W + sx;
+        u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (valid) {
+          if (kInF32) {
+            const float* ip = (const float*)inV + src * Cin + ci + 8 * half;
+            float4 f0 = *(const float4*)ip, f1 = *(const float4*)(ip + 4);
+            a = u16x8{f2bf(f0.x), f2bf(f0.y), f2bf(f0.z), f2bf(f0.w),
+                      f2bf(f1.x), f2bf(f1.y), f2bf(f1.z), f2bf(f1.w)};
+          } else {
+            a = *(const u16x8*)((const uint16_t*)inV + src * Cin + ci + 8 * half);
+          }
+        }
+        const int v = (kk >> 3) + half;
+#pragma unroll
+        for (int j = 0; j < kNT; j++) {
+          const int r = j * 32 + row;
+          u16x8 b = slab[r * kVecs + (v ^ (r & 7))];
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[j], 0, 0, 0);
+        }
+        ci += 16;
+        if (ci == Cin) {
+          ci = 0;
+          tap++;
         }
       }
-      const uint16_t* wk = wrow + (size_t)t * Cin + c + 8 * half;
-#pragma unroll
-      for (int j = 0; j < kNT; j++) {
-        u16x8 b = *(const u16x8*)(wk + j * wStride);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[j], 0, 0, 0);
-      }
     }
+    if (q + 1 < numChunks) convStoreChunk<kNT>(sB[(q + 1) & 1], reg);
+    __syncthreads();
   }
+  if (!waveActive) return;
 #pragma unroll
   for (int j = 0; j < kNT; j++) {
     const int co = co0 + j * kTile + row;  // D column = lane % 32
@@ -258,17 +324,19 @@ __global__ void k_pool_bwd(const float* __restrict__ gout, float* __restrict__ g
 }
 
 // out = concat(up2(a) [Ca channels], skip [Cb channels]) at the skip's size
+// One thread per 8 channels (16 bytes) of the output; Ca % 8 == Cb % 8 == 0
+// and N * H * W < 2^31.
 __global__ void k_upcat_fwd(const uint16_t* __restrict__ a, const uint16_t* __restrict__ skip,
                             uint16_t* __restrict__ out, int N, int H, int W, int Ca, int Cb) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int C = Ca + Cb;
-  size_t total = (size_t)N * H * W * C;
-  if (i >= total) return;
-  int c = i % C;
-  size_t p = i / C;
-  int x = p % W, y = (p / W) % H, n = p / ((size_t)W * H);
-  if (c < Ca) out[i] = a[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c];
-  else out[i] = skip[p * Cb + (c - Ca)];
+  const int Ca8 = Ca / 8, Cb8 = Cb / 8, C8 = Ca8 + Cb8;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * H * W * C8) return;
+  const int p = (int)(i / C8), c8 = (int)(i - (size_t)p * C8);
+  const int x = p % W, yn = p / W, y = yn % H, n = yn / H;
+  u16x8 v;
+  if (c8 < Ca8) v = ((const u16x8*)a)[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca8 + c8];
+  else v = ((const u16x8*)skip)[(size_t)p * Cb8 + (c8 - Ca8)];
+  ((u16x8*)out)[i] = v;
 }
 
 // gA[y/2][x/2] += sum over the 4 children (gather form, no atomics);
@@ -609,7 +677,7 @@ struct sundog_denoiser {
     for (int l = 3; l >= 0; l--) {
       int h = H >> l, w = W >> l;
       int skipConv = 2 * l + 1;
-      size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]);
+      size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]) / 8;  // 16-byte vectors
       hipLaunchKernelGGL(k_upcat_fwd, dim3(blocks(tot)), dim3(256), 0, stream, cur, y[skipConv].as<uint16_t>(),
                          cat[l].as<uint16_t>(), N, h, w, kCh[l + 1], kCh[l]);
       conv(k, cat[l].p, false, y[k].p, false, h, w);

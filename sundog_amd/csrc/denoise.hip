@@ -76,16 +76,16 @@ constexpr int kTile = 32;  // MFMA 32x32x16
 // (N x chunk, the B operand of all four waves) is staged once per block in
 // LDS, double-buffered: the next chunk is prefetched into registers while
 // the current one feeds the MFMAs, then stored to the other buffer before
-// the chunk's single barrier. Chunks are longer for fewer output tiles (the
-// shallow full-resolution layers stage their whole weight matrix once and
-// run without a barrier). Each wave's activation fragment (its 32 pixels x
+// the chunk's single barrier. Chunks are longer for fewer output tiles (128
+// K elements for one or two tiles, 64 beyond, so the slab stays at 16-40 KB
+// per block). Each wave's activation fragment (its 32 pixels x
 // 16 channels of one tap, the A operand) is loaded from global memory once
 // per K step and reused by its kNT MFMAs. Vector v of a slab row r is stored
 // at v ^ (r & 7) (rows hold a multiple of 8 16-byte vectors) so the 32 rows
 // a half-wave reads land in 8 distinct 16-byte bank groups.
 template <int kNT>
 struct ConvChunk {
-  static constexpr int kK = kNT == 1 ? 320 : (kNT == 2 ? 128 : 64);  // K elements per chunk
+  static constexpr int kK = kNT <= 2 ? 128 : 64;  // K elements per chunk
   static constexpr int kVecs = kK / 8;                                 // 16-byte vectors per slab row
   static constexpr int kRegs = kNT * 32 * kVecs / 256;                 // slab vectors per thread
 };

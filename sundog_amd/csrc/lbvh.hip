@@ -344,6 +344,66 @@ __global__ void k_single_leaf(const float4* __restrict__ boxes, const uint64_t* 
 
 inline unsigned grid(size_t n) { return (unsigned)((n + kLbvhBlock - 1) / kLbvhBlock); }
 
+// World box of each mesh instance from its transformed vertices: one block
+// per job, a min/max reduction over the mesh's vertices, padded by a few
+// ulps of the magnitude as the host pads transformed boxes. Jobs with no
+// vertices (quadric instances) keep the box the host wrote.
+__global__ __launch_bounds__(256) void k_instance_boxes(const float* __restrict__ pos,
+                                                        const InstanceBoxJob* __restrict__ jobs, int n,
+                                                        float4* __restrict__ boxes) {
+  __shared__ float red[6][256 / 64];
+  const InstanceBoxJob& jb = jobs[blockIdx.x];
+  if (jb.numVerts <= 0) return;  // uniform per block
+  const float* m = jb.xf;
+  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+  for (int v = threadIdx.x; v < jb.numVerts; v += blockDim.x) {
+    const float* q = pos + 3 * ((size_t)jb.posOff + v);
+    const float x = q[0], y = q[1], z = q[2];
+    const float w[3] = {m[0] * x + m[1] * y + m[2] * z + m[3], m[4] * x + m[5] * y + m[6] * z + m[7],
+                        m[8] * x + m[9] * y + m[10] * z + m[11]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      lo[a] = fminf(lo[a], w[a]);
+      hi[a] = fmaxf(hi[a], w[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      red[a][wave] = lo[a];
+      red[3 + a][wave] = hi[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float l[3], h[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      l[a] = red[a][0];
+      h[a] = red[3 + a][0];
+      for (int k = 1; k < (int)(blockDim.x / 64); k++) {
+        l[a] = fminf(l[a], red[a][k]);
+        h[a] = fmaxf(h[a], red[3 + a][k]);
+      }
+      const float mag = fmaxf(fabsf(l[a]), fabsf(h[a]));
+      const float pad = mag * 2e-6f + 1e-7f;
+      l[a] -= pad;
+      h[a] += pad;
+    }
+    boxes[2 * (size_t)blockIdx.x] = make_float4(l[0], l[1], l[2], 0.0f);
+    boxes[2 * (size_t)blockIdx.x + 1] = make_float4(h[0], h[1], h[2], 0.0f);
+  }
+}
+
 }  // namespace
 
 // Builds one tree into `out` (4-wide nodes at [0, numNodes)) from 2n float4
@@ -417,6 +477,11 @@ int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t 
   HIP_OK(hipStreamSynchronize(s));
   stackBound = sb;
   return num4;
+}
+
+hipError_t launchInstanceBoxes(const float* pos, const InstanceBoxJob* jobs, int n, float4* boxes, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_instance_boxes, dim3(n), dim3(256), 0, s, pos, jobs, n, boxes);
+  return hipGetLastError();
 }
 
 hipError_t launchTriBoxes(const float* pos, const uint32_t* idx, int n, float4* boxes, hipStream_t s) {

@@ -48,6 +48,13 @@ int stackLdsMax();
 struct DevBuf;
 int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t nodeBase, uint32_t primBase,
               DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s);
+// One world-box job of the GPU rebuild: the object-to-world transform and the
+// object's vertices in the position pool (numVerts 0: keep the host's box).
+struct InstanceBoxJob {
+  float xf[12];
+  int32_t posOff, numVerts, pad[2];
+};
+hipError_t launchInstanceBoxes(const float* pos, const InstanceBoxJob* jobs, int n, float4* boxes, hipStream_t s);
 hipError_t launchTriBoxes(const float* pos, const uint32_t* idx, int n, float4* boxes, hipStream_t s);
 hipError_t launchEmitTris(const uint64_t* sortedKeys, int n, const float* pos, const uint32_t* idx, float4* tris,
                           hipStream_t s);
@@ -729,13 +736,31 @@ struct sundog_session {
     HIP_OK(hipStreamSynchronize(stream));
     auto t0 = std::chrono::steady_clock::now();
     const Scene& S = prep.scene;
+    // world boxes: quadrics from their transformed boxes on the host, mesh
+    // instances from their transformed vertices on the GPU
     std::vector<float4> ib;
-    for (const Aabb& w : objectWorldBoxes(S, prep.meshes, meshBox)) {
+    std::vector<InstanceBoxJob> jobs(S.objects.size());
+    for (size_t i = 0; i < S.objects.size(); i++) {
+      const Object& o = S.objects[i];
+      InstanceBoxJob& jb = jobs[i];
+      memcpy(jb.xf, o.xform.m, sizeof(jb.xf));
+      jb.posOff = jb.numVerts = 0;
+      Aabb w;
+      if (o.geomKind == GK_MESH) {
+        jb.posOff = meshPosOff[o.meshId];
+        jb.numVerts = (int)prep.meshes[o.meshId].positions.size();
+        if (jb.numVerts == 0) w = transformBox(o.xform, meshBox[o.meshId]);
+      } else {
+        w = transformBox(o.xform, quadricBox(o.geomKind));
+      }
       ib.push_back(make_float4(w.lo.x, w.lo.y, w.lo.z, 0.0f));
       ib.push_back(make_float4(w.hi.x, w.hi.y, w.hi.z, 0.0f));
     }
-    DevBuf instBoxes, tlas, tlasKeys;
+    DevBuf instBoxes, tlas, tlasKeys, boxJobs;
     instBoxes.upload(ib, stream);
+    boxJobs.upload(jobs, stream);
+    HIP_OK(launchInstanceBoxes(positions.as<float>(), boxJobs.as<InstanceBoxJob>(), (int)jobs.size(),
+                               instBoxes.as<float4>(), stream));
     int tlasStack = 0;
     const int nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack,
                              stream);

@@ -233,7 +233,7 @@ def instanced(asset_dir, width=1920, height=1080, spp=256, max_depth=12, seed=1,
 
 
 def volumes(asset_dir, width=1280, height=720, spp=512, max_depth=16, seed=1, env_w=2048, env_h=1024,
-            clamp=0.0, flame_step=0.01, flame_octaves=5, water_octaves=4):
+            clamp=0.0, flame_step=0.01, flame_octaves=5, water_octaves=4, flame_seed=3):
     os.makedirs(asset_dir, exist_ok=True)
     hdr = os.path.join(asset_dir, "sky_%dx%d.hdr" % (env_w, env_h))
     if not os.path.exists(hdr):
@@ -260,7 +260,7 @@ def volumes(asset_dir, width=1280, height=720, spp=512, max_depth=16, seed=1, en
         s.add("rect", g, scale(0.4, 1, 0.6), rotate_x(90), translate(x, 0.8, -2.0 + 0.03))
         s.add("rect", g, scale(0.4, 1, 0.6), rotate_x(-90), translate(x, 0.8, -2.0 - 0.03))
     s.flame(base=[-1.8, 0.06, -1.2], height=2.0, radius=0.5, intensity=20.0, sigma=4.0,
-            noise_scale=3.0, seed=3, light_intensity=12.0, octaves=flame_octaves,
+            noise_scale=3.0, seed=flame_seed, light_intensity=12.0, octaves=flame_octaves,
             step=flame_step, bounds="box")
     return s
 

@@ -165,6 +165,15 @@ def instanced_variant(v, asset_dir, width, height, spp, seed):
     return s
 
 
+def volumes_variant(v, asset_dir, width, height, spp, seed):
+    rng = np.random.default_rng(3000 + v)
+    s = configs.volumes(asset_dir, width=width, height=height, spp=spp, max_depth=8, seed=seed, env_w=512,
+                        env_h=256, flame_seed=int(rng.integers(1, 1 << 20)))
+    cx, cy = rng.uniform(-0.8, 0.8), rng.uniform(-0.3, 0.5)
+    s.camera(lookfrom=[cx, 1.6 + cy, 5.0], lookat=[0, 0.6, -1.0], vfov=float(rng.uniform(38, 52)))
+    return s
+
+
 def render_pair(scene_fn, noisy_spp, ref_spp, seed):
     """(input [h][w][9], target [h][w][3]) of one scene variant: the noisy
     render and an independent high-spp reference."""
@@ -184,8 +193,11 @@ def render_pair(scene_fn, noisy_spp, ref_spp, seed):
 def dataset(asset_dir, n_variants, width, height, noisy_spp, ref_spp, first_variant=0, seed=11):
     xs, ys = [], []
     for v in range(first_variant, first_variant + n_variants):
+        # the three benchmark frame families in turn: Cornell, volumes, instanced
         if v % 3 == 2:
             fn = lambda spp, sd, v=v: instanced_variant(v, asset_dir, width, height, spp, sd)  # noqa: E731
+        elif v % 3 == 1:
+            fn = lambda spp, sd, v=v: volumes_variant(v, asset_dir, width, height, spp, sd)  # noqa: E731
         else:
             fn = lambda spp, sd, v=v: cornell_variant(v, width, height, spp, sd)  # noqa: E731
         x, y = render_pair(fn, noisy_spp, ref_spp, seed + 101 * v)
@@ -222,10 +234,11 @@ def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0):
 
 
 def measure(args):
-    """Trains on seed-randomised variants (--size squared renders, inputs at 8
-    and 16 spp, targets at --ref-spp, random --crop crops), then renders a
-    held-out Cornell variant at --width x --height: --input-spp noisy +
-    denoise against a plain --target-spp render."""
+    """Trains on seed-randomised variants of the three frame families
+    (--size squared renders, inputs at 8 and 16 spp, targets at --ref-spp,
+    random --crop crops), then renders one held-out variant of each family
+    at --width x --height: --input-spp noisy + denoise against a plain
+    --target-spp render."""
     t0 = time.time()
     xa, ya = dataset(args.asset_dir, args.variants // 2, args.size, args.size, 8, args.ref_spp)
     xb, yb = dataset(args.asset_dir, args.variants - args.variants // 2, args.size, args.size, 16, args.ref_spp,
@@ -239,25 +252,33 @@ def measure(args):
     if args.out:
         den.save(args.out)
     w, h = args.width, args.height
-    v = 1000
-    noisy_sc = cornell_variant(v, w, h, args.input_spp, 5).build()
-    s = noisy_sc.session()
-    s.render_all()
-    b, a, n = s.download()
-    s.close()
-    t2 = time.time()
-    ref_sc = cornell_variant(v, w, h, args.target_spp, 77).build()
-    r = ref_sc.session()
-    r.render_all(chunk=256)
-    ref, _, _ = r.download()
-    r.close()
-    ref_s = time.time() - t2
-    out = den.infer_image(b, a, n)
+    held_out = {
+        "cornell": lambda spp, sd: cornell_variant(1000, w, h, spp, sd),
+        "volumes": lambda spp, sd: volumes_variant(1001, args.asset_dir, w, h, spp, sd),
+        "instanced": lambda spp, sd: instanced_variant(1002, args.asset_dir, w, h, spp, sd),
+    }
+    frames = {}
+    ref_s = 0.0
+    for name, fn in held_out.items():
+        s = fn(args.input_spp, 5).build().session()
+        s.render_all()
+        b, a, n = s.download()
+        s.close()
+        t2 = time.time()
+        r = fn(args.target_spp, 77).build().session()
+        r.render_all(chunk=256)
+        ref, _, _ = r.download()
+        r.close()
+        ref_s += time.time() - t2
+        out = den.infer_image(b, a, n)
+        frames[name] = {"rel_rmse_noisy": rel_rmse(b, ref), "rel_rmse_denoised": rel_rmse(out, ref)}
+        print("%s: noisy %.4f denoised %.4f" % (name, frames[name]["rel_rmse_noisy"],
+                                                 frames[name]["rel_rmse_denoised"]), flush=True)
     ms, fl = den.time_forward(1, (h + 15) // 16 * 16, (w + 15) // 16 * 16, iters=5)
     res = {"train_steps": args.steps, "train_s": train_s, "data_s": data_s,
            "final_loss": float(np.mean(losses[-50:])),
            "frame": [w, h], "input_spp": args.input_spp, "target_spp": args.target_spp, "target_render_s": ref_s,
-           "rel_rmse_noisy": rel_rmse(b, ref), "rel_rmse_denoised": rel_rmse(out, ref),
+           "held_out": frames,
            "denoise_forward_ms": ms, "denoise_gflop": fl / 1e9,
            "mfma_frac_bf16_dense": fl / (ms * 1e-3) / 2.5e15}
     print(json.dumps(res), flush=True)

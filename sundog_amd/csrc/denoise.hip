@@ -200,59 +200,125 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
 }
 
 // Forward convolution for layers with one or two 32-wide output tiles (the
-// full- and half-resolution levels), tiled in space: a block computes 8 x 16
-// output pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) for
-// kNT x 32 output channels. Per chunk of up to 32 input channels it stages
-// the 10 x 18 input halo (zero outside the image; each halo row is one
+// full- and half-resolution levels), tiled in space: a tile is 8 x 16 output
+// pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) by kNT x 32
+// output channels. Per chunk of up to 32 input channels the block stages the
+// 10 x 18 input halo (zero outside the image; each halo row is one
 // contiguous run of pixels in memory) and all nine taps of the chunk's
 // weights in LDS, then runs the chunk's 9 x (chunk / 16) K steps from LDS
-// only. The nine taps of a pixel read the same halo instead of issuing nine
-// scattered 16-byte global loads per fragment, which kept the texture
-// address unit stalled behind the L1 cache. K runs chunk-major (chunk, tap,
-// channel); the sum is the same as the tap-major kernel's up to FP32
-// rounding order. A halo pixel holds 4 16-byte vectors (64 bytes), a weight
-// row 36; vector v of pixel / row i is stored at v ^ ((i >> 2) & 3), so 16
-// consecutive pixels or rows reading the same vector cover all 16 16-byte
-// bank groups.
+// only: the nine taps of a pixel read the same halo instead of issuing nine
+// scattered 16-byte global loads per fragment. K runs chunk-major (chunk,
+// tap, channel); the sum is the same as the tap-major kernel's up to FP32
+// rounding order.
+//
+// Blocks are persistent: block b takes tiles b, b + gridDim.x, ... (so the
+// blocks resident at one time cover a contiguous band of tiles and share
+// halo rows in L2) and walks the steps (tile, chunk) in order. The global
+// loads of step s + 1 are issued into registers before step s's MFMAs and
+// stored to LDS after them, so memory latency overlaps the matrix work. With
+// a single chunk the weights are the same for every step and are staged
+// once per block.
+//
+// A halo pixel holds 4 16-byte vectors (64 bytes), a weight row 36; vector v
+// of pixel / row i is stored at v ^ ((i >> 2) & 3), so 16 consecutive pixels
+// or rows reading the same vector cover all 16 16-byte bank groups.
 constexpr int kHaloTH = 8, kHaloTW = 16, kHaloCi = 32;
 constexpr int kHaloPix = (kHaloTH + 2) * (kHaloTW + 2);  // 180
 constexpr int kHaloRowVecs = 9 * kHaloCi / 8;            // 36 weight vectors per output channel
+constexpr int kHaloAV = (kHaloPix * 4 + 255) / 256;      // halo vectors per thread (3)
 
+template <int kNT>
+struct HaloStage {
+  static constexpr int kBV = (kNT * 32 * kHaloRowVecs + 255) / 256;  // weight vectors per thread
+  u16x8 a[kHaloAV];
+  u16x8 b[kBV];
+};
+
+template <int kNT>
+__device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const uint16_t* __restrict__ in,
+                                         const uint16_t* __restrict__ w, int n, int y0, int x0, int ci0, int cn,
+                                         int H, int W, int Cin, bool weights) {
+#pragma unroll
+  for (int k = 0; k < kHaloAV; k++) {
+    const int i = (int)threadIdx.x + k * 256;
+    const int pix = i >> 2, v = i & 3;
+    const int gy = y0 + pix / (kHaloTW + 2) - 1, gx = x0 + pix % (kHaloTW + 2) - 1;
+    st.a[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < kHaloPix * 4 && v * 8 < cn && gy >= 0 && gy < H && gx >= 0 && gx < W)
+      st.a[k] = *(const u16x8*)(in + (((size_t)n * H + gy) * W + gx) * Cin + ci0 + v * 8);
+  }
+  if (!weights) return;
+#pragma unroll
+  for (int k = 0; k < HaloStage<kNT>::kBV; k++) {
+    const int i = (int)threadIdx.x + k * 256;
+    const int r = i / kHaloRowVecs, q = i - r * kHaloRowVecs;  // q = tap * 4 + vector
+    const int t = q >> 2, v = q & 3;
+    st.b[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < kNT * 32 * kHaloRowVecs && v * 8 < cn)
+      st.b[k] = *(const u16x8*)(w + ((size_t)r * 9 + t) * Cin + ci0 + v * 8);
+  }
+}
+
+template <int kNT>
+__device__ __forceinline__ void haloStore(const HaloStage<kNT>& st, u16x8* sA, u16x8* sB, bool weights) {
+#pragma unroll
+  for (int k = 0; k < kHaloAV; k++) {
+    const int i = (int)threadIdx.x + k * 256;
+    const int pix = i >> 2, v = i & 3;
+    if (i < kHaloPix * 4) sA[pix * 4 + (v ^ ((pix >> 2) & 3))] = st.a[k];
+  }
+  if (!weights) return;
+#pragma unroll
+  for (int k = 0; k < HaloStage<kNT>::kBV; k++) {
+    const int i = (int)threadIdx.x + k * 256;
+    const int r = i / kHaloRowVecs, q = i - r * kHaloRowVecs;
+    if (i < kNT * 32 * kHaloRowVecs) sB[r * kHaloRowVecs + (q ^ ((r >> 2) & 3))] = st.b[k];
+  }
+}
+
+// Cout == 32 * kNT; Cin % 16 == 0; gridDim.x <= numTiles = N x tilesY x tilesX
 template <bool kOutF32, int kNT>
-__global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
+__global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
                                                    const float* __restrict__ bias, void* __restrict__ outV, int H,
-                                                   int W, int Cin, int Cout, int relu) {
+                                                   int W, int Cin, int numTiles, int relu) {
+  constexpr int Cout = 32 * kNT;
   __shared__ u16x8 sA[kHaloPix * 4];
   __shared__ u16x8 sB[kNT * 32 * kHaloRowVecs];
-  const int tilesX = (W + kHaloTW - 1) / kHaloTW;
-  const int x0 = (blockIdx.x % tilesX) * kHaloTW, y0 = (blockIdx.x / tilesX) * kHaloTH;
-  const int n = blockIdx.z;
-  const int co0 = blockIdx.y * 32 * kNT;
+  const int tilesX = (W + kHaloTW - 1) / kHaloTW, tilesY = (H + kHaloTH - 1) / kHaloTH;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 31, half = lane >> 5;
   const int py = 2 * wave + (row >> 4), px = row & 15;  // this lane's A row (tile coordinates)
+  const int nc = (Cin + kHaloCi - 1) / kHaloCi;
+  const int myTiles = (numTiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int steps = myTiles * nc;
+  // (image, tile origin) of step s
+  auto origin = [&](int s, int& n, int& y0, int& x0) {
+    const int tile = (int)blockIdx.x + (s / nc) * (int)gridDim.x;
+    n = tile / (tilesX * tilesY);
+    const int ty = tile - n * tilesX * tilesY;
+    y0 = (ty / tilesX) * kHaloTH;
+    x0 = (ty % tilesX) * kHaloTW;
+  };
+  HaloStage<kNT> st;
+  {
+    int n, y0, x0;
+    origin(0, n, y0, x0);
+    haloLoad<kNT>(st, in, w, n, y0, x0, 0, min(kHaloCi, Cin), H, W, Cin, true);
+  }
   f32x16 acc[kNT];
 #pragma unroll
   for (int j = 0; j < kNT; j++) acc[j] = f32x16{};
-  for (int ci0 = 0; ci0 < Cin; ci0 += kHaloCi) {
-    const int cn = min(kHaloCi, Cin - ci0);  // 16 or 32
-    __syncthreads();                          // the previous chunk is consumed
-    for (int i = threadIdx.x; i < kHaloPix * 4; i += 256) {
-      const int pix = i >> 2, v = i & 3;
-      const int gy = y0 + pix / (kHaloTW + 2) - 1, gx = x0 + pix % (kHaloTW + 2) - 1;
-      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (v * 8 < cn && gy >= 0 && gy < H && gx >= 0 && gx < W)
-        val = *(const u16x8*)(in + (((size_t)n * H + gy) * W + gx) * Cin + ci0 + v * 8);
-      sA[pix * 4 + (v ^ ((pix >> 2) & 3))] = val;
-    }
-    for (int i = threadIdx.x; i < kNT * 32 * kHaloRowVecs; i += 256) {
-      const int r = i / kHaloRowVecs, q = i - r * kHaloRowVecs;  // q = tap * 4 + vector
-      const int t = q >> 2, v = q & 3;
-      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (v * 8 < cn) val = *(const u16x8*)(w + ((size_t)(co0 + r) * 9 + t) * Cin + ci0 + v * 8);
-      sB[r * kHaloRowVecs + (q ^ ((r >> 2) & 3))] = val;
-    }
+  for (int s = 0; s < steps; s++) {
+    const int c = s % nc, ci0 = c * kHaloCi, cn = min(kHaloCi, Cin - ci0);
+    __syncthreads();  // step s - 1's MFMAs are done with the LDS
+    haloStore<kNT>(st, sA, sB, nc > 1 || s == 0);
     __syncthreads();
+    if (s + 1 < steps) {
+      int n1, y1, x1;
+      origin(s + 1, n1, y1, x1);
+      const int c1 = (s + 1) % nc;
+      haloLoad<kNT>(st, in, w, n1, y1, x1, c1 * kHaloCi, min(kHaloCi, Cin - c1 * kHaloCi), H, W, Cin, nc > 1);
+    }
 #pragma unroll 3
     for (int t = 0; t < 9; t++) {
       const int hp = (py + t / 3) * (kHaloTW + 2) + px + t % 3;
@@ -268,24 +334,50 @@ __global__ __launch_bounds__(256) void k_conv_halo(const uint16_t* __restrict__ 
         }
       }
     }
-  }
+    if (c == nc - 1) {
+      int n, y0, x0;
+      origin(s, n, y0, x0);
 #pragma unroll
-  for (int j = 0; j < kNT; j++) {
-    const int co = co0 + j * kTile + row;  // D column = lane % 32
-    const float bv = bias ? bias[co] : 0.0f;
+      for (int j = 0; j < kNT; j++) {
+        const int co = j * kTile + row;  // D column = lane % 32
+        const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
-      const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
-      if (oy < H && ox < W) {
-        const size_t pp = ((size_t)n * H + oy) * W + ox;
-        float v = acc[j][r] + bv;
-        if (relu) v = fmaxf(v, 0.0f);
-        if (kOutF32) ((float*)outV)[pp * Cout + co] = v;
-        else ((uint16_t*)outV)[pp * Cout + co] = f2bf(v);
+        for (int r = 0; r < 16; r++) {
+          const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
+          const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
+          if (oy < H && ox < W) {
+            const size_t pp = ((size_t)n * H + oy) * W + ox;
+            float v = acc[j][r] + bv;
+            if (relu) v = fmaxf(v, 0.0f);
+            if (kOutF32) ((float*)outV)[pp * Cout + co] = v;
+            else ((uint16_t*)outV)[pp * Cout + co] = f2bf(v);
+          }
+        }
+        acc[j] = f32x16{};
       }
     }
   }
+}
+
+// Blocks of k_conv_halo<outF32, nt> resident on the current device at once
+// (the persistent grid size), cached per device.
+static int haloResident(int nt, bool outF32) {
+  static int cache[16][4] = {};
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  int& slot = cache[dev & 15][(nt - 1) * 2 + (outF32 ? 1 : 0)];
+  if (slot == 0) {
+    int cus = 0, per = 0;
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (outF32)
+      HIP_OK(nt == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<true, 1>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<true, 2>, 256, 0));
+    else
+      HIP_OK(nt == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<false, 1>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<false, 2>, 256, 0));
+    slot = std::max(1, cus * std::max(1, per));
+  }
+  return slot;
 }
 
 // Output-channel tiles per wave for a convolution with `tiles` 32-wide tiles:
@@ -736,17 +828,18 @@ struct sundog_denoiser {
     const int nt = convTilesPerWave(c.cout / 32);
     const float* b = net.params.as<float>() + c.bOff;
     if (nt <= 2 && c.cout == 32 * nt) {
-      // one or two output tiles: the spatially tiled kernel
-      dim3 hg(((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH), 1, N);
+      // one or two output tiles: the spatially tiled kernel, persistent blocks
+      const int numTiles = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH);
       const uint16_t* ib = (const uint16_t*)in;
       const uint16_t* wf = c.wf.as<uint16_t>();
       const int rl = c.relu ? 1 : 0;
+      const int g = std::min(numTiles, haloResident(nt, outF32));
       if (outF32) {
-        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<true, 1>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
-        else hipLaunchKernelGGL((k_conv_halo<true, 2>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<true, 1>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
+        else hipLaunchKernelGGL((k_conv_halo<true, 2>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
       } else {
-        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<false, 1>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
-        else hipLaunchKernelGGL((k_conv_halo<false, 2>), hg, dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, c.cout, rl);
+        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<false, 1>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
+        else hipLaunchKernelGGL((k_conv_halo<false, 2>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
       }
       (void)inF32;
       HIP_OK(hipGetLastError());

@@ -69,6 +69,8 @@ constexpr int kTile = 32;  // MFMA 32x32x16
 // out[p][co] = bias[co] + sum_t sum_ci in[p + tapOff(t)][ci] * w[co][t][ci];
 // tapSign = +1 for forward, -1 for backward-data (the transposed conv reads
 // the neighbour on the opposite side). Cin % 16 == 0, Cout % (32 * kNT) == 0.
+// The network runs its backward-data pass here (FP32 gradient input); its
+// forward layers run the spatially tiled k_conv_halo below.
 //
 // A block is 4 waves x 32 pixels (M = 128) by 32 * kNT output channels (N).
 // K = 9 * Cin runs tap-major (k = tap * Cin + ci, the weights' own row
@@ -199,10 +201,11 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
   }
 }
 
-// Forward convolution for layers with one or two 32-wide output tiles (the
-// full- and half-resolution levels), tiled in space: a tile is 8 x 16 output
-// pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) by kNT x 32
-// output channels. Per chunk of up to 32 input channels the block stages the
+// Forward convolution (BF16 input), tiled in space: a work item is 8 x 16
+// output pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) by one
+// block of kNT x 32 output channels; a layer has Cout / (32 kNT) channel
+// blocks, and the channel blocks of one tile are consecutive items, so they
+// run together and read the same halo from L2. Per chunk of up to 32 input channels the block stages the
 // 10 x 18 input halo (zero outside the image; each halo row is one
 // contiguous run of pixels in memory) and all nine taps of the chunk's
 // weights in LDS, then runs the chunk's 9 x (chunk / 16) K steps from LDS
@@ -211,13 +214,13 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, cons
 // tap, channel); the sum is the same as the tap-major kernel's up to FP32
 // rounding order.
 //
-// Blocks are persistent: block b takes tiles b, b + gridDim.x, ... (so the
+// Blocks are persistent: block b takes items b, b + gridDim.x, ... (so the
 // blocks resident at one time cover a contiguous band of tiles and share
-// halo rows in L2) and walks the steps (tile, chunk) in order. The global
+// halo rows in L2) and walks the steps (item, chunk) in order. The global
 // loads of step s + 1 are issued into registers before step s's MFMAs and
 // stored to LDS after them, so memory latency overlaps the matrix work. With
-// a single chunk the weights are the same for every step and are staged
-// once per block.
+// a single chunk and a single channel block the weights are the same for
+// every step and are staged once per block.
 //
 // A halo pixel holds 4 16-byte vectors (64 bytes), a weight row 36; vector v
 // of pixel / row i is stored at v ^ ((i >> 2) & 3), so 16 consecutive pixels
@@ -236,8 +239,8 @@ struct HaloStage {
 
 template <int kNT>
 __device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const uint16_t* __restrict__ in,
-                                         const uint16_t* __restrict__ w, int n, int y0, int x0, int ci0, int cn,
-                                         int H, int W, int Cin, bool weights) {
+                                         const uint16_t* __restrict__ w, int n, int y0, int x0, int co0, int ci0,
+                                         int cn, int H, int W, int Cin, bool weights) {
 #pragma unroll
   for (int k = 0; k < kHaloAV; k++) {
     const int i = (int)threadIdx.x + k * 256;
@@ -255,7 +258,7 @@ __device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const uint16_t* __r
     const int t = q >> 2, v = q & 3;
     st.b[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
     if (i < kNT * 32 * kHaloRowVecs && v * 8 < cn)
-      st.b[k] = *(const u16x8*)(w + ((size_t)r * 9 + t) * Cin + ci0 + v * 8);
+      st.b[k] = *(const u16x8*)(w + ((size_t)(co0 + r) * 9 + t) * Cin + ci0 + v * 8);
   }
 }
 
@@ -276,12 +279,13 @@ __device__ __forceinline__ void haloStore(const HaloStage<kNT>& st, u16x8* sA, u
   }
 }
 
-// Cout == 32 * kNT; Cin % 16 == 0; gridDim.x <= numTiles = N x tilesY x tilesX
+// Cout % (32 * kNT) == 0; Cin % 16 == 0;
+// gridDim.x <= numItems = N x tilesY x tilesX x Cout / (32 * kNT)
 template <bool kOutF32, int kNT>
 __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
                                                    const float* __restrict__ bias, void* __restrict__ outV, int H,
-                                                   int W, int Cin, int numTiles, int relu) {
-  constexpr int Cout = 32 * kNT;
+                                                   int W, int Cin, int Cout, int numItems, int relu) {
+  const int coBlocks = Cout / (32 * kNT);
   __shared__ u16x8 sA[kHaloPix * 4];
   __shared__ u16x8 sB[kNT * 32 * kHaloRowVecs];
   const int tilesX = (W + kHaloTW - 1) / kHaloTW, tilesY = (H + kHaloTH - 1) / kHaloTH;
@@ -289,11 +293,14 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
   const int row = lane & 31, half = lane >> 5;
   const int py = 2 * wave + (row >> 4), px = row & 15;  // this lane's A row (tile coordinates)
   const int nc = (Cin + kHaloCi - 1) / kHaloCi;
-  const int myTiles = (numTiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int steps = myTiles * nc;
-  // (image, tile origin) of step s
-  auto origin = [&](int s, int& n, int& y0, int& x0) {
-    const int tile = (int)blockIdx.x + (s / nc) * (int)gridDim.x;
+  const int myItems = (numItems - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int steps = myItems * nc;
+  const bool fixedW = nc == 1 && coBlocks == 1;  // the same weights for every step
+  // (image, tile origin, first output channel) of step s
+  auto origin = [&](int s, int& n, int& y0, int& x0, int& co0) {
+    const int item = (int)blockIdx.x + (s / nc) * (int)gridDim.x;
+    const int tile = item / coBlocks;
+    co0 = (item - tile * coBlocks) * 32 * kNT;
     n = tile / (tilesX * tilesY);
     const int ty = tile - n * tilesX * tilesY;
     y0 = (ty / tilesX) * kHaloTH;
@@ -301,9 +308,9 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
   };
   HaloStage<kNT> st;
   {
-    int n, y0, x0;
-    origin(0, n, y0, x0);
-    haloLoad<kNT>(st, in, w, n, y0, x0, 0, min(kHaloCi, Cin), H, W, Cin, true);
+    int n, y0, x0, co0;
+    origin(0, n, y0, x0, co0);
+    haloLoad<kNT>(st, in, w, n, y0, x0, co0, 0, min(kHaloCi, Cin), H, W, Cin, true);
   }
   f32x16 acc[kNT];
 #pragma unroll
@@ -311,13 +318,13 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
   for (int s = 0; s < steps; s++) {
     const int c = s % nc, ci0 = c * kHaloCi, cn = min(kHaloCi, Cin - ci0);
     __syncthreads();  // step s - 1's MFMAs are done with the LDS
-    haloStore<kNT>(st, sA, sB, nc > 1 || s == 0);
+    haloStore<kNT>(st, sA, sB, !fixedW || s == 0);
     __syncthreads();
     if (s + 1 < steps) {
-      int n1, y1, x1;
-      origin(s + 1, n1, y1, x1);
+      int n1, y1, x1, co1;
+      origin(s + 1, n1, y1, x1, co1);
       const int c1 = (s + 1) % nc;
-      haloLoad<kNT>(st, in, w, n1, y1, x1, c1 * kHaloCi, min(kHaloCi, Cin - c1 * kHaloCi), H, W, Cin, nc > 1);
+      haloLoad<kNT>(st, in, w, n1, y1, x1, co1, c1 * kHaloCi, min(kHaloCi, Cin - c1 * kHaloCi), H, W, Cin, !fixedW);
     }
 #pragma unroll 3
     for (int t = 0; t < 9; t++) {
@@ -335,11 +342,11 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
       }
     }
     if (c == nc - 1) {
-      int n, y0, x0;
-      origin(s, n, y0, x0);
+      int n, y0, x0, co0;
+      origin(s, n, y0, x0, co0);
 #pragma unroll
       for (int j = 0; j < kNT; j++) {
-        const int co = j * kTile + row;  // D column = lane % 32
+        const int co = co0 + j * kTile + row;  // D column = lane % 32
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
         for (int r = 0; r < 16; r++) {
@@ -822,37 +829,32 @@ struct sundog_denoiser {
     lossPartial.alloc(blocks(P0 * 32) * 4);
   }
 
-  void conv(int k, const void* in, bool inF32, void* out, bool outF32, int h, int w) {
+  // forward convolution k on a BF16 input: the spatially tiled kernel with
+  // persistent blocks, two output tiles per item where the channel count
+  // allows, else one
+  void conv(int k, const uint16_t* in, void* out, bool outF32, int h, int w) {
     const Conv& c = net.convs[k];
-    int P = N * h * w;
-    const int nt = convTilesPerWave(c.cout / 32);
     const float* b = net.params.as<float>() + c.bOff;
-    if (nt <= 2 && c.cout == 32 * nt) {
-      // one or two output tiles: the spatially tiled kernel, persistent blocks
-      const int numTiles = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH);
-      const uint16_t* ib = (const uint16_t*)in;
-      const uint16_t* wf = c.wf.as<uint16_t>();
-      const int rl = c.relu ? 1 : 0;
-      const int g = std::min(numTiles, haloResident(nt, outF32));
-      if (outF32) {
-        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<true, 1>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
-        else hipLaunchKernelGGL((k_conv_halo<true, 2>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
-      } else {
-        if (nt == 1) hipLaunchKernelGGL((k_conv_halo<false, 1>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
-        else hipLaunchKernelGGL((k_conv_halo<false, 2>), dim3(g), dim3(256), 0, stream, ib, wf, b, out, h, w, c.cin, numTiles, rl);
-      }
-      (void)inF32;
-      HIP_OK(hipGetLastError());
-      return;
+    const int ht = (c.cout / 32) % 2 == 0 ? 2 : 1;
+    const int numItems = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH) * (c.cout / (32 * ht));
+    const uint16_t* wf = c.wf.as<uint16_t>();
+    const int rl = c.relu ? 1 : 0;
+    const int g = std::min(numItems, haloResident(ht, outF32));
+    if (outF32) {
+      if (ht == 1)
+        hipLaunchKernelGGL((k_conv_halo<true, 1>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
+                           c.cout, numItems, rl);
+      else
+        hipLaunchKernelGGL((k_conv_halo<true, 2>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
+                           c.cout, numItems, rl);
+    } else {
+      if (ht == 1)
+        hipLaunchKernelGGL((k_conv_halo<false, 1>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
+                           c.cout, numItems, rl);
+      else
+        hipLaunchKernelGGL((k_conv_halo<false, 2>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
+                           c.cout, numItems, rl);
     }
-    dim3 grid((P + 127) / 128, c.cout / (32 * nt));
-    if (outF32)
-      launchConv<false, true>(nt, grid, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w, c.cin, c.cout, 1,
-                              c.relu ? 1 : 0);
-    else
-      launchConv<false, false>(nt, grid, stream, in, c.wf.as<uint16_t>(), b, out, N, h, w, c.cin, c.cout, 1,
-                               c.relu ? 1 : 0);
-    (void)inF32;
     HIP_OK(hipGetLastError());
   }
 
@@ -868,8 +870,8 @@ struct sundog_denoiser {
                            pooled[l - 1].as<uint16_t>(), N, h * 2, w * 2, kCh[l - 1]);
         cur = pooled[l - 1].as<uint16_t>();
       }
-      conv(k, cur, false, y[k].p, false, h, w);
-      conv(k + 1, y[k].p, false, y[k + 1].p, false, h, w);
+      conv(k, cur, y[k].p, false, h, w);
+      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, false, h, w);
       cur = y[k + 1].as<uint16_t>();
       k += 2;
     }
@@ -879,12 +881,12 @@ struct sundog_denoiser {
       size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]) / 8;  // 16-byte vectors
       hipLaunchKernelGGL(k_upcat_fwd, dim3(blocks(tot)), dim3(256), 0, stream, cur, y[skipConv].as<uint16_t>(),
                          cat[l].as<uint16_t>(), N, h, w, kCh[l + 1], kCh[l]);
-      conv(k, cat[l].p, false, y[k].p, false, h, w);
-      conv(k + 1, y[k].p, false, y[k + 1].p, false, h, w);
+      conv(k, cat[l].as<uint16_t>(), y[k].p, false, h, w);
+      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, false, h, w);
       cur = y[k + 1].as<uint16_t>();
       k += 2;
     }
-    conv(k, cur, false, pred.p, true, H, W);
+    conv(k, cur, pred.p, true, H, W);
     size_t P = (size_t)N * H * W;
     hipLaunchKernelGGL(k_final, dim3(blocks(P * 3)), dim3(256), 0, stream, pred.as<float>(), devIn.as<float>(),
                        fin.as<float>(), P);

@@ -1551,7 +1551,7 @@ DEV float4 runningMean(float4 m, vec3 x, int s) {
 // (device/programs.cu:493-505), so everything after the query waits for it.
 // ---------------------------------------------------------------------------
 
-enum PathNext { PN_TRACE = 0, PN_SHADOW = 1, PN_END = 2 };
+enum PathNext { PN_TRACE = 0, PN_SHADOW = 1, PN_END = 2, PN_SCATTER = 3 };
 
 struct PathState {
   vec3 o, d;  // the path ray of the next closest-hit query
@@ -1670,11 +1670,32 @@ DEV int pathScatter(const LaunchParams& P, PathState& ps, const SurfState& sf, P
   return ps.depth < P.maxDepth ? PN_TRACE : PN_END;
 }
 
+// The sample's first-hit AOVs: into the path state, or, when aovOut is set,
+// straight into the sample's slot of a pooled sample buffer (planes 3..8 at
+// a stride of kPoolSamples x 64 floats), so they are not carried in
+// registers through the rest of the path.
+DEV void storeAov(PathState& ps, float* aovOut, vec3 a, vec3 n) {
+  if (aovOut) {
+    constexpr int kS = kPoolSamples * 64;
+    aovOut[0] = a.x;
+    aovOut[kS] = a.y;
+    aovOut[2 * kS] = a.z;
+    aovOut[3 * kS] = n.x;
+    aovOut[4 * kS] = n.y;
+    aovOut[5 * kS] = n.z;
+  } else {
+    ps.aovA = a;
+    ps.aovN = n;
+  }
+}
+
 // After the closest-hit query of ps.o/ps.d (isHit, hit; oObj/dObj is the
 // ray in the hit instance's object space). Returns PN_SHADOW when an NEE
-// shadow query (shadowRay) must run before the bounce can finish.
+// shadow query (shadowRay) must run before the bounce can finish; with
+// deferScatter, PN_SCATTER when the bounce only needs its BSDF sample
+// (pathScatter from ps.rngState), otherwise the bounce is finished here.
 DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& hit, vec3 oObj, vec3 dObj,
-                  SurfState& sf) {
+                  SurfState& sf, float* aovOut = nullptr, bool deferScatter = false) {
   const int nStrat = neeStrategies(P);
   Pcg rng = pathRng(P, ps);
   if (ps.medDepth > 0) {
@@ -1700,10 +1721,7 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
     vec3 c = ps.beta * bg * w;
     if (ps.depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
     ps.L += c;
-    if (ps.depth == 0) {
-      ps.aovA = vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)};
-      ps.aovN = v3(0.0f);
-    }
+    if (ps.depth == 0) storeAov(ps, aovOut, vec3{fminf(bg.x, 1.0f), fminf(bg.y, 1.0f), fminf(bg.z, 1.0f)}, v3(0.0f));
     ps.rngState = rng.state;
     return PN_END;
   }
@@ -1720,10 +1738,8 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
     ns = waterNormal(sf.x, ns, -ps.d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
   sf.ns = ns;
 
-  if (ps.depth == 0) {
-    ps.aovA = sf.albedo;
-    ps.aovN = vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))};
-  }
+  if (ps.depth == 0)
+    storeAov(ps, aovOut, sf.albedo, vec3{dot(ns, ld3(P.cam.u)), dot(ns, ld3(P.cam.v)), dot(ns, ld3(P.cam.w))});
 
   if (mat.kind == MT_EMISSIVE) {
     if (sp.frontface || mat.twoSided) {
@@ -1766,6 +1782,10 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
       return PN_SHADOW;
     }
   }
+  if (deferScatter) {
+    ps.rngState = rng.state;
+    return PN_SCATTER;
+  }
   int r = pathScatter(P, ps, sf, rng);
   ps.rngState = rng.state;
   return r;
@@ -1778,28 +1798,40 @@ DEV void shadowRay(const SurfState& sf, vec3& so, vec3& wi, float& tmax) {
   tmax = sf.dist * 0.999f;
 }
 
+// MIS-weighted light contribution of the suspended NEE sample before the
+// shadow query's transmittance is applied (device/programs.cu:481-524).
+// Draws no random numbers.
+DEV vec3 neeUnshadowed(const LaunchParams& P, const PathState& ps, const SurfState& sf) {
+  const int nStrat = neeStrategies(P);
+  const DevMaterial& mat = P.materials[sf.matId];
+  float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
+  vec3 f = bsdfEval(mat, sf.albedo, -ps.d, sf.wi, sf.ns, sf.etaRel);
+  float w = 1.0f;
+  if (!sf.isDelta) {
+    float pdfB = bsdfPdf(mat, -ps.d, sf.wi, sf.ns, sf.etaRel);
+    w = pdfLe / (pdfLe + pdfB);
+  }
+  return ps.beta * f * fabsf(sf.cosS) * sf.Li * w / pdfLe;
+}
+
+// Whether the random numbers a bounce draws after its shadow query depend on
+// that query's outcome: only the flame transmittance of a transparent
+// shadow ray, which is marched (and draws) when the ray is unoccluded.
+DEV bool shadowDraws(const LaunchParams& P) { return P.numFlames > 0 && P.transparentShadows; }
+
 // After the shadow query (transmittance tr): MIS-weighted light
 // contribution (device/programs.cu:481-524), then the BSDF sample.
 DEV int pathOnShadow(const LaunchParams& P, PathState& ps, const SurfState& sf, vec3 tr) {
-  const int nStrat = neeStrategies(P);
   Pcg rng = pathRng(P, ps);
   if (maxc(tr) > 0.0f) {
-    const DevMaterial& mat = P.materials[sf.matId];
-    if (P.numFlames > 0 && P.transparentShadows) {
+    if (shadowDraws(P)) {
       vec3 so, wi;
       float tmaxS;
       shadowRay(sf, so, wi, tmaxS);
       int skip = (sf.k < P.numLights) ? P.lights[sf.k].flameId : -1;
       tr = tr * flameTransmittance(P, so, wi, tmaxS, rng, skip);
     }
-    float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
-    vec3 f = bsdfEval(mat, sf.albedo, -ps.d, sf.wi, sf.ns, sf.etaRel);
-    float w = 1.0f;
-    if (!sf.isDelta) {
-      float pdfB = bsdfPdf(mat, -ps.d, sf.wi, sf.ns, sf.etaRel);
-      w = pdfLe / (pdfLe + pdfB);
-    }
-    vec3 c = ps.beta * f * fabsf(sf.cosS) * sf.Li * w / pdfLe;
+    vec3 c = neeUnshadowed(P, ps, sf);
     c = c * tr;
     if (ps.depth >= 1 && P.clampVal > 0.0f) c = vmin(c, v3(P.clampVal));
     ps.L += sanitize(c);
@@ -1876,20 +1908,57 @@ DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* 
       bool isHit = traceClosest(P, st, ps.o, ps.d, 0.0f, 1e16f, hit, oObj, dObj, wc);
       rays++;
       SurfState sf;
-      int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf);
+      // first-hit AOVs go straight to the sample's slot
+      int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf, waveBuf + 3 * (kPoolSamples * 64) + item, true);
+      // The NEE sample's unshadowed contribution is computed before its
+      // shadow query. Unless the flame march of a transparent shadow ray
+      // draws random numbers after the query (which the BSDF sample's
+      // draws must follow), the BSDF sample is taken before the shadow
+      // query too, so the surface state is dead while the shadow ray
+      // traverses. Same operations on the same values, in the same draw
+      // order, as pathOnShadow.
+      vec3 so = v3(0.0f), wi = v3(0.0f), cNee = v3(0.0f);
+      float tmaxS = 0.0f;
+      bool clampNee = false, pending = false;
       if (r == PN_SHADOW) {
-        vec3 so, wi;
-        float tmaxS;
         shadowRay(sf, so, wi, tmaxS);
         rays++;
-        vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
-        r = pathOnShadow(P, ps, sf, tr);
+        cNee = neeUnshadowed(P, ps, sf);
+        clampNee = ps.depth >= 1 && P.clampVal > 0.0f;
+        if (shadowDraws(P)) {
+          vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
+          if (maxc(tr) > 0.0f) {
+            Pcg rng = pathRng(P, ps);
+            const int skip = (sf.k < P.numLights) ? P.lights[sf.k].flameId : -1;
+            tr = tr * flameTransmittance(P, so, wi, tmaxS, rng, skip);
+            ps.rngState = rng.state;
+            vec3 c = cNee * tr;
+            if (clampNee) c = vmin(c, v3(P.clampVal));
+            ps.L += sanitize(c);
+          }
+        } else {
+          pending = true;
+        }
+        r = PN_SCATTER;
+      }
+      if (r == PN_SCATTER) {
+        Pcg rng = pathRng(P, ps);
+        r = pathScatter(P, ps, sf, rng);
+        ps.rngState = rng.state;
+      }
+      if (pending) {
+        const vec3 tr = traceShadow(P, st, so, wi, tmaxS, wc);
+        if (maxc(tr) > 0.0f) {
+          vec3 c = cNee * tr;
+          if (clampNee) c = vmin(c, v3(P.clampVal));
+          ps.L += sanitize(c);
+        }
       }
       if (r == PN_END) {
         const vec3 L = sanitize(ps.L);
-        const float v[kPoolPlanes] = {L.x, L.y, L.z, ps.aovA.x, ps.aovA.y, ps.aovA.z, ps.aovN.x, ps.aovN.y, ps.aovN.z};
+        const float v[3] = {L.x, L.y, L.z};
 #pragma unroll
-        for (int k = 0; k < kPoolPlanes; k++) waveBuf[k * (kPoolSamples * 64) + item] = v[k];
+        for (int k = 0; k < 3; k++) waveBuf[k * (kPoolSamples * 64) + item] = v[k];
         item = (int)atomicAdd(poolNext, 1u);
         alive = claim();
       }

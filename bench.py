@@ -36,15 +36,21 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_render_instanced_1080p.json")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
-    WRITE_SIZE counter profile of this workload, or None."""
+def counter_profile(kernel):
+    """The committed counter profile of this workload's kernel (FETCH_SIZE /
+    WRITE_SIZE traffic, VALU busy and lane utilisation), or {}."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
-        return None
-    return prof["traffic_bytes"] if prof.get("kernel") == kernel else None
+        return {}
+    return prof if prof.get("kernel") == kernel else {}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter profile of this workload, or None."""
+    return counter_profile(kernel).get("traffic_bytes")
 
 
 def copy_probe(mib):
@@ -212,6 +218,13 @@ def main():
             "kernel_ms_avg": kernel_ms_avg,
             "algorithmic_flops_per_query": flops_per_query,
             "work_per_query": {k2: v / queries for k2, v in work.items()},
+            # issue-side view from the same committed counter profile:
+            # VALU busy = SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE per XCD,
+            # lane utilisation = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)
+            "valu_busy": (counter_profile(kname).get("valu_busy")
+                          if args.workload == "instanced" else None),
+            "valu_lane_utilization": (counter_profile(kname).get("valu_lane_utilization")
+                                      if args.workload == "instanced" else None),
         },
         "setup_s": setup_s,
         "bvh_build_ms": {"host_sah_total_build": st0["build_ms"], "gpu_lbvh": st0["bvh_gpu_ms"]},

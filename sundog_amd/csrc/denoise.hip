@@ -25,6 +25,7 @@
 // order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -237,18 +238,68 @@ struct HaloStage {
   u16x8 b[kBV];
 };
 
-template <int kNT>
-__device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const uint16_t* __restrict__ in,
-                                         const uint16_t* __restrict__ w, int n, int y0, int x0, int co0, int ci0,
-                                         int cn, int H, int W, int Cin, bool weights) {
+// Where a halo kernel's input pixels come from, and what its epilogue
+// writes. The inference forward reads the pooled and the concatenated
+// layer inputs straight from their sources instead of materialising them
+// (training keeps them: its backward pass reads them).
+enum HaloSrc {
+  HS_PLAIN = 0,  // in[P][Cin]
+  HS_POOL = 1,   // 2x2 average of in[N][2H][2W][Cin], rounded to BF16 as k_pool_fwd does
+  HS_UPCAT = 2,  // channels < ca: in[N][H/2][W/2][ca] (nearest up-sample); the rest: in2[P][Cin - ca]
+};
+enum HaloOut {
+  HO_BF16 = 0,   // out[P][Cout] BF16
+  HO_F32 = 1,    // out[P][Cout] FP32
+  HO_FINAL = 2,  // out[P][3] FP32 = conv + log1p(max(raw[P][9] beauty, 0)), as k_final does
+};
+
+struct HaloArgs {
+  const uint16_t* in;
+  const uint16_t* in2;
+  const uint16_t* w;
+  const float* bias;
+  void* out;
+  const float* raw;  // HO_FINAL: the network's FP32 input [P][9]
+  int H, W, Cin, Cout, ca, numItems, relu;
+};
+
+__device__ __forceinline__ float4 bf4lo(u16x8 v) { return make_float4(bf2f(v[0]), bf2f(v[1]), bf2f(v[2]), bf2f(v[3])); }
+__device__ __forceinline__ float4 bf4hi(u16x8 v) { return make_float4(bf2f(v[4]), bf2f(v[5]), bf2f(v[6]), bf2f(v[7])); }
+
+// 8 channels (16 bytes) of input pixel (n, gy, gx), channels c .. c + 7
+template <int kSrc>
+__device__ __forceinline__ u16x8 haloPixel(const HaloArgs& A, int n, int gy, int gx, int c) {
+  if (kSrc == HS_POOL) {
+    // the same sum order as k_pool_fwd: (y, x), (y, x+1), (y+1, x), (y+1, x+1)
+    const size_t W2 = (size_t)A.W * 2;
+    const uint16_t* b = A.in + (((size_t)n * A.H * 2 + 2 * gy) * W2 + 2 * gx) * A.Cin + c;
+    const u16x8 p0 = *(const u16x8*)b, p1 = *(const u16x8*)(b + A.Cin);
+    const u16x8 p2 = *(const u16x8*)(b + W2 * A.Cin), p3 = *(const u16x8*)(b + W2 * A.Cin + A.Cin);
+    u16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = f2bf(0.25f * (bf2f(p0[e]) + bf2f(p1[e]) + bf2f(p2[e]) + bf2f(p3[e])));
+    return r;
+  } else if (kSrc == HS_UPCAT) {
+    if (c < A.ca)
+      return *(const u16x8*)(A.in + (((size_t)n * (A.H / 2) + gy / 2) * (A.W / 2) + gx / 2) * A.ca + c);
+    const int cb = A.Cin - A.ca;
+    return *(const u16x8*)(A.in2 + (((size_t)n * A.H + gy) * A.W + gx) * cb + (c - A.ca));
+  } else {
+    return *(const u16x8*)(A.in + (((size_t)n * A.H + gy) * A.W + gx) * A.Cin + c);
+  }
+}
+
+template <int kNT, int kSrc>
+__device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const HaloArgs& A, int n, int y0, int x0, int co0,
+                                         int ci0, int cn, bool weights) {
 #pragma unroll
   for (int k = 0; k < kHaloAV; k++) {
     const int i = (int)threadIdx.x + k * 256;
     const int pix = i >> 2, v = i & 3;
     const int gy = y0 + pix / (kHaloTW + 2) - 1, gx = x0 + pix % (kHaloTW + 2) - 1;
     st.a[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < kHaloPix * 4 && v * 8 < cn && gy >= 0 && gy < H && gx >= 0 && gx < W)
-      st.a[k] = *(const u16x8*)(in + (((size_t)n * H + gy) * W + gx) * Cin + ci0 + v * 8);
+    if (i < kHaloPix * 4 && v * 8 < cn && gy >= 0 && gy < A.H && gx >= 0 && gx < A.W)
+      st.a[k] = haloPixel<kSrc>(A, n, gy, gx, ci0 + v * 8);
   }
   if (!weights) return;
 #pragma unroll
@@ -258,7 +309,7 @@ __device__ __forceinline__ void haloLoad(HaloStage<kNT>& st, const uint16_t* __r
     const int t = q >> 2, v = q & 3;
     st.b[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
     if (i < kNT * 32 * kHaloRowVecs && v * 8 < cn)
-      st.b[k] = *(const u16x8*)(w + ((size_t)(co0 + r) * 9 + t) * Cin + ci0 + v * 8);
+      st.b[k] = *(const u16x8*)(A.w + ((size_t)(co0 + r) * 9 + t) * A.Cin + ci0 + v * 8);
   }
 }
 
@@ -279,21 +330,21 @@ __device__ __forceinline__ void haloStore(const HaloStage<kNT>& st, u16x8* sA, u
   }
 }
 
-// Cout % (32 * kNT) == 0; Cin % 16 == 0;
+// Cout % (32 * kNT) == 0; Cin % 16 == 0 (HS_UPCAT: ca % 32 == 0);
 // gridDim.x <= numItems = N x tilesY x tilesX x Cout / (32 * kNT)
-template <bool kOutF32, int kNT>
-__global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
-                                                   const float* __restrict__ bias, void* __restrict__ outV, int H,
-                                                   int W, int Cin, int Cout, int numItems, int relu) {
+template <int kOut, int kNT, int kSrc>
+__global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(HaloArgs A) {
+  const int H = A.H, W = A.W, Cin = A.Cin, Cout = A.Cout;
   const int coBlocks = Cout / (32 * kNT);
   __shared__ u16x8 sA[kHaloPix * 4];
   __shared__ u16x8 sB[kNT * 32 * kHaloRowVecs];
+  __shared__ float sF[kOut == HO_FINAL ? 4 * 96 : 1];  // HO_FINAL: the output channels 0..2 per wave
   const int tilesX = (W + kHaloTW - 1) / kHaloTW, tilesY = (H + kHaloTH - 1) / kHaloTH;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 31, half = lane >> 5;
   const int py = 2 * wave + (row >> 4), px = row & 15;  // this lane's A row (tile coordinates)
   const int nc = (Cin + kHaloCi - 1) / kHaloCi;
-  const int myItems = (numItems - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int myItems = (A.numItems - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
   const int steps = myItems * nc;
   const bool fixedW = nc == 1 && coBlocks == 1;  // the same weights for every step
   // (image, tile origin, first output channel) of step s
@@ -310,7 +361,7 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
   {
     int n, y0, x0, co0;
     origin(0, n, y0, x0, co0);
-    haloLoad<kNT>(st, in, w, n, y0, x0, co0, 0, min(kHaloCi, Cin), H, W, Cin, true);
+    haloLoad<kNT, kSrc>(st, A, n, y0, x0, co0, 0, min(kHaloCi, Cin), true);
   }
   f32x16 acc[kNT];
 #pragma unroll
@@ -324,7 +375,7 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
       int n1, y1, x1, co1;
       origin(s + 1, n1, y1, x1, co1);
       const int c1 = (s + 1) % nc;
-      haloLoad<kNT>(st, in, w, n1, y1, x1, co1, c1 * kHaloCi, min(kHaloCi, Cin - c1 * kHaloCi), H, W, Cin, !fixedW);
+      haloLoad<kNT, kSrc>(st, A, n1, y1, x1, co1, c1 * kHaloCi, min(kHaloCi, Cin - c1 * kHaloCi), !fixedW);
     }
 #pragma unroll 3
     for (int t = 0; t < 9; t++) {
@@ -347,17 +398,62 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
 #pragma unroll
       for (int j = 0; j < kNT; j++) {
         const int co = co0 + j * kTile + row;  // D column = lane % 32
-        const float bv = bias ? bias[co] : 0.0f;
+        const float bv = A.bias ? A.bias[co] : 0.0f;
+        if (kOut == HO_FINAL) {
+          // channels 0..2 of the wave's 32 pixels go through LDS so that all
+          // 64 lanes share the 96 log1p evaluations and stores
+          if (co < 3) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-          const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
-          const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
-          if (oy < H && ox < W) {
-            const size_t pp = ((size_t)n * H + oy) * W + ox;
+            for (int r = 0; r < 16; r++) {
+              const int i = 8 * (r / 4) + 4 * half + (r % 4);
+              float v = acc[j][r] + bv;
+              if (A.relu) v = fmaxf(v, 0.0f);
+              sF[wave * 96 + i * 3 + co] = v;
+            }
+          }
+          __syncthreads();
+          for (int e = lane; e < 96; e += 64) {
+            const int i = e / 3, ch = e - (e / 3) * 3;
+            const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
+            if (oy < H && ox < W) {
+              const size_t pp = ((size_t)n * H + oy) * W + ox;
+              ((float*)A.out)[pp * 3 + ch] = sF[wave * 96 + e] + log1pf(fmaxf(A.raw[pp * 9 + ch], 0.0f));
+            }
+          }
+        } else if (kOut == HO_BF16) {
+          // through LDS (the halo buffer, free once every wave is past this
+          // step's MFMAs) so that each lane stores 16-byte vectors: the
+          // wave's 32 pixels x 32 channels are 128 vectors, 2 per lane,
+          // instead of 16 two-byte stores per lane
+          __syncthreads();
+          uint16_t* tb = reinterpret_cast<uint16_t*>(sA) + wave * (32 * 32);
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
             float v = acc[j][r] + bv;
-            if (relu) v = fmaxf(v, 0.0f);
-            if (kOutF32) ((float*)outV)[pp * Cout + co] = v;
-            else ((uint16_t*)outV)[pp * Cout + co] = f2bf(v);
+            if (A.relu) v = fmaxf(v, 0.0f);
+            tb[i * 32 + row] = f2bf(v);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            const int e = lane + 64 * k, i = e >> 2, v = e & 3;
+            const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
+            if (oy < H && ox < W)
+              *(u16x8*)((uint16_t*)A.out + (((size_t)n * H + oy) * W + ox) * Cout + co0 + j * kTile + v * 8) =
+                  reinterpret_cast<const u16x8*>(tb)[e];
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int i = 8 * (r / 4) + 4 * half + (r % 4);  // D row: this wave's i-th pixel
+            const int oy = y0 + 2 * wave + (i >> 4), ox = x0 + (i & 15);
+            if (oy < H && ox < W) {
+              const size_t pp = ((size_t)n * H + oy) * W + ox;
+              float v = acc[j][r] + bv;
+              if (A.relu) v = fmaxf(v, 0.0f);
+              ((float*)A.out)[pp * Cout + co] = v;
+            }
           }
         }
         acc[j] = f32x16{};
@@ -366,25 +462,36 @@ __global__ __launch_bounds__(256, kNT == 1 ? 4 : 3) void k_conv_halo(const uint1
   }
 }
 
-// Blocks of k_conv_halo<outF32, nt> resident on the current device at once
-// (the persistent grid size), cached per device.
-static int haloResident(int nt, bool outF32) {
-  static int cache[16][4] = {};
+template <int kOut, int kNT, int kSrc>
+static int haloOccupancy() {
+  int per = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<kOut, kNT, kSrc>, 256, 0));
+  return per;
+}
+
+// Blocks of k_conv_halo<kOut, kNT, kSrc> resident on the current device at
+// once (the persistent grid size), cached per device (devices 0..63; others
+// are queried on every call).
+template <int kOut, int kNT, int kSrc>
+static int haloResident() {
+  static std::atomic<int> cache[64];
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
-  int& slot = cache[dev & 15][(nt - 1) * 2 + (outF32 ? 1 : 0)];
-  if (slot == 0) {
-    int cus = 0, per = 0;
-    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (outF32)
-      HIP_OK(nt == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<true, 1>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<true, 2>, 256, 0));
-    else
-      HIP_OK(nt == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<false, 1>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv_halo<false, 2>, 256, 0));
-    slot = std::max(1, cus * std::max(1, per));
+  if (dev >= 0 && dev < 64) {
+    const int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
   }
-  return slot;
+  int cus = 0;
+  HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int v = std::max(1, cus * std::max(1, haloOccupancy<kOut, kNT, kSrc>()));
+  if (dev >= 0 && dev < 64) cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
+template <int kOut, int kNT, int kSrc>
+static void launchHalo(const HaloArgs& A, hipStream_t s) {
+  const int g = std::min(A.numItems, haloResident<kOut, kNT, kSrc>());
+  hipLaunchKernelGGL((k_conv_halo<kOut, kNT, kSrc>), dim3(g), dim3(256), 0, s, A);
 }
 
 // Output-channel tiles per wave for a convolution with `tiles` 32-wide tiles:
@@ -829,67 +936,93 @@ struct sundog_denoiser {
     lossPartial.alloc(blocks(P0 * 32) * 4);
   }
 
-  // forward convolution k on a BF16 input: the spatially tiled kernel with
-  // persistent blocks, two output tiles per item where the channel count
-  // allows, else one
-  void conv(int k, const uint16_t* in, void* out, bool outF32, int h, int w) {
+  // forward convolution k: the spatially tiled kernel with persistent
+  // blocks, two output tiles per item where the channel count allows, else
+  // one. src / in2 / ca describe the input (HaloSrc), kOut the output.
+  void conv(int k, const uint16_t* in, void* out, int outKind, int h, int w, int src = HS_PLAIN,
+            const uint16_t* in2 = nullptr, int ca = 0) {
     const Conv& c = net.convs[k];
-    const float* b = net.params.as<float>() + c.bOff;
+    HaloArgs A;
+    A.in = in;
+    A.in2 = in2;
+    A.w = c.wf.as<uint16_t>();
+    A.bias = net.params.as<float>() + c.bOff;
+    A.out = out;
+    A.raw = devIn.as<float>();
+    A.H = h;
+    A.W = w;
+    A.Cin = c.cin;
+    A.Cout = c.cout;
+    A.ca = ca;
+    A.relu = c.relu ? 1 : 0;
     const int ht = (c.cout / 32) % 2 == 0 ? 2 : 1;
-    const int numItems = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH) * (c.cout / (32 * ht));
-    const uint16_t* wf = c.wf.as<uint16_t>();
-    const int rl = c.relu ? 1 : 0;
-    const int g = std::min(numItems, haloResident(ht, outF32));
-    if (outF32) {
-      if (ht == 1)
-        hipLaunchKernelGGL((k_conv_halo<true, 1>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
-                           c.cout, numItems, rl);
-      else
-        hipLaunchKernelGGL((k_conv_halo<true, 2>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
-                           c.cout, numItems, rl);
-    } else {
-      if (ht == 1)
-        hipLaunchKernelGGL((k_conv_halo<false, 1>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
-                           c.cout, numItems, rl);
-      else
-        hipLaunchKernelGGL((k_conv_halo<false, 2>), dim3(g), dim3(256), 0, stream, in, wf, b, out, h, w, c.cin,
-                           c.cout, numItems, rl);
-    }
-    HIP_OK(hipGetLastError());
+    A.numItems = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH) * (c.cout / (32 * ht));
+    if (src == HS_UPCAT && (ca % kHaloCi || (c.cin - ca) % 8))
+      throw std::runtime_error("denoiser: concatenated input needs 32-channel-aligned sources");
+#define SD_HALO(O, T, S)                                       \
+  if (outKind == O && ht == T && src == S) {                   \
+    launchHalo<O, T, S>(A, stream);                            \
+    HIP_OK(hipGetLastError());                                 \
+    return;                                                    \
+  }
+    SD_HALO(HO_BF16, 1, HS_PLAIN) SD_HALO(HO_BF16, 2, HS_PLAIN)
+    SD_HALO(HO_F32, 1, HS_PLAIN) SD_HALO(HO_F32, 2, HS_PLAIN)
+    SD_HALO(HO_BF16, 1, HS_POOL) SD_HALO(HO_BF16, 2, HS_POOL)
+    SD_HALO(HO_BF16, 1, HS_UPCAT) SD_HALO(HO_BF16, 2, HS_UPCAT)
+    SD_HALO(HO_FINAL, 1, HS_PLAIN)
+#undef SD_HALO
+    throw std::runtime_error("denoiser: unsupported convolution variant");
   }
 
-  // forward; in = packed BF16 input (already in `input`)
-  void forward() {
+  // forward; in = packed BF16 input (already in `input`). Training keeps
+  // the pooled and concatenated layer inputs and the network output `pred`
+  // (its backward pass reads them); inference (`infer`) has the
+  // convolutions read the pooled and concatenated inputs from their sources
+  // and the output layer write the final log-space image directly. Both
+  // perform the same floating-point operations, so `fin` is bit-identical.
+  void forward(bool infer = false) {
     const uint16_t* cur = input.as<uint16_t>();
     int k = 0;
     for (int l = 0; l < 5; l++) {
       int h = H >> l, w = W >> l;
-      if (l > 0) {
-        size_t tot = (size_t)N * h * w * kCh[l - 1];
-        hipLaunchKernelGGL(k_pool_fwd, dim3(blocks(tot)), dim3(256), 0, stream, y[k - 1].as<uint16_t>(),
-                           pooled[l - 1].as<uint16_t>(), N, h * 2, w * 2, kCh[l - 1]);
-        cur = pooled[l - 1].as<uint16_t>();
+      if (l > 0 && infer) {
+        conv(k, y[k - 1].as<uint16_t>(), y[k].p, HO_BF16, h, w, HS_POOL);
+      } else {
+        if (l > 0) {
+          size_t tot = (size_t)N * h * w * kCh[l - 1];
+          hipLaunchKernelGGL(k_pool_fwd, dim3(blocks(tot)), dim3(256), 0, stream, y[k - 1].as<uint16_t>(),
+                             pooled[l - 1].as<uint16_t>(), N, h * 2, w * 2, kCh[l - 1]);
+          cur = pooled[l - 1].as<uint16_t>();
+        }
+        conv(k, cur, y[k].p, HO_BF16, h, w);
       }
-      conv(k, cur, y[k].p, false, h, w);
-      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, false, h, w);
+      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, HO_BF16, h, w);
       cur = y[k + 1].as<uint16_t>();
       k += 2;
     }
     for (int l = 3; l >= 0; l--) {
       int h = H >> l, w = W >> l;
       int skipConv = 2 * l + 1;
-      size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]) / 8;  // 16-byte vectors
-      hipLaunchKernelGGL(k_upcat_fwd, dim3(blocks(tot)), dim3(256), 0, stream, cur, y[skipConv].as<uint16_t>(),
-                         cat[l].as<uint16_t>(), N, h, w, kCh[l + 1], kCh[l]);
-      conv(k, cat[l].as<uint16_t>(), y[k].p, false, h, w);
-      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, false, h, w);
+      if (infer) {
+        conv(k, cur, y[k].p, HO_BF16, h, w, HS_UPCAT, y[skipConv].as<uint16_t>(), kCh[l + 1]);
+      } else {
+        size_t tot = (size_t)N * h * w * (kCh[l + 1] + kCh[l]) / 8;  // 16-byte vectors
+        hipLaunchKernelGGL(k_upcat_fwd, dim3(blocks(tot)), dim3(256), 0, stream, cur, y[skipConv].as<uint16_t>(),
+                           cat[l].as<uint16_t>(), N, h, w, kCh[l + 1], kCh[l]);
+        conv(k, cat[l].as<uint16_t>(), y[k].p, HO_BF16, h, w);
+      }
+      conv(k + 1, y[k].as<uint16_t>(), y[k + 1].p, HO_BF16, h, w);
       cur = y[k + 1].as<uint16_t>();
       k += 2;
     }
-    conv(k, cur, pred.p, true, H, W);
-    size_t P = (size_t)N * H * W;
-    hipLaunchKernelGGL(k_final, dim3(blocks(P * 3)), dim3(256), 0, stream, pred.as<float>(), devIn.as<float>(),
-                       fin.as<float>(), P);
+    if (infer) {
+      conv(k, cur, fin.p, HO_FINAL, H, W);
+    } else {
+      conv(k, cur, pred.p, HO_F32, H, W);
+      size_t P = (size_t)N * H * W;
+      hipLaunchKernelGGL(k_final, dim3(blocks(P * 3)), dim3(256), 0, stream, pred.as<float>(), devIn.as<float>(),
+                         fin.as<float>(), P);
+    }
     HIP_OK(hipGetLastError());
   }
 
@@ -1075,7 +1208,7 @@ double denoiseFilms(const char* weightsPath, const float4* beauty, const float4*
                      d->devIn.as<float>());
   hipLaunchKernelGGL(k_pack_input, dim3(blocks(P * 16)), dim3(256), 0, d->stream, d->devIn.as<float>(),
                      d->input.as<uint16_t>(), P);
-  d->forward();
+  d->forward(true);
   DevBuf o;
   o.alloc((size_t)w * h * 16);
   hipLaunchKernelGGL(k_unpack_films, dim3(blocks((size_t)w * h)), dim3(256), 0, d->stream, d->fin.as<float>(), w, h,
@@ -1163,7 +1296,7 @@ extern "C" int sundog_denoiser_infer(sundog_denoiser* d, const float* input, int
   return guardedD([&] {
     if (!d || !input || !out) throw std::runtime_error("denoiser: null argument");
     d->uploadBatch(input, nullptr, n, h, w);
-    d->forward();
+    d->forward(true);
     size_t P = (size_t)n * h * w;
     DevBuf o;
     o.alloc(P * 3 * 4);
@@ -1185,12 +1318,12 @@ extern "C" int sundog_denoiser_time_forward(sundog_denoiser* d, int n, int h, in
     d->resize(n, h, w);
     HIP_OK(hipMemsetAsync(d->input.p, 0, (size_t)n * h * w * 16 * 2, d->stream));
     HIP_OK(hipMemsetAsync(d->devIn.p, 0, (size_t)n * h * w * 9 * 4, d->stream));
-    d->forward();
+    d->forward(true);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, d->stream));
-    for (int i = 0; i < iters; i++) d->forward();
+    for (int i = 0; i < iters; i++) d->forward(true);
     HIP_OK(hipEventRecord(e1, d->stream));
     HIP_OK(hipEventSynchronize(e1));
     float ms = 0;

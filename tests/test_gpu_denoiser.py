@@ -68,6 +68,23 @@ def test_forward_backward_matches_oracle(shape):
         assert ew < 5e-2 and eb < 5e-2, (k, ew, eb)
 
 
+def test_inference_forward_equals_training_forward():
+    """The inference forward reads the pooled and concatenated layer inputs
+    from their sources and writes the final image from the output layer;
+    the training forward materialises them. Same operations, so the final
+    log-space image is the same (compared through expm1, which the inference
+    entry point applies on the GPU)."""
+    den = denoise.Denoiser(seed=7)
+    rng = np.random.default_rng(11)
+    for n, h, w in [(1, 48, 80), (2, 32, 32)]:
+        x, y = _batch(rng, n, h, w)
+        _, _, fin = den.forward_backward(x, y, want_grads=False)
+        out = den.infer(x)
+        ref = np.expm1(fin.astype(np.float64))
+        err = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-6)
+        assert err.max() < 2e-6, float(err.max())
+
+
 def test_deterministic_and_params_unchanged():
     den = denoise.Denoiser(seed=5)
     p0 = den.get_params()

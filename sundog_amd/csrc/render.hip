@@ -235,9 +235,24 @@ DEV void flameField(const DevFlame& fl, vec3 p, float& sigma, vec3& emission) {
   float rx = p.x - fl.base[0], rz = p.z - fl.base[2];
   float d = sqrtf(rx * rx + rz * rz) / fl.radius;
   float prof = 3.2f * sqrtf(h) * powf(1.0f - h, 1.55f);
+  const float a = d / fmaxf(prof, 1e-3f);
+  // The noise is a sum of non-negative octaves, and adding a non-negative
+  // term never lowers a float sum, so every partial sum bounds dd from
+  // below: once a + (partial - 0.45) >= 1.01, dd >= 1 and the mask below is
+  // 0 whatever the remaining octaves (0.01 of margin for rounding). The
+  // remaining octaves are skipped there; the result is the same zero field.
+  if (a >= 1.46f) return;
   vec3 q = vec3{rx / fl.radius, (p.y - fl.base[1]) / fl.radius * 0.32f, rz / fl.radius};
-  float n = fbm(q * fl.noiseScale, fl.seed, fl.octaves);
-  float dd = d / fmaxf(prof, 1e-3f) + (n - 0.45f) * 1.0f;
+  // fbm(q * noiseScale, seed, octaves), the same operations in the same order
+  vec3 pn = q * fl.noiseScale;
+  float amp = 0.5f, n = 0.0f;
+  for (int i = 0; i < fl.octaves; i++) {
+    n += amp * vnoise(pn, fl.seed + (uint32_t)i * 101u);
+    if (a + (n - 0.45f) >= 1.01f) return;
+    pn = 2.03f * pn + vec3{11.7f, 5.3f, 7.9f};
+    amp *= 0.5f;
+  }
+  float dd = a + (n - 0.45f) * 1.0f;
   float mask = smoothstepf(1.0f, 0.62f, dd) * smoothstepf(0.0f, 0.12f, h);
   if (mask <= 0.0f) return;
   sigma = fl.sigma * mask;

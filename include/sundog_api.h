@@ -273,6 +273,12 @@ SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,
  * marginal row and conditional column picked by the binary searches. */
 SUNDOG_API int sundog_session_env_bins(sundog_session*, const float* u, int n,
                                        int32_t* row_out, int32_t* col_out);
+/* Flame density field on device: extinction sigma (n floats) and emission
+ * (n x 3 floats) of flame `flame` at n points (x, y, z), the values the
+ * flame march integrates (device/volume.cuh:27-153). */
+SUNDOG_API int sundog_session_flame_field(sundog_session*, int flame, const float* pts,
+                                          int n, float* sigma_out,
+                                          float* emission_out);
 
 /* ---- serving: resident scenes + continuous batching ------------------------
  * A server keeps every added scene resident (BVHs, tables, textures) and

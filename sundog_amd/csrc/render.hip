@@ -2554,6 +2554,18 @@ __global__ void k_env_bins(LaunchParams P, const float* u, int n, int* rowOut, i
   colOut[i] = col;
 }
 
+__global__ void k_flame_field(LaunchParams P, int flame, const float* pts, int n, float* sigmaOut, float* emOut) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float sg;
+  vec3 em;
+  flameField(P.flames[flame], vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, sg, em);
+  sigmaOut[i] = sg;
+  emOut[3 * i] = em.x;
+  emOut[3 * i + 1] = em.y;
+  emOut[3 * i + 2] = em.z;
+}
+
 __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   Pcg r = Pcg::make(seed, seq);
@@ -2600,6 +2612,12 @@ hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* 
 
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s) {
   hipLaunchKernelGGL(k_env_bins, dim3((n + 255) / 256), dim3(256), 0, s, P, u, n, row, col);
+  return hipGetLastError();
+}
+
+hipError_t launchFlameField(const LaunchParams& P, int flame, const float* pts, int n, float* sigma, float* em,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_flame_field, dim3((n + 255) / 256), dim3(256), 0, s, P, flame, pts, n, sigma, em);
   return hipGetLastError();
 }
 

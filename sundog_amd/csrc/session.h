@@ -28,6 +28,8 @@ hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool bru
                        int* inst, int* prim, int* counters, hipStream_t s);
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
+hipError_t launchFlameField(const LaunchParams& P, int flame, const float* pts, int n, float* sigma, float* em,
+                            hipStream_t s);
 hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s);
 hipError_t launchQueueReset(QueueState* q, int n, hipStream_t s);
 hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* firstSample, int numReq,

@@ -288,6 +288,25 @@ extern "C" int sundog_session_env_bins(sundog_session* s, const float* u, int n,
   });
 }
 
+extern "C" int sundog_session_flame_field(sundog_session* s, int flame, const float* pts, int n, float* sigma_out,
+                                          float* emission_out) {
+  return guarded([&] {
+    if (!s || !pts || !sigma_out || !emission_out) throw std::runtime_error("flame field: null argument");
+    if (flame < 0 || flame >= s->P.numFlames) throw std::runtime_error("flame field: flame id out of range");
+    if (n <= 0) return SUNDOG_OK;
+    DevBuf dp, ds, de;
+    dp.alloc((size_t)n * 12);
+    ds.alloc((size_t)n * 4);
+    de.alloc((size_t)n * 12);
+    HIP_OK(hipMemcpyAsync(dp.p, pts, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_OK(launchFlameField(s->P, flame, dp.as<float>(), n, ds.as<float>(), de.as<float>(), s->stream));
+    HIP_OK(hipMemcpyAsync(sigma_out, ds.p, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(emission_out, de.p, (size_t)n * 12, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return SUNDOG_OK;
+  });
+}
+
 extern "C" int sundog_device_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
   return guarded([&] {
     if (!out || n < 0) throw std::runtime_error("pcg32: bad arguments");

@@ -571,6 +571,17 @@ class Session(object):
         _lib.check(self.lib.sundog_session_shadow(self.ptr, _fptr(rays), n, _fptr(tr)), self.lib)
         return tr
 
+    def flame_field(self, flame, pts):
+        """(sigma (n,), emission (n, 3)) of flame `flame` at (n, 3) points,
+        evaluated on the device."""
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        sg = np.zeros(n, np.float32)
+        em = np.zeros((n, 3), np.float32)
+        _lib.check(self.lib.sundog_session_flame_field(self.ptr, int(flame), _fptr(pts), n, _fptr(sg), _fptr(em)),
+                   self.lib)
+        return sg, em
+
     def env_bins(self, u):
         u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
         n = u.shape[0]

@@ -93,6 +93,32 @@ def test_shadow_transmittance_matches_oracle(orc, asset_dir):
     np.testing.assert_allclose(tr_g, tr_c, rtol=2e-5, atol=1e-6)
 
 
+def test_flame_field_matches_oracle(orc, asset_dir):
+    """The device flame field stops evaluating noise octaves once the density
+    mask is provably zero; the oracle evaluates every octave. On a dense grid
+    over the flame's box plus random points: wherever the device returns a
+    zero field the oracle's is zero too (up to a value rounding can leave at
+    the mask's edge), and elsewhere sigma and emission agree to FP32
+    contraction differences."""
+    sc = configs.volumes(asset_dir, width=32, height=18, spp=1, env_w=64, env_h=32).build()
+    sess = sc.session()
+    flat = sc.flatten()
+    # the volumes flame: base (-1.8, 0.06, -1.2), height 2, radius 0.5, box bounds
+    gx, gy, gz = np.meshgrid(np.linspace(-2.35, -1.25, 45), np.linspace(0.0, 2.1, 85),
+                             np.linspace(-1.75, -0.65, 45), indexing="ij")
+    grid = np.stack([gx, gy, gz], -1).reshape(-1, 3)
+    rnd = np.random.default_rng(5).uniform([-2.3, 0.06, -1.7], [-1.3, 2.06, -0.7], (50000, 3))
+    p = np.concatenate([grid, rnd]).astype(np.float32)
+    sg_g, em_g = sess.flame_field(0, p)
+    sg_c, em_c = orc.flame_field(flat, 0, p)
+    zero_g = sg_g == 0
+    assert 0.2 < zero_g.mean() < 0.95, zero_g.mean()        # both regimes sampled
+    assert np.all(sg_c[zero_g] <= 1e-5 * sg_c.max()), float(sg_c[zero_g].max())
+    assert np.all(em_c[zero_g] <= 1e-5 * em_c.max())
+    np.testing.assert_allclose(sg_g, sg_c, rtol=1e-4, atol=1e-5 * sg_c.max())
+    np.testing.assert_allclose(em_g, em_c, rtol=1e-4, atol=1e-5 * em_c.max())
+
+
 def test_env_bins_bit_exact(orc, asset_dir):
     sc = configs.volumes(asset_dir, width=32, height=18, spp=1, env_w=256, env_h=128).build()
     sess = sc.session()

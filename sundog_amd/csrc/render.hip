@@ -1890,16 +1890,23 @@ DEV void flushCounters(bool countRays, unsigned long long* rayCounter, unsigned 
 // bit-identical to per-pixel rendering. Launches of more than kPoolSamples
 // samples drain one pool per kPoolSamples. slotPixel(j, px, py) names the
 // pixel of slot j (0..63) and returns false for a slot outside the image.
+// With fold == false the wave renders only the launch's samples
+// [gBegin, gEnd) (at most kPoolSamples) into slots gBegin * 64 + item of its
+// region and leaves the fold to k_fold_samples, which runs once every
+// sample group of the launch has written its slots.
 template <class SlotPixel>
 DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* poolNext, float* waveBuf, Stack& st,
-                          WorkCount& wc, uint32_t& rays) {
+                          WorkCount& wc, uint32_t& rays, int gBegin, int gEnd, bool fold) {
   const int lane = threadIdx.x & 63;
   int myPx = 0, myPy = 0;
   const bool mine = slotPixel(lane, myPx, myPy);
-  for (int c0 = 0; c0 < P.sppThisLaunch; c0 += kPoolSamples) {
-    const int n = min(kPoolSamples, P.sppThisLaunch - c0);
+  for (int c0 = gBegin; c0 < gEnd; c0 += kPoolSamples) {
+    const int n = min(kPoolSamples, gEnd - c0);
     const int items = n * 64;  // item = sample-in-pool * 64 + slot
     if (lane == 0) *poolNext = 64;  // item `lane` is each lane's first
+    // this pool's slots: from 0 when the wave folds each pool itself, else
+    // at the group's first sample so that groups do not overlap
+    float* poolBuf = waveBuf + (fold ? 0 : c0 * 64);
     int item = lane;
     PathState ps;
     ps.si = 0;
@@ -1924,7 +1931,7 @@ DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* 
       rays++;
       SurfState sf;
       // first-hit AOVs go straight to the sample's slot
-      int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf, waveBuf + 3 * (kPoolSamples * 64) + item, true);
+      int r = pathOnHit(P, ps, isHit, hit, oObj, dObj, sf, poolBuf + 3 * (kPoolSamples * 64) + item, true);
       // The NEE sample's unshadowed contribution is computed before its
       // shadow query. Unless the flame march of a transparent shadow ray
       // draws random numbers after the query (which the BSDF sample's
@@ -1973,7 +1980,7 @@ DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* 
         const vec3 L = sanitize(ps.L);
         const float v[3] = {L.x, L.y, L.z};
 #pragma unroll
-        for (int k = 0; k < 3; k++) waveBuf[k * (kPoolSamples * 64) + item] = v[k];
+        for (int k = 0; k < 3; k++) poolBuf[k * (kPoolSamples * 64) + item] = v[k];
         item = (int)atomicAdd(poolNext, 1u);
         alive = claim();
       }
@@ -1981,6 +1988,7 @@ DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* 
     // every lane of the wave is past its last path: make the wave's
     // per-sample stores visible to the lane that folds each pixel
     __threadfence_block();
+    if (!fold) continue;
     // (the film is only touched here, so no film value is live while paths
     // are traced)
     if (mine) {
@@ -2014,12 +2022,19 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
   WorkCount wc;
   Stack st;
   st.lds = ldsColumn(s_stack);
-  st.spill = reinterpret_cast<uint64_t*>(P.spill) + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+  st.spill = reinterpret_cast<uint64_t*>(P.spill) +
+             ((size_t)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
   st.stride = P.spillStride;
   st.sp = 0;
   st.cap = P.stackLds;
   const int wave = threadIdx.x >> 6;
   const size_t waveGlobal = (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kBlock / 64) + wave;
+  // sample group blockIdx.z of gridDim.z: a contiguous share of the launch's
+  // samples (one group: all of them, folded here)
+  const int groups = (int)gridDim.z;
+  const int per = (P.sppThisLaunch + groups - 1) / groups;
+  const int gBegin = min(P.sppThisLaunch, (int)blockIdx.z * per);
+  const int gEnd = min(P.sppThisLaunch, gBegin + per);
   // 16x16-pixel blocks; each wave covers an 8x8 quarter, slot j at (j % 8, j / 8)
   const int x0 = blockIdx.x * 16 + (wave & 1) * 8, y0 = blockIdx.y * 16 + (wave >> 1) * 8;
   auto slotPixel = [&](int j, int& px, int& py) -> bool {
@@ -2028,8 +2043,34 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_render(LaunchParams P) {
     return px < P.width && py < P.height;
   };
   renderWavePooled(P, slotPixel, &s_poolNext[wave], P.sampleBuf + waveGlobal * (kPoolPlanes * kPoolSamples * 64), st,
-                   wc, rays);
+                   wc, rays, gBegin, gEnd, groups == 1);
   flushCounters<kCountWork>(P.countRays != 0, P.rayCounter, P.workCounters, rays, wc);
+}
+
+// The fold of a launch rendered in several sample groups: each pixel's
+// samples, in sample order, into its running means (the same operations as
+// the in-kernel fold of renderWavePooled).
+__global__ void k_fold_samples(LaunchParams P) {
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= P.width || py >= P.height) return;
+  const int tx = px & 15, ty = py & 15;
+  const int wave = (ty >> 3) * 2 + (tx >> 3), lane = (ty & 7) * 8 + (tx & 7);
+  const size_t waveGlobal = (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kBlock / 64) + wave;
+  const float* buf = P.sampleBuf + waveGlobal * (kPoolPlanes * kPoolSamples * 64);
+  const int pixel = py * P.width + px;
+  float4 fAcc = P.accum[pixel], fAlb = P.albedo[pixel], fNrm = P.normal[pixel];
+  for (int si = 0; si < P.sppThisLaunch; si++) {
+    float x[kPoolPlanes];
+#pragma unroll
+    for (int k = 0; k < kPoolPlanes; k++) x[k] = buf[k * (kPoolSamples * 64) + si * 64 + lane];
+    const int s = P.sampleOffset + si;
+    fAcc = runningMean(fAcc, vec3{x[0], x[1], x[2]}, s);
+    fAlb = runningMean(fAlb, vec3{x[3], x[4], x[5]}, s);
+    fNrm = runningMean(fNrm, vec3{x[6], x[7], x[8]}, s);
+  }
+  P.accum[pixel] = fAcc;
+  P.albedo[pixel] = fAlb;
+  P.normal[pixel] = fNrm;
 }
 
 // ---------------------------------------------------------------------------
@@ -2125,7 +2166,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
         py = ty + (k / tileSize);
         return k < pxPerTile && px < P.width && py < P.height;
       };
-      renderWavePooled(P, slotPixel, &s_poolNext[wave], waveBuf, st, wc, rays);
+      renderWavePooled(P, slotPixel, &s_poolNext[wave], waveBuf, st, wc, rays, 0, P.sppThisLaunch, true);
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
@@ -2576,11 +2617,16 @@ __global__ void k_pcg(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
 // host-side launch helpers (called from session.hip)
 // ---------------------------------------------------------------------------
 
-hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s) {
-  dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+// groups > 1 splits the launch's samples (at most kPoolSamples) over that
+// many blocks per tile, then folds them with k_fold_samples: small frames
+// fill the GPU. The spill region must hold groups x the grid's threads.
+hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, int groups, hipStream_t s) {
+  dim3 grid((P.width + 15) / 16, (P.height + 15) / 16, groups);
   const size_t lds = (size_t)P.stackLds * kBlock * sizeof(uint64_t);
   // every wave of the grid owns one per-sample result region
   if (!P.sampleBuf || P.sampleBufWaves < (int64_t)grid.x * grid.y * (kBlock / 64)) return hipErrorInvalidValue;
+  if (groups < 1 || groups > kPoolSamples || (groups > 1 && P.sppThisLaunch > kPoolSamples))
+    return hipErrorInvalidValue;
   if (countWork)
     hipLaunchKernelGGL((k_render<true, 2>), grid, dim3(kBlock), lds, s, P);
   else if (waves >= 5)
@@ -2591,6 +2637,11 @@ hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStr
     hipLaunchKernelGGL((k_render<false, 3>), grid, dim3(kBlock), lds, s, P);
   else
     hipLaunchKernelGGL((k_render<false, 2>), grid, dim3(kBlock), lds, s, P);
+  if (groups > 1) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fold_samples, dim3(grid.x, grid.y), dim3(kBlock), 0, s, P);
+  }
   return hipGetLastError();
 }
 

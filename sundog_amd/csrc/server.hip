@@ -650,12 +650,16 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     HIP_OK(hipMemsetAsync(r.normal.p, 0, n, s->stream));
     LaunchParams P = s->requestParams(r);
     sundog_session& sc = *s->scenes[scene_id];
-    size_t threads = (size_t)((width + 15) / 16) * ((height + 15) / 16) * renderBlockThreads();
+    // the sample groups a session would use for these passes (one setting
+    // for the request: its passes are 16 samples except perhaps the last)
+    const size_t tiles = (size_t)((width + 15) / 16) * ((height + 15) / 16);
+    const int groups = sampleGroupsFor(tiles, std::min(spp, 16), s->numCUs);
+    size_t threads = tiles * groups * renderBlockThreads();
     DevBuf sp;
     sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * kStackEntryBytes));
     P.spill = sp.as<uint32_t>();
     P.spillStride = (int)threads;
-    const size_t waves = threads / 64;
+    const size_t waves = tiles * (renderBlockThreads() / 64);
     DevBuf sb;
     sb.alloc(waves * (size_t)kPoolPlanes * kPoolSamples * 64 * sizeof(float));
     P.sampleBuf = sb.as<float>();
@@ -669,7 +673,7 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
       bool trial = false;
       const int w = s->serialWaves != 0 ? s->serialWaves : pick.pick(trial);
       auto t0 = Clock::now();
-      HIP_OK(launchRender(P, false, w, s->stream));
+      HIP_OK(launchRender(P, false, w, P.sppThisLaunch <= kPoolSamples ? groups : 1, s->stream));
       HIP_OK(hipStreamSynchronize(s->stream));
       if (trial)
         pick.record(w, std::chrono::duration<double, std::milli>(Clock::now() - t0).count() /

@@ -23,7 +23,7 @@
 
 namespace sdk {
 
-hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, hipStream_t s);
+hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, int groups, hipStream_t s);
 hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool brute, int numInst, float* t,
                        int* inst, int* prim, int* counters, hipStream_t s);
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
@@ -71,6 +71,19 @@ constexpr size_t kStackEntryBytes = 8;  // traversal stack entry: node reference
 // more than 5% slower than the best so far, and later passes use the best.
 // Large frames favour many waves (latency hiding); small frames, whose grids
 // cannot fill the chip, favour the spill-free low-occupancy variants.
+// Sample groups for a megakernel launch of `blocks` 16x16 tiles and `count`
+// samples on a device of `cus` CUs: up to 4 groups, each rendering a share
+// of the samples in its own block per tile, until the grid has 4 blocks per
+// CU (a 256x256 frame has one tile per CU). More than one group needs the
+// launch's samples to fit one shared pool.
+inline int sampleGroupsFor(size_t blocks, int count, int cus) {
+  if (count > kPoolSamples || blocks == 0) return 1;
+  const size_t want = (size_t)std::max(cus, 1) * 4;
+  int g = 1;
+  while (g < 4 && g < count && blocks * (size_t)g < want) g++;
+  return g;
+}
+
 struct OccupancyPicker {
   static constexpr int kCandidates[4] = {5, 4, 3, 2};
   int rounds = 1;           // timed passes per candidate (the fastest counts)
@@ -326,6 +339,7 @@ struct sundog_session {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double renderMs = 0.0;
   double buildMs = 0.0;
+  int cuCount = 0;  // CUs of the session's device (queried at the first pass)
   uint64_t paths = 0;
   int tlasNodes = 0, blasNodes = 0;
   int bvhBuilder = 0;        // 0: host binned SAH, 1: GPU LBVH
@@ -835,14 +849,20 @@ struct sundog_session {
       return;
     }
     size_t gx = (P.width + 15) / 16, gy = (P.height + 15) / 16;
-    ensureSpill(gx * gy * renderBlockThreads());
+    if (cuCount == 0) {
+      int dev = 0;
+      HIP_OK(hipGetDevice(&dev));
+      HIP_OK(hipDeviceGetAttribute(&cuCount, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int groups = sampleGroupsFor(gx * gy, count, cuCount);
+    ensureSpill(gx * gy * groups * renderBlockThreads());
     ensureSampleBuf(gx * gy * (renderBlockThreads() / 64));
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
     bool trial = false;
     const int w = countWork ? 2 : pickWaves(trial);
     HIP_OK(hipEventRecord(ev0, stream));
-    HIP_OK(launchRender(P, countWork, w, stream));
+    HIP_OK(launchRender(P, countWork, w, groups, stream));
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0.0f;

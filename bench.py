@@ -80,6 +80,8 @@ def main():
                          "lets the session time the variants on real passes before the timed region")
     ap.add_argument("--stack", type=int, default=12,
                     help="traversal stack entries per lane kept in LDS (deeper entries spill to HBM)")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="megakernel sample groups per tile (0: the session's choice by frame size)")
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
                     help="minimum waves per SIMD of the wavefront query kernels")
@@ -128,6 +130,7 @@ def main():
     sess = scene.session()
     sess.tune("waves", args.waves)
     sess.tune("stack", args.stack)
+    sess.tune("groups", args.groups)
     sess.set_kernel(args.kernel)
     sess.tune("wf_waves", args.wf_waves)
     if args.bvh == "lbvh":
@@ -156,8 +159,9 @@ def main():
     # occupancy variant: fixed, or timed on real passes before the warm-up
     # (sundog_session_autotune; the film is cleared afterwards)
     waves = args.waves
-    if args.kernel == "megakernel" and waves == 0:
-        waves = sess.autotune(k)
+    if args.kernel == "megakernel" and (waves == 0 or args.groups == 0):
+        waves = sess.autotune(k)   # times what is not fixed: the occupancy variant, the sample groups
+    groups_used = sess.stats()["sample_groups"] if args.kernel == "megakernel" else 1
 
     # warmup, then the timed passes
     for i in range(args.warmup):
@@ -202,6 +206,8 @@ def main():
             "waves_per_simd": waves,
             "waves_chosen_by": "autotune" if args.waves == 0 and args.kernel == "megakernel" else "flag",
             "stack_lds": args.stack,
+            "sample_groups": groups_used,
+            "sample_groups_chosen_by": "flag" if args.groups else "autotune",
             "bvh": args.bvh,
         },
         "mrays_per_s": rays / wall / 1e6,

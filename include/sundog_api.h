@@ -220,6 +220,8 @@ typedef struct {
   double bvh_gpu_ms;        /* wall time of the last GPU LBVH rebuild */
   int32_t waves;            /* megakernel occupancy variant in use (0: auto mode still timing) */
   int32_t stack_lds;        /* LDS traversal-stack entries per lane */
+  int32_t sample_groups;    /* megakernel sample groups per tile in use (0: auto mode still timing) */
+  int32_t pad3;
 } sundog_session_stats;
 
 SUNDOG_API sundog_session* sundog_session_create(sundog_scene*,
@@ -236,13 +238,19 @@ SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
  * minimum waves per SIMD the megakernel's register budget allows; 0, the
  * default, picks one per scene by timing the first passes), "wf_waves" (4|5: the same for the wavefront query kernels), "stack"
  * (4..24: LDS traversal-stack entries per lane of the megakernel; default
- * 12), and
+ * 12), "groups" (1..16: blocks per tile sharing a pass's samples, which a
+ * separate kernel then folds in sample order; 0, the default, picks from
+ * 1, 2, 4, 8, 16 by timing passes after the occupancy variant is chosen;
+ * films are bit-identical for every choice), and
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Times the megakernel's occupancy variants on passes of `count` samples
  * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per
- * SIMD twice each, the faster pass counting), keeps the fastest for later
- * passes and clears the film. *waves_out receives the choice. */
+ * SIMD twice each, the faster pass counting), then, when `count` <= 16 and
+ * "groups" is 0, the sample groups 1, 2, 4, 8, 16 at that variant the same
+ * way; keeps the fastest for later passes and clears the film. A value
+ * fixed by "waves" or "groups" is kept and not timed. *waves_out receives
+ * the occupancy variant in use (the groups are in the stats). */
 SUNDOG_API int sundog_session_autotune(sundog_session*, int count, int* waves_out);
 /* Runs the instrumented kernel variant that counts traversal work. */
 SUNDOG_API int sundog_session_set_count_work(sundog_session*, int on);

@@ -57,7 +57,7 @@ class SessionStats(ctypes.Structure):
                 ("tlas_nodes", c_i32), ("blas_nodes", c_i32), ("pad", c_i32),
                 ("node_visits", c_u64), ("tri_tests", c_u64), ("quad_tests", c_u64),
                 ("inst_transforms", c_u64), ("bvh_builder", c_i32), ("pad2", c_i32), ("bvh_gpu_ms", c_d),
-                ("waves", c_i32), ("stack_lds", c_i32)]
+                ("waves", c_i32), ("stack_lds", c_i32), ("sample_groups", c_i32), ("pad3", c_i32)]
 
 
 class RequestInfo(ctypes.Structure):

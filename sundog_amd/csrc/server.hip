@@ -650,11 +650,9 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     HIP_OK(hipMemsetAsync(r.normal.p, 0, n, s->stream));
     LaunchParams P = s->requestParams(r);
     sundog_session& sc = *s->scenes[scene_id];
-    // the sample groups a session would use for these passes (one setting
-    // for the request: its passes are 16 samples except perhaps the last)
+    // spill room for the largest sample-group count a pass can use
     const size_t tiles = (size_t)((width + 15) / 16) * ((height + 15) / 16);
-    const int groups = sampleGroupsFor(tiles, std::min(spp, 16), s->numCUs);
-    size_t threads = tiles * groups * renderBlockThreads();
+    size_t threads = tiles * kPoolSamples * renderBlockThreads();
     DevBuf sp;
     sp.alloc(std::max<size_t>(16, (size_t)sc.spillRows * threads * kStackEntryBytes));
     P.spill = sp.as<uint32_t>();
@@ -665,19 +663,30 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     P.sampleBuf = sb.as<float>();
     P.sampleBufWaves = (int64_t)waves;
     HIP_OK(hipStreamSynchronize(s->stream));
-    OccupancyPicker pick;  // serialWaves 0: the per-request choice a session makes
+    // serialWaves 0: the per-request choices a session makes (occupancy
+    // variant, then sample groups, from timed passes)
+    OccupancyPicker pick;
+    GroupPicker gpick;
     auto t = Clock::now();
     for (int done = 0; done < spp; done += 16) {
       P.sampleOffset = done;
       P.sppThisLaunch = std::min(16, spp - done);
-      bool trial = false;
+      bool trial = false, gTrial = false;
       const int w = s->serialWaves != 0 ? s->serialWaves : pick.pick(trial);
+      int groups = 1;
+      if (P.sppThisLaunch <= kPoolSamples) {
+        if (trial || (s->serialWaves == 0 && pick.chosen == 0))
+          groups = sampleGroupsFor(tiles, P.sppThisLaunch, s->numCUs);
+        else
+          groups = gpick.pick(gTrial, P.sppThisLaunch);
+      }
       auto t0 = Clock::now();
-      HIP_OK(launchRender(P, false, w, P.sppThisLaunch <= kPoolSamples ? groups : 1, s->stream));
+      HIP_OK(launchRender(P, false, w, groups, s->stream));
       HIP_OK(hipStreamSynchronize(s->stream));
-      if (trial)
-        pick.record(w, std::chrono::duration<double, std::milli>(Clock::now() - t0).count() /
-                           ((double)width * height * P.sppThisLaunch));
+      const double msPerPath = std::chrono::duration<double, std::milli>(Clock::now() - t0).count() /
+                               ((double)width * height * P.sppThisLaunch);
+      if (trial) pick.record(w, msPerPath);
+      if (gTrial) gpick.record(groups, msPerPath);
     }
     double ms = std::chrono::duration<double, std::milli>(Clock::now() - t).count();
     if (ms_out) *ms_out = ms;

@@ -123,6 +123,49 @@ struct OccupancyPicker {
   }
 };
 
+// Sample groups per tile for the megakernel, chosen like the occupancy
+// variant from timed passes (after it, at the chosen variant): 1, 2, 4, 8,
+// 16 groups until one is more than 5% slower than the best so far. Frames
+// whose cost varies a lot between tiles (flames) want many groups: the
+// expensive tiles' samples spread over more blocks, so the grid does not end
+// on a few long blocks.
+struct GroupPicker {
+  static constexpr int kCandidates[5] = {1, 2, 4, 8, 16};
+  int rounds = 1;
+  bool exhaustive = false;
+  int chosen = 0, next = 0, round = 0;
+  double msPerPath[kPoolSamples + 1] = {};
+  int best() const {
+    int b = kCandidates[0];
+    for (int i = 0; i < next && i < 5; i++)
+      if (msPerPath[kCandidates[i]] < msPerPath[b]) b = kCandidates[i];
+    return b;
+  }
+  // Groups for a pass of `count` samples; trial is set when the pass is timed.
+  int pick(bool& trial, int count) {
+    trial = false;
+    if (chosen == 0 && next > 0 && next < 5 && kCandidates[next] > count) chosen = best();
+    if (chosen != 0) return std::min(chosen, std::max(count, 1));
+    trial = true;
+    return kCandidates[next];
+  }
+  void record(int g, double ms) {
+    msPerPath[g] = round == 0 ? ms : std::min(msPerPath[g], ms);
+    if (++round < rounds) return;
+    round = 0;
+    next++;
+    const int b = best();
+    if (next == 5 || (!exhaustive && msPerPath[g] > 1.05 * msPerPath[b])) chosen = b;
+  }
+  void reset() {
+    const int r = rounds;
+    const bool ex = exhaustive;
+    *this = GroupPicker{};
+    rounds = r;
+    exhaustive = ex;
+  }
+};
+
 #define HIP_OK(expr)                                                                          \
   do {                                                                                        \
     hipError_t e_ = (expr);                                                                   \
@@ -340,6 +383,8 @@ struct sundog_session {
   double renderMs = 0.0;
   double buildMs = 0.0;
   int cuCount = 0;  // CUs of the session's device (queried at the first pass)
+  int sampleGroups = 0;  // megakernel sample groups per tile; 0: chosen by groupPick
+  GroupPicker groupPick;
   uint64_t paths = 0;
   int tlasNodes = 0, blasNodes = 0;
   int bvhBuilder = 0;        // 0: host binned SAH, 1: GPU LBVH
@@ -375,7 +420,22 @@ struct sundog_session {
     return waves != 0 ? waves : autoPick.pick(trial);
   }
   void recordTrial(int w, double msPerPath) { autoPick.record(w, msPerPath); }
-  void resetAuto() { autoPick.reset(); }
+  void resetAuto() {
+    autoPick.reset();
+    groupPick.reset();
+  }
+  // Sample groups for a pass: fixed by tuning, the frame-size rule while the
+  // occupancy variant is being timed or for the work-counting pass, else the
+  // group picker's choice or candidate (gTrial: the pass is a group trial).
+  int pickGroups(size_t tiles, int count, bool wavesTrial, bool& gTrial) {
+    gTrial = false;
+    if (count > kPoolSamples) return 1;
+    if (sampleGroups > 0) return std::min(sampleGroups, count);
+    if (wavesTrial || countWork || (waves == 0 && autoPick.chosen == 0))
+      return sampleGroupsFor(tiles, count, cuCount);
+    return groupPick.pick(gTrial, count);
+  }
+  int currentGroups() const { return sampleGroups > 0 ? sampleGroups : groupPick.chosen; }
   // The variant renders use now (the fixed one, the auto choice, or 0 while
   // auto mode is still timing candidates).
   int currentWaves() const { return waves != 0 ? waves : autoPick.chosen; }
@@ -854,13 +914,13 @@ struct sundog_session {
       HIP_OK(hipGetDevice(&dev));
       HIP_OK(hipDeviceGetAttribute(&cuCount, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    const int groups = sampleGroupsFor(gx * gy, count, cuCount);
+    bool trial = false, gTrial = false;
+    const int w = countWork ? 2 : pickWaves(trial);
+    const int groups = pickGroups(gx * gy, count, trial, gTrial);
     ensureSpill(gx * gy * groups * renderBlockThreads());
     ensureSampleBuf(gx * gy * (renderBlockThreads() / 64));
     P.sampleOffset = offset;
     P.sppThisLaunch = count;
-    bool trial = false;
-    const int w = countWork ? 2 : pickWaves(trial);
     HIP_OK(hipEventRecord(ev0, stream));
     HIP_OK(launchRender(P, countWork, w, groups, stream));
     HIP_OK(hipEventRecord(ev1, stream));
@@ -868,6 +928,7 @@ struct sundog_session {
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
     if (trial) recordTrial(w, ms / ((double)P.width * P.height * count));
+    if (gTrial) groupPick.record(groups, ms / ((double)P.width * P.height * count));
     renderMs += ms;
     paths += (uint64_t)P.width * P.height * count;
   }

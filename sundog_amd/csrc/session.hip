@@ -132,6 +132,11 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
     } else if (k == "wf_waves") {
       if (value < 4 || value > 5) throw std::runtime_error("tuning: wf_waves must be 4 or 5");
       s->wfWaves = value;
+    } else if (k == "groups") {
+      if (value < 0 || value > kPoolSamples)
+        throw std::runtime_error("tuning: groups must be 0 (auto) or 1.." + std::to_string(kPoolSamples));
+      s->sampleGroups = value;
+      if (value == 0) s->groupPick.reset();
     } else {
       throw std::runtime_error("tuning: unknown key '" + k + "'");
     }
@@ -146,17 +151,25 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     if (s->kernel == SUNDOG_KERNEL_WAVEFRONT) throw std::runtime_error("autotune: times the megakernel");
     const bool counting = s->countWork;
     s->countWork = false;
-    s->waves = 0;
-    // every candidate, twice (the faster pass counts): one untimed warm-up
-    // pass and eight timed ones; later passes of this session then use the
-    // winner
+    // a variant fixed by the "waves" tuning is kept; the rest is timed:
+    // every candidate twice (the faster pass counts), one untimed warm-up
+    // pass and eight timed ones for the occupancy variant, then up to ten
+    // for the sample groups at that variant; later passes of this session
+    // then use the winners
+    const bool tuneWaves = s->waves == 0;
+    const bool tuneGroups = s->sampleGroups == 0 && count <= kPoolSamples;
     s->autoPick = OccupancyPicker{};
     s->autoPick.rounds = 2;
     s->autoPick.exhaustive = true;
-    for (int i = 0; i < 9 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
+    s->groupPick = GroupPicker{};
+    s->groupPick.rounds = 2;
+    s->groupPick.exhaustive = true;
+    for (int i = 0; i < 20 && ((tuneWaves && s->autoPick.chosen == 0) || (tuneGroups && s->groupPick.chosen == 0));
+         i++)
+      s->renderPass(0, count);
     s->countWork = counting;
     s->reset();
-    if (waves_out) *waves_out = s->autoPick.chosen;
+    if (waves_out) *waves_out = s->currentWaves();
     return SUNDOG_OK;
   });
 }
@@ -213,6 +226,7 @@ extern "C" int sundog_session_get_stats(sundog_session* s, sundog_session_stats*
     out->blas_nodes = s->blasNodes;
     out->waves = s->currentWaves();
     out->stack_lds = s->P.stackLds;
+    out->sample_groups = s->currentGroups();
     return SUNDOG_OK;
   });
 }

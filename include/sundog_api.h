@@ -248,7 +248,8 @@ SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int v
  * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per
  * SIMD twice each, the faster pass counting), then, when `count` <= 16 and
  * "groups" is 0, the sample groups 1, 2, 4, 8, 16 at that variant the same
- * way; keeps the fastest for later passes and clears the film. A value
+ * way and then the occupancy variants again at the chosen groups; keeps the
+ * fastest for later passes and clears the film. A value
  * fixed by "waves" or "groups" is kept and not timed. *waves_out receives
  * the occupancy variant in use (the groups are in the stats). */
 SUNDOG_API int sundog_session_autotune(sundog_session*, int count, int* waves_out);

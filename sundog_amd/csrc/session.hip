@@ -154,8 +154,8 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     // a variant fixed by the "waves" tuning is kept; the rest is timed:
     // every candidate twice (the faster pass counts), one untimed warm-up
     // pass and eight timed ones for the occupancy variant, then up to ten
-    // for the sample groups at that variant; later passes of this session
-    // then use the winners
+    // for the sample groups at that variant, then the occupancy variant
+    // again at those groups; later passes of this session use the winners
     const bool tuneWaves = s->waves == 0;
     const bool tuneGroups = s->sampleGroups == 0 && count <= kPoolSamples;
     s->autoPick = OccupancyPicker{};
@@ -167,6 +167,16 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     for (int i = 0; i < 20 && ((tuneWaves && s->autoPick.chosen == 0) || (tuneGroups && s->groupPick.chosen == 0));
          i++)
       s->renderPass(0, count);
+    if (tuneWaves && tuneGroups && s->groupPick.chosen != 0) {
+      // the occupancy variant again, at the chosen groups (the groups change
+      // how many blocks are resident at once, so the best variant can move)
+      s->sampleGroups = s->groupPick.chosen;
+      s->autoPick = OccupancyPicker{};
+      s->autoPick.rounds = 2;
+      s->autoPick.exhaustive = true;
+      for (int i = 0; i < 9 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
+      s->sampleGroups = 0;  // auto mode, with the picker's choice kept
+    }
     s->countWork = counting;
     s->reset();
     if (waves_out) *waves_out = s->currentWaves();

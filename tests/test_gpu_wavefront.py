@@ -67,6 +67,30 @@ def test_occupancy_variants_bit_identical(name, asset_dir):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), waves
 
 
+@pytest.mark.parametrize("name", ["cornell", "instanced_mini", "volumes_mini"])
+def test_sample_groups_bit_identical(name, asset_dir):
+    """Sample groups split a pass's samples over several blocks per tile and
+    fold them in a separate kernel; every group count, even splits and
+    uneven ones (3 or 7 groups of a 12-sample pass, 16 groups of 12 samples
+    leaving some groups empty), gives the same bits as one group."""
+    sc = _scene(name, asset_dir)
+    spp = sc.session().spp
+    ref = None
+    for groups in (1, 2, 3, 7, 16):
+        s = sc.session()
+        s.tune("waves", 3)
+        s.tune("groups", groups)
+        for off in range(0, spp, 12):
+            s.render(off, min(12, spp - off))
+        films = s.download()
+        assert s.stats()["sample_groups"] == groups
+        if ref is None:
+            ref = films
+        else:
+            for a, b in zip(ref, films):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), groups
+
+
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini"])
 def test_autotune_picks_a_variant_and_keeps_the_bits(name, asset_dir):
     """sundog_session_autotune times the variants on real passes, keeps one
@@ -83,6 +107,7 @@ def test_autotune_picks_a_variant_and_keeps_the_bits(name, asset_dir):
     assert w in (2, 3, 4, 5)
     st = s.stats()
     assert st["waves"] == w and st["paths"] == 0
+    assert st["sample_groups"] in (1, 2, 4)       # candidates up to the pass's 4 samples
     assert not any(f.any() for f in s.download())      # film cleared
     s.render(0, spp)
     for a, b in zip(ref, s.download()):

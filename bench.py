@@ -85,8 +85,9 @@ def main():
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
                     help="minimum waves per SIMD of the wavefront query kernels")
-    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh"],
-                    help="host binned-SAH build (default) or the GPU LBVH rebuild")
+    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh", "lbvh-sah-top"],
+                    help="host binned-SAH build (default), the GPU LBVH rebuild of both levels, or the GPU "
+                         "LBVH for the meshes with a host SAH top level over GPU instance boxes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
@@ -135,6 +136,8 @@ def main():
     sess.tune("wf_waves", args.wf_waves)
     if args.bvh == "lbvh":
         sess.tune("bvh", 1)
+    elif args.bvh == "lbvh-sah-top":
+        sess.tune("bvh", 2)
     setup_s = time.time() - t0
     k = args.spp_per_step
 

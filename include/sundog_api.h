@@ -215,7 +215,7 @@ typedef struct {
    * 4-wide node visits (4 box tests each), triangle tests, quadric tests,
    * world->object instance transforms */
   uint64_t node_visits, tri_tests, quad_tests, inst_transforms;
-  int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH */
+  int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH, 2: GPU LBVH meshes + host SAH top */
   int32_t pad2;
   double bvh_gpu_ms;        /* wall time of the last GPU LBVH rebuild */
   int32_t waves;            /* megakernel occupancy variant in use (0: auto mode still timing) */
@@ -242,7 +242,9 @@ SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
  * separate kernel then folds in sample order; 0, the default, picks from
  * 1, 2, 4, 8, 16 by timing passes after the occupancy variant is chosen;
  * films are bit-identical for every choice), and
- * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder). */
+ * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder; 2:
+ * the mesh levels with the GPU LBVH and the top level with the host SAH
+ * builder over instance boxes computed on the GPU). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Times the megakernel's occupancy variants on passes of `count` samples
  * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per

@@ -808,7 +808,10 @@ struct sundog_session {
   // one instance per leaf), then every used mesh's BLAS (up to 4 triangles
   // per leaf, triangles re-emitted in leaf order). Closest hits do not depend
   // on the tree, so films stay bit-identical wherever shadow rays are opaque.
-  void buildBvhGpu() {
+  // GPU rebuild of the BVH. sahTop: the top level is built with the host's
+  // binned SAH over the instance boxes the GPU computed (4,096 boxes take
+  // about a millisecond), the mesh levels with the GPU LBVH.
+  void buildBvhGpu(bool sahTop = false) {
     HIP_OK(hipStreamSynchronize(stream));
     auto t0 = std::chrono::steady_clock::now();
     const Scene& S = prep.scene;
@@ -838,8 +841,25 @@ struct sundog_session {
     HIP_OK(launchInstanceBoxes(positions.as<float>(), boxJobs.as<InstanceBoxJob>(), (int)jobs.size(),
                                instBoxes.as<float4>(), stream));
     int tlasStack = 0;
-    const int nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack,
-                             stream);
+    int nT = 0;
+    if (sahTop) {
+      std::vector<float4> hb(ib.size());
+      HIP_OK(hipMemcpyAsync(hb.data(), instBoxes.p, hb.size() * sizeof(float4), hipMemcpyDeviceToHost, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      std::vector<Aabb> boxes(S.objects.size());
+      for (size_t i = 0; i < boxes.size(); i++) {
+        boxes[i].lo = vec3{hb[2 * i].x, hb[2 * i].y, hb[2 * i].z};
+        boxes[i].hi = vec3{hb[2 * i + 1].x, hb[2 * i + 1].y, hb[2 * i + 1].z};
+      }
+      Bvh4 top = buildBvh4(boxes, 1);
+      std::vector<BvhNode4> tn = top.nodes;
+      relocate(tn, 0, tn.size(), 0, 0, &top.primOrder);
+      tlas.upload(tn, stream);
+      nT = (int)tn.size();
+      tlasStack = top.stackBound;
+    } else {
+      nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack, stream);
+    }
     struct Blas {
       DevBuf nodes, keys;
       int num = 0, stack = 0;
@@ -897,7 +917,7 @@ struct sundog_session {
     blasNodes = (int)nodeBase - nT;
     stackNeed = tlasStack + 1 + blasStack;
     spillRows = spillRowsFor(P.stackLds);
-    bvhBuilder = 1;
+    bvhBuilder = sahTop ? 2 : 1;
     bvhGpuMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 

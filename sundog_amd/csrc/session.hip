@@ -127,8 +127,10 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
       HIP_OK(hipStreamSynchronize(s->stream));
       s->P.stackLds = value;
     } else if (k == "bvh") {
-      if (value != 1) throw std::runtime_error("tuning: bvh must be 1 (rebuild both levels with the GPU LBVH)");
-      s->buildBvhGpu();
+      if (value != 1 && value != 2)
+        throw std::runtime_error("tuning: bvh must be 1 (rebuild both levels with the GPU LBVH) or 2 (GPU LBVH "
+                                 "mesh levels, host SAH top level over GPU instance boxes)");
+      s->buildBvhGpu(value == 2);
     } else if (k == "wf_waves") {
       if (value < 4 || value > 5) throw std::runtime_error("tuning: wf_waves must be 4 or 5");
       s->wfWaves = value;

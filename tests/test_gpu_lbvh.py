@@ -1,7 +1,8 @@
-"""GPU LBVH build: closest hits through the GPU-built two-level BVH equal
-brute force bit-exactly (t bits, instance, primitive), and films equal the
-host-SAH BVH's films bit-exactly where shadow rays are opaque (closest hits
-and opaque occlusion do not depend on the tree)."""
+"""GPU LBVH build (both levels, or the mesh levels under a host-SAH top
+level over GPU instance boxes): closest hits through the rebuilt two-level
+BVH equal brute force bit-exactly (t bits, instance, primitive), and films
+equal the host-SAH BVH's films bit-exactly where shadow rays are opaque
+(closest hits and opaque occlusion do not depend on the tree)."""
 
 import numpy as np
 import pytest
@@ -31,13 +32,14 @@ def _scene(name, asset_dir, **kw):
     return configs.instanced(asset_dir, instances=2048, subdiv=4, **kw)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini", "instanced_2k"])
-def test_lbvh_closest_hit_equals_brute_force(name, asset_dir):
+def test_lbvh_closest_hit_equals_brute_force(name, mode, asset_dir):
     sc = _scene(name, asset_dir, width=32, height=32, spp=1).build()
     sess = sc.session()
-    sess.tune("bvh", 1)
+    sess.tune("bvh", mode)
     st = sess.stats()
-    assert st["bvh_builder"] == 1 and st["bvh_gpu_ms"] > 0 and st["tlas_nodes"] >= 1
+    assert st["bvh_builder"] == mode and st["bvh_gpu_ms"] > 0 and st["tlas_nodes"] >= 1
     if name == "cornell":
         rays = _rays(4096, 3, -0.9, 0.9, -0.9, 0.9)
     else:
@@ -49,14 +51,15 @@ def test_lbvh_closest_hit_equals_brute_force(name, asset_dir):
     assert np.array_equal(t.view(np.uint32), tb.view(np.uint32))
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini"])
-def test_lbvh_film_equals_sah_film(name, asset_dir):
+def test_lbvh_film_equals_sah_film(name, mode, asset_dir):
     sc = _scene(name, asset_dir, width=48, height=32, spp=8).build()
     a = sc.session()
     a.render_all()
     fa, _, _ = a.download()
     b = sc.session()
-    b.tune("bvh", 1)
+    b.tune("bvh", mode)
     b.render_all()
     fb, _, _ = b.download()
     assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))

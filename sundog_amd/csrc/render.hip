@@ -932,7 +932,7 @@ DEV float lightPdfSolidAngle(const DevLight& lt, vec3 from, vec3 hitP, vec3 nHit
 
 // ---------------------------------------------------------------------------
 // geometry. Contraction is disabled in these functions so that the CPU
-// checker, compiled without FMA contraction, reproduces every t bit.
+// oracle, compiled without FMA contraction, reproduces every t bit.
 // ---------------------------------------------------------------------------
 
 // Canonical quadrics in object space (device/intersect.cuh:37-162): up to

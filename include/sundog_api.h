@@ -240,7 +240,7 @@ SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
  * (4..24: LDS traversal-stack entries per lane of the megakernel; default
  * 12), "groups" (1..16: blocks per tile sharing a pass's samples, which a
  * separate kernel then folds in sample order; 0, the default, picks from
- * 1, 2, 4, 8, 16 by timing passes after the occupancy variant is chosen;
+ * 1, 2, 3, 4, 6, 8, 16 by timing passes after the occupancy variant is chosen;
  * films are bit-identical for every choice), and
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder; 2:
  * the mesh levels with the GPU LBVH and the top level with the host SAH
@@ -249,7 +249,7 @@ SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int v
 /* Times the megakernel's occupancy variants on passes of `count` samples
  * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per
  * SIMD twice each, the faster pass counting), then, when `count` <= 16 and
- * "groups" is 0, the sample groups 1, 2, 4, 8, 16 at that variant the same
+ * "groups" is 0, the sample groups 1, 2, 3, 4, 6, 8, 16 at that variant the same
  * way and then the occupancy variants again at the chosen groups; keeps the
  * fastest for later passes and clears the film. A value
  * fixed by "waves" or "groups" is kept and not timed. *waves_out receives

@@ -124,27 +124,28 @@ struct OccupancyPicker {
 };
 
 // Sample groups per tile for the megakernel, chosen like the occupancy
-// variant from timed passes (after it, at the chosen variant): 1, 2, 4, 8,
-// 16 groups until one is more than 5% slower than the best so far. Frames
+// variant from timed passes (after it, at the chosen variant): 1, 2, 3, 4,
+// 6, 8, 16 groups until one is more than 5% slower than the best so far. Frames
 // whose cost varies a lot between tiles (flames) want many groups: the
 // expensive tiles' samples spread over more blocks, so the grid does not end
 // on a few long blocks.
 struct GroupPicker {
-  static constexpr int kCandidates[5] = {1, 2, 4, 8, 16};
+  static constexpr int kN = 7;
+  static constexpr int kCandidates[kN] = {1, 2, 3, 4, 6, 8, 16};
   int rounds = 1;
   bool exhaustive = false;
   int chosen = 0, next = 0, round = 0;
   double msPerPath[kPoolSamples + 1] = {};
   int best() const {
     int b = kCandidates[0];
-    for (int i = 0; i < next && i < 5; i++)
+    for (int i = 0; i < next && i < kN; i++)
       if (msPerPath[kCandidates[i]] < msPerPath[b]) b = kCandidates[i];
     return b;
   }
   // Groups for a pass of `count` samples; trial is set when the pass is timed.
   int pick(bool& trial, int count) {
     trial = false;
-    if (chosen == 0 && next > 0 && next < 5 && kCandidates[next] > count) chosen = best();
+    if (chosen == 0 && next > 0 && next < kN && kCandidates[next] > count) chosen = best();
     if (chosen != 0) return std::min(chosen, std::max(count, 1));
     trial = true;
     return kCandidates[next];
@@ -155,7 +156,7 @@ struct GroupPicker {
     round = 0;
     next++;
     const int b = best();
-    if (next == 5 || (!exhaustive && msPerPath[g] > 1.05 * msPerPath[b])) chosen = b;
+    if (next == kN || (!exhaustive && msPerPath[g] > 1.05 * msPerPath[b])) chosen = b;
   }
   void reset() {
     const int r = rounds;

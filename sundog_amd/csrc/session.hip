@@ -155,8 +155,8 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     s->countWork = false;
     // a variant fixed by the "waves" tuning is kept; the rest is timed:
     // every candidate twice (the faster pass counts), one untimed warm-up
-    // pass and eight timed ones for the occupancy variant, then up to ten
-    // for the sample groups at that variant, then the occupancy variant
+    // pass and eight timed ones for the occupancy variant, then up to
+    // fourteen for the sample groups at that variant, then the occupancy variant
     // again at those groups; later passes of this session use the winners
     const bool tuneWaves = s->waves == 0;
     const bool tuneGroups = s->sampleGroups == 0 && count <= kPoolSamples;
@@ -166,7 +166,7 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     s->groupPick = GroupPicker{};
     s->groupPick.rounds = 2;
     s->groupPick.exhaustive = true;
-    for (int i = 0; i < 20 && ((tuneWaves && s->autoPick.chosen == 0) || (tuneGroups && s->groupPick.chosen == 0));
+    for (int i = 0; i < 32 && ((tuneWaves && s->autoPick.chosen == 0) || (tuneGroups && s->groupPick.chosen == 0));
          i++)
       s->renderPass(0, count);
     if (tuneWaves && tuneGroups && s->groupPick.chosen != 0) {

@@ -107,7 +107,7 @@ def test_autotune_picks_a_variant_and_keeps_the_bits(name, asset_dir):
     assert w in (2, 3, 4, 5)
     st = s.stats()
     assert st["waves"] == w and st["paths"] == 0
-    assert st["sample_groups"] in (1, 2, 4)       # candidates up to the pass's 4 samples
+    assert st["sample_groups"] in (1, 2, 3, 4)    # candidates up to the pass's 4 samples
     assert not any(f.any() for f in s.download())      # film cleared
     s.render(0, spp)
     for a, b in zip(ref, s.download()):

@@ -419,3 +419,38 @@ def test_scene_program_ids_by_sorted_name(lib):
     d = describe(lib, h.ptr)
     assert d["materials"][0]["kind"] == 1 and d["objects"][0]["mat_front"] == 1
     assert math.isnan(prog[1][5]) and prog[1][6] == -1   # clamp NaN, seed -1 sentinels
+
+
+def test_ctypes_struct_layouts_match_the_header(tmp_path):
+    """Every struct the ctypes bindings mirror has the header's size and
+    field offsets (compiled with the host C compiler from include/): the
+    library writes whole structs, so a shorter mirror would be overrun."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    pairs = [("sundog_xform_step", _lib.XformStep), ("sundog_physics_body", _lib.PhysicsBody),
+             ("sundog_render_options", _lib.RenderOptions), ("sundog_device_info", _lib.DeviceInfo),
+             ("sundog_session_stats", _lib.SessionStats), ("sundog_request_info", _lib.RequestInfo),
+             ("sundog_server_stats", _lib.ServerStats), ("sundog_tune_result", _lib.TuneResult)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sundog_api.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        parts = line.split()
+        if len(parts) == 3:
+            got[(parts[0], parts[1])] = int(parts[2])
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -1556,6 +1556,27 @@ V background(const Scene& S, V dir) {
   return ld(f->bg_a);
 }
 
+// Strata per axis over the total spp: the largest n with n * n <= spp
+// (rounded square root, stepped down when it overshoots).
+int strataPerAxis(int spp) {
+  int n = (int)floorf(sqrtf((float)spp) + 0.5f);
+  if (n * n > spp) n--;
+  return n < 1 ? 1 : n;
+}
+
+// Jitter of sample s in its pixel (device/programs.cu:316-340): sample s <
+// n^2 lies in stratum (s mod n, s div n) of the n x n grid, later samples are
+// plain uniforms; two draws either way.
+void cameraJitter(int s, int nStrata, Rng& rng, float& jx, float& jy) {
+  if (s < nStrata * nStrata) {
+    jx = ((s % nStrata) + rng.rnd()) / nStrata;
+    jy = ((s / nStrata) + rng.rnd()) / nStrata;
+  } else {
+    jx = rng.rnd();
+    jy = rng.rnd();
+  }
+}
+
 void renderPixel(const Scene& S, int px, int py, int s0, int s1, bool brute, float* acc, float* alb, float* nrm,
                  uint64_t& radiance, uint64_t& shadow) {
   const sundog_flat_scene* f = S.f;
@@ -1565,9 +1586,7 @@ void renderPixel(const Scene& S, int px, int py, int s0, int s1, bool brute, flo
   V camOrigin = ld(cam), lowerLeft = ld(cam + 3), horizontal = ld(cam + 6), vertical = ld(cam + 9);
   V cu = ld(cam + 12), cv = ld(cam + 15), cw = ld(cam + 18);
   float lens = cam[21];
-  int nStrata = (int)floorf(sqrtf((float)f->spp) + 0.5f);
-  if (nStrata * nStrata > f->spp) nStrata--;
-  if (nStrata < 1) nStrata = 1;
+  const int nStrata = strataPerAxis(f->spp);
   const bool hasEnv = f->bg_kind == 2;
   const int nStrat = f->num_lights + (hasEnv ? 1 : 0);
   float* A = acc + 4 * (size_t)pixel;
@@ -1576,13 +1595,7 @@ void renderPixel(const Scene& S, int px, int py, int s0, int s1, bool brute, flo
   for (int s = s0; s < s1; s++) {
     Rng rng = Rng::init(((uint64_t)pixel << 32) ^ (uint64_t)s, (uint64_t)f->seed);
     float jx, jy;
-    if (s < nStrata * nStrata) {
-      jx = ((s % nStrata) + rng.rnd()) / nStrata;
-      jy = ((s / nStrata) + rng.rnd()) / nStrata;
-    } else {
-      jx = rng.rnd();
-      jy = rng.rnd();
-    }
+    cameraJitter(s, nStrata, rng, jx, jy);
     float u = (px + jx) / W;
     float v = ((H - 1 - py) + jy) / H;
     V org = camOrigin;
@@ -1995,5 +2008,65 @@ extern "C" int oracle_mesh_light_sample(const sundog_flat_scene* f, int light, c
       pdf_out[i] = s.valid ? s.pdf : 0.0f;
       tri_out[i] = tri;
     }
+  });
+}
+
+// Light samples of light `light` from n shading points xs (n x 3), sample i
+// drawing from the stream init(seed0 + i, seq) (device/light_sample.cuh:24-188):
+// direction, distance, radiance, pdf (0 when invalid), flags (bit 0 valid,
+// bit 1 delta), the draws taken, and the MIS-side pdf that
+// lightPdfSolidAngle (device/light_sample.cuh:197-226) returns for the
+// sampled point x + wi * dist (NaN for mesh and delta lights, whose MIS side
+// needs the hit normal or does not exist).
+extern "C" int oracle_light_sample(const sundog_flat_scene* f, int light, const float* xs, int n, uint64_t seed0,
+                                   uint64_t seq, float* wi_out, float* dist_out, float* li_out, float* pdf_out,
+                                   int32_t* flags_out, int32_t* draws_out, float* pdf_back_out) {
+  return guard([&] {
+    if (light < 0 || light >= f->num_lights) throw std::runtime_error("oracle_light_sample: light id out of range");
+    Scene S;
+    setupScene(f, S);
+    const sundog_flat_light& lt = f->lights[light];
+    for (int i = 0; i < n; i++) {
+      Rng rng = Rng::init(seed0 + (uint64_t)i, seq);
+      V x = ld(xs + 3 * i);
+      LightSample s = sampleLight(S, light, x, rng);
+      wi_out[3 * i] = s.wi.x;
+      wi_out[3 * i + 1] = s.wi.y;
+      wi_out[3 * i + 2] = s.wi.z;
+      dist_out[i] = s.dist;
+      li_out[3 * i] = s.Li.x;
+      li_out[3 * i + 1] = s.Li.y;
+      li_out[3 * i + 2] = s.Li.z;
+      pdf_out[i] = s.valid ? s.pdf : 0.0f;
+      flags_out[i] = (s.valid ? 1 : 0) | (s.isDelta ? 2 : 0);
+      draws_out[i] = (int32_t)rng.draws;
+      float back = NAN;
+      if (s.valid && !s.isDelta && lt.kind != 5) back = lightPdfSolidAngle(S, light, x, x + s.wi * s.dist, spl(0.0f));
+      pdf_back_out[i] = back;
+    }
+  });
+}
+
+// lightPdfSolidAngle (device/light_sample.cuh:197-226) for n (from, hit
+// point, hit normal) triples (n x 9 floats).
+extern "C" int oracle_light_pdf(const sundog_flat_scene* f, int light, const float* q, int n, float* pdf_out) {
+  return guard([&] {
+    if (light < 0 || light >= f->num_lights) throw std::runtime_error("oracle_light_pdf: light id out of range");
+    Scene S;
+    setupScene(f, S);
+    for (int i = 0; i < n; i++)
+      pdf_out[i] = lightPdfSolidAngle(S, light, ld(q + 9 * i), ld(q + 9 * i + 3), ld(q + 9 * i + 6));
+  });
+}
+
+// The camera-ray jitter of sample s of pixel (px, py), by the path loop's
+// own cameraJitter.
+extern "C" int oracle_camera_jitter(const sundog_flat_scene* f, int px, int py, int s, float* jx, float* jy) {
+  return guard([&] {
+    if (px < 0 || py < 0 || px >= f->width || py >= f->height || s < 0)
+      throw std::runtime_error("oracle_camera_jitter: pixel or sample out of range");
+    const int pixel = py * f->width + px;
+    Rng rng = Rng::init(((uint64_t)pixel << 32) ^ (uint64_t)s, (uint64_t)f->seed);
+    cameraJitter(s, strataPerAxis(f->spp), rng, *jx, *jy);
   });
 }

@@ -85,6 +85,17 @@ int oracle_flame_field(const sundog_flat_scene* s, int flame, const float* p, in
  * the light's total world-space area. */
 int oracle_mesh_light_sample(const sundog_flat_scene* s, int light, const float x[3], const float* u, int n,
                              float* wi_out, float* dist_out, float* pdf_out, int32_t* tri_out, float* area_out);
+/* Light samples of light `light` from n points xs (n x 3); sample i draws
+ * from init(seed0 + i, seq). Outputs: wi (n x 3), dist, Li (n x 3), pdf (0 when
+ * invalid), flags (bit 0 valid, bit 1 delta), draws taken, and the MIS-side
+ * solid-angle pdf at the sampled point (NaN for mesh and delta lights). */
+int oracle_light_sample(const sundog_flat_scene* s, int light, const float* xs, int n, uint64_t seed0,
+                        uint64_t seq, float* wi_out, float* dist_out, float* li_out, float* pdf_out,
+                        int32_t* flags_out, int32_t* draws_out, float* pdf_back_out);
+/* MIS-side light pdf for n (from, hit point, hit normal) triples (n x 9). */
+int oracle_light_pdf(const sundog_flat_scene* s, int light, const float* q, int n, float* pdf_out);
+/* Camera jitter (jx, jy) in [0,1)^2 of sample s of pixel (px, py). */
+int oracle_camera_jitter(const sundog_flat_scene* s, int px, int py, int sample, float* jx, float* jy);
 /* Last error message. */
 const char* oracle_last_error(void);
 

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIB points the checks at another build of the oracle (planted-mistake runs)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 P_f = ctypes.POINTER(ctypes.c_float)
 P_i32 = ctypes.POINTER(ctypes.c_int32)
@@ -89,6 +90,10 @@ def load():
     lib.oracle_flame_field.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, ctypes.c_int, P_f, P_f]
     lib.oracle_mesh_light_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, P_f, ctypes.c_int, P_f, P_f,
                                              P_f, P_i32, P_f]
+    lib.oracle_light_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, ctypes.c_int, ctypes.c_uint64,
+                                        ctypes.c_uint64, P_f, P_f, P_f, P_f, P_i32, P_i32, P_f]
+    lib.oracle_light_pdf.argtypes = [ctypes.c_void_p, ctypes.c_int, P_f, ctypes.c_int, P_f]
+    lib.oracle_camera_jitter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P_f, P_f]
     lib.oracle_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -297,3 +302,46 @@ def mesh_light_sample(flat, light, x, u):
                                     _fp(dist), _fp(pdf), tri.ctypes.data_as(P_i32), _fp(area)) != 0:
         raise RuntimeError(_err(lib))
     return {"wi": wi, "dist": dist, "pdf": pdf, "tri": tri, "area": float(area[0])}
+
+
+def light_sample(flat, light, xs, seed0=1, seq=7):
+    """Samples of light `light` from (n, 3) points xs, sample i from the
+    stream init(seed0 + i, seq): dict of wi (n, 3), dist, Li (n, 3), pdf,
+    valid, delta, draws and pdf_back (the MIS-side pdf at the sampled point,
+    NaN for mesh and delta lights)."""
+    lib = load()
+    xs = np.ascontiguousarray(xs, np.float32).reshape(-1, 3)
+    n = len(xs)
+    wi = np.zeros((n, 3), np.float32)
+    dist = np.zeros(n, np.float32)
+    li = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    flags = np.zeros(n, np.int32)
+    draws = np.zeros(n, np.int32)
+    back = np.zeros(n, np.float32)
+    if lib.oracle_light_sample(flat.ptr, light, _fp(xs), n, seed0, seq, _fp(wi), _fp(dist), _fp(li), _fp(pdf),
+                               flags.ctypes.data_as(P_i32), draws.ctypes.data_as(P_i32), _fp(back)) != 0:
+        raise RuntimeError(_err(lib))
+    return {"wi": wi, "dist": dist, "Li": li, "pdf": pdf, "valid": (flags & 1) != 0,
+            "delta": (flags & 2) != 0, "draws": draws, "pdf_back": back}
+
+
+def light_pdf(flat, light, frm, hit, nhit):
+    """The MIS-side solid-angle pdf of light `light` for (n, 3) arrays of
+    shading points, points on the light and the light's normal there."""
+    lib = load()
+    q = np.ascontiguousarray(np.concatenate([np.reshape(frm, (-1, 3)), np.reshape(hit, (-1, 3)),
+                                             np.reshape(nhit, (-1, 3))], axis=1), np.float32)
+    out = np.zeros(len(q), np.float32)
+    if lib.oracle_light_pdf(flat.ptr, light, _fp(q), len(q), _fp(out)) != 0:
+        raise RuntimeError(_err(lib))
+    return out
+
+
+def camera_jitter(flat, px, py, s):
+    lib = load()
+    jx = np.zeros(1, np.float32)
+    jy = np.zeros(1, np.float32)
+    if lib.oracle_camera_jitter(flat.ptr, px, py, s, _fp(jx), _fp(jy)) != 0:
+        raise RuntimeError(_err(lib))
+    return float(jx[0]), float(jy[0])

@@ -284,6 +284,15 @@ SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,
  * marginal row and conditional column picked by the binary searches. */
 SUNDOG_API int sundog_session_env_bins(sundog_session*, const float* u, int n,
                                        int32_t* row_out, int32_t* col_out);
+/* Light sampling on device: n samples of light `light` (an index into the
+ * scene's light list) from the points xs (n x 3); sample i draws from the
+ * PCG32 stream init(seed0 + i, seq). out: 12 floats per sample — wi (3),
+ * distance, Li (3), solid-angle pdf (0 when invalid), flags (bit 0 valid,
+ * bit 1 delta), draws taken, the MIS-side pdf lightPdfSolidAngle returns for
+ * the sampled point (NaN for mesh and delta lights), 0. The sampler is the
+ * one the render kernels call (device/light_sample.cuh:24-226). */
+SUNDOG_API int sundog_session_light_samples(sundog_session*, int light, const float* xs, int n,
+                                            uint64_t seed0, uint64_t seq, float* out);
 /* Flame density field on device: extinction sigma (n floats) and emission
  * (n x 3 floats) of flame `flame` at n points (x, y, z), the values the
  * flame march integrates (device/volume.cuh:27-153). */

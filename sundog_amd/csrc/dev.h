@@ -123,6 +123,12 @@ struct LaunchParams {
   float envRotate, envIntensity;
   int32_t envImportance;
   int32_t hasEnv;
+  // 16-ary lookup tables of the CDFs (buildCdfSearch): the marginal's
+  // table, then one table per row; null when a CDF is too long for three
+  // levels (the lookups then bisect the CDFs)
+  const float4* envSearch;
+  int32_t envSearchMargBlocks, envSearchMargMid;  // 16-float blocks of the marginal table; its mid-level blocks
+  int32_t envSearchRowBlocks, envSearchRowMid;    // the same for each row's table
 
   const DevTexture* textures;
   const float4* texels;

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dev.h"
+#include "shade.h"
 
 namespace sdk {
 
@@ -23,1099 +24,6 @@ constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) LDS entries
 // (24 KiB per block) leave LDS for all 20 waves of a CU; measured fastest at
 // 1080p on the instanced scene (deeper entries spill to HBM).
 constexpr int kStackLdsMegakernel = 12;
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-
-DEV vec3 ld3(const float* p) { return vec3{p[0], p[1], p[2]}; }
-DEV vec3 ld3i(const float* pool, int i) { return vec3{pool[3 * i], pool[3 * i + 1], pool[3 * i + 2]}; }
-DEV vec3 f4xyz(float4 a) { return vec3{a.x, a.y, a.z}; }
-DEV bool finite3(vec3 c) { return isfinite(c.x) && isfinite(c.y) && isfinite(c.z); }
-DEV vec3 sanitize(vec3 c) { return finite3(c) ? c : v3(0.0f); }
-
-// ---------------------------------------------------------------------------
-// PCG32 (O'Neill 2014, PCG-XSH-RR 64/32); stream init as the reference
-// device/rng.cuh:10-35.
-// ---------------------------------------------------------------------------
-
-struct Pcg {
-  uint64_t state, inc;
-  DEV uint32_t next() {
-    uint64_t old = state;
-    state = old * 6364136223846793005ULL + inc;
-    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-    uint32_t rot = (uint32_t)(old >> 59u);
-    return (xs >> rot) | (xs << ((32u - rot) & 31u));
-  }
-  DEV static Pcg make(uint64_t seed, uint64_t seq) {
-    Pcg r;
-    r.state = 0u;
-    r.inc = (seq << 1u) | 1u;
-    r.next();
-    r.state += seed;
-    r.next();
-    return r;
-  }
-  DEV float rnd() { return (float)(next() >> 8) * 0x1p-24f; }
-  DEV vec2 rnd2() {
-    float a = rnd();
-    float b = rnd();
-    return vec2{a, b};
-  }
-};
-
-// warps: device/rng.cuh:37-61
-DEV vec3 uniformSphere(vec2 u) {
-  float z = 1.0f - 2.0f * u.x;
-  float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-  float phi = 2.0f * kPi * u.y;
-  return vec3{r * cosf(phi), r * sinf(phi), z};
-}
-DEV vec2 concentricDisk(vec2 u) {
-  float ox = 2.0f * u.x - 1.0f, oy = 2.0f * u.y - 1.0f;
-  if (ox == 0.0f && oy == 0.0f) return vec2{0.0f, 0.0f};
-  float r, th;
-  if (fabsf(ox) > fabsf(oy)) {
-    r = ox;
-    th = (kPi / 4.0f) * (oy / ox);
-  } else {
-    r = oy;
-    th = (kPi / 2.0f) - (kPi / 4.0f) * (ox / oy);
-  }
-  return vec2{r * cosf(th), r * sinf(th)};
-}
-DEV vec3 cosineHemisphere(vec2 u) {
-  vec2 d = concentricDisk(u);
-  float z = sqrtf(fmaxf(0.0f, 1.0f - d.x * d.x - d.y * d.y));
-  return vec3{d.x, d.y, z};
-}
-
-// ---------------------------------------------------------------------------
-// textures: device/texture_eval.cuh:11-46; image textures are bilinear with
-// wrap addressing over linear float texels (sRGB decoded at load).
-// ---------------------------------------------------------------------------
-
-DEV int wrapi(int i, int n) {
-  int r = i % n;
-  return r < 0 ? r + n : r;
-}
-
-DEV float4 bilinear(const float4* texels, int w, int h, float u, float v, bool wrapV) {
-  u = u - floorf(u);  // u always wraps; v wraps for textures, clamps for the sky
-  if (wrapV) v = v - floorf(v);
-  float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
-  // keep float->int conversions in range (NaN maps to -1 via fmaxf)
-  x = fminf(fmaxf(x, -1.0f), (float)w);
-  y = fminf(fmaxf(y, -1.0f), (float)h);
-  float fx = floorf(x), fy = floorf(y);
-  float ax = x - fx, ay = y - fy;
-  int x0 = wrapi((int)fx, w), x1 = wrapi((int)fx + 1, w);
-  int y0, y1;
-  if (wrapV) {
-    y0 = wrapi((int)fy, h);
-    y1 = wrapi((int)fy + 1, h);
-  } else {
-    y0 = min(max((int)fy, 0), h - 1);
-    y1 = min(max((int)fy + 1, 0), h - 1);
-  }
-  float4 t00 = texels[(size_t)y0 * w + x0], t10 = texels[(size_t)y0 * w + x1];
-  float4 t01 = texels[(size_t)y1 * w + x0], t11 = texels[(size_t)y1 * w + x1];
-  float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay);
-  float w01 = (1.0f - ax) * ay, w11 = ax * ay;
-  return make_float4(t00.x * w00 + t10.x * w10 + t01.x * w01 + t11.x * w11,
-                     t00.y * w00 + t10.y * w10 + t01.y * w01 + t11.y * w11,
-                     t00.z * w00 + t10.z * w10 + t01.z * w01 + t11.z * w11,
-                     t00.w * w00 + t10.w * w10 + t01.w * w01 + t11.w * w11);
-}
-
-DEV float4 evalTexture(const LaunchParams& P, int id, float u, float v) {
-  const DevTexture& t = P.textures[id];
-  switch (t.kind) {
-    case TX_SOLID:
-      return make_float4(t.a[0], t.a[1], t.a[2], 1.0f);
-    case TX_CHECKER: {
-      int iu = (int)floorf(u * t.scale[0]);
-      int iv = (int)floorf(v * t.scale[1]);
-      const float* c = (((iu + iv) & 1) == 0) ? t.a : t.b;
-      return make_float4(c[0], c[1], c[2], 1.0f);
-    }
-    case TX_GRID: {
-      float fu = u * t.scale[0] - floorf(u * t.scale[0]);
-      float fv = v * t.scale[1] - floorf(v * t.scale[1]);
-      float w = t.width;
-      bool line = fu < w || fv < w || fu > 1.0f - w || fv > 1.0f - w;
-      const float* c = line ? t.b : t.a;
-      return make_float4(c[0], c[1], c[2], 1.0f);
-    }
-    case TX_IMAGE:
-      return bilinear(P.texels + t.imgOffset, t.imgW, t.imgH, u, 1.0f - v, true);
-    default:
-      return make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-  }
-}
-
-DEV vec3 materialAlbedo(const LaunchParams& P, const DevMaterial& m, float u, float v) {
-  if (m.texId < 0) return ld3(m.color);
-  float4 c = evalTexture(P, m.texId, u, v);
-  return vec3{c.x, c.y, c.z};
-}
-
-// ---------------------------------------------------------------------------
-// value noise / fbm: device/noise.cuh:13-79 (octave count parameterised; 3
-// reproduces the reference)
-// ---------------------------------------------------------------------------
-
-DEV float smoothstepf(float e0, float e1, float x) {
-  float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
-  return t * t * (3.0f - 2.0f * t);
-}
-DEV uint32_t hashU(uint32_t x) {
-  x = x * 747796405u + 2891336453u;
-  x = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
-  return (x >> 22u) ^ x;
-}
-DEV float hashLattice(int x, int y, int z, uint32_t seed) {
-  uint32_t h = hashU((uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u ^ seed);
-  return (float)h * (1.0f / 4294967296.0f);
-}
-DEV float vnoise(vec3 p, uint32_t seed) {
-  float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
-  int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  float tx = p.x - fx, ty = p.y - fy, tz = p.z - fz;
-  tx = tx * tx * (3.0f - 2.0f * tx);
-  ty = ty * ty * (3.0f - 2.0f * ty);
-  tz = tz * tz * (3.0f - 2.0f * tz);
-  float c000 = hashLattice(ix, iy, iz, seed), c100 = hashLattice(ix + 1, iy, iz, seed);
-  float c010 = hashLattice(ix, iy + 1, iz, seed), c110 = hashLattice(ix + 1, iy + 1, iz, seed);
-  float c001 = hashLattice(ix, iy, iz + 1, seed), c101 = hashLattice(ix + 1, iy, iz + 1, seed);
-  float c011 = hashLattice(ix, iy + 1, iz + 1, seed), c111 = hashLattice(ix + 1, iy + 1, iz + 1, seed);
-  float x00 = c000 + (c100 - c000) * tx;
-  float x10 = c010 + (c110 - c010) * tx;
-  float x01 = c001 + (c101 - c001) * tx;
-  float x11 = c011 + (c111 - c011) * tx;
-  float y0 = x00 + (x10 - x00) * ty;
-  float y1 = x01 + (x11 - x01) * ty;
-  return y0 + (y1 - y0) * tz;
-}
-DEV float fbm(vec3 p, uint32_t seed, int octaves) {
-  float amp = 0.5f, sum = 0.0f;
-  for (int i = 0; i < octaves; i++) {
-    sum += amp * vnoise(p, seed + (uint32_t)i * 101u);
-    p = 2.03f * p + vec3{11.7f, 5.3f, 7.9f};
-    amp *= 0.5f;
-  }
-  return sum;
-}
-DEV vec3 waterNormal(vec3 x, vec3 ns, vec3 wo, float amp, float freq, int octaves) {
-  const uint32_t SEED = 1337u;
-  const float e = 0.05f;
-  vec3 p = vec3{x.x * freq, 0.0f, x.z * freq};
-  float hx1 = fbm(p + vec3{e, 0.0f, 0.0f}, SEED, octaves), hx0 = fbm(p - vec3{e, 0.0f, 0.0f}, SEED, octaves);
-  float hz1 = fbm(p + vec3{0.0f, 0.0f, e}, SEED, octaves), hz0 = fbm(p - vec3{0.0f, 0.0f, e}, SEED, octaves);
-  float dhdx = (hx1 - hx0) / (2.0f * e);
-  float dhdz = (hz1 - hz0) / (2.0f * e);
-  float side = ns.y >= 0.0f ? 1.0f : -1.0f;
-  vec3 n = normalize(vec3{-amp * dhdx, 1.0f, -amp * dhdz}) * side;
-  if (dot(n, wo) <= 1e-4f) return ns;
-  return n;
-}
-
-// ---------------------------------------------------------------------------
-// flames: device/volume.cuh:27-153. Bounds are the reference's upright
-// cylinder or an axis-aligned box; the march uses the reference's 32 fixed
-// steps, or ceil(length/step) equal steps when a step length is set.
-// ---------------------------------------------------------------------------
-
-DEV void flameField(const DevFlame& fl, vec3 p, float& sigma, vec3& emission) {
-  sigma = 0.0f;
-  emission = v3(0.0f);
-  float h = (p.y - fl.base[1]) / fl.height;
-  if (h <= 0.0f || h >= 1.0f) return;
-  float rx = p.x - fl.base[0], rz = p.z - fl.base[2];
-  float d = sqrtf(rx * rx + rz * rz) / fl.radius;
-  float prof = 3.2f * sqrtf(h) * powf(1.0f - h, 1.55f);
-  const float a = d / fmaxf(prof, 1e-3f);
-  // The noise is a sum of non-negative octaves, and adding a non-negative
-  // term never lowers a float sum, so every partial sum bounds dd from
-  // below: once a + (partial - 0.45) >= 1.01, dd >= 1 and the mask below is
-  // 0 whatever the remaining octaves (0.01 of margin for rounding). The
-  // remaining octaves are skipped there; the result is the same zero field.
-  if (a >= 1.46f) return;
-  vec3 q = vec3{rx / fl.radius, (p.y - fl.base[1]) / fl.radius * 0.32f, rz / fl.radius};
-  // fbm(q * noiseScale, seed, octaves), the same operations in the same order
-  vec3 pn = q * fl.noiseScale;
-  float amp = 0.5f, n = 0.0f;
-  for (int i = 0; i < fl.octaves; i++) {
-    n += amp * vnoise(pn, fl.seed + (uint32_t)i * 101u);
-    if (a + (n - 0.45f) >= 1.01f) return;
-    pn = 2.03f * pn + vec3{11.7f, 5.3f, 7.9f};
-    amp *= 0.5f;
-  }
-  float dd = a + (n - 0.45f) * 1.0f;
-  float mask = smoothstepf(1.0f, 0.62f, dd) * smoothstepf(0.0f, 0.12f, h);
-  if (mask <= 0.0f) return;
-  sigma = fl.sigma * mask;
-  float heat = clampf((0.88f - dd) * (1.25f - h), 0.0f, 1.0f);
-  vec3 col = vec3{fminf(heat * 2.6f, 1.0f), fminf(heat * heat * 2.1f, 1.0f),
-                  fminf(heat * heat * heat * 1.7f, 1.0f)};
-  emission = col * (fl.intensity * mask);
-}
-
-DEV bool clipFlame(const DevFlame& fl, vec3 o, vec3 d, float& t0, float& t1) {
-  if (fabsf(d.y) < 1e-8f) {
-    if (o.y < fl.base[1] || o.y > fl.base[1] + fl.height) return false;
-  } else {
-    float ta = (fl.base[1] - o.y) / d.y;
-    float tb = (fl.base[1] + fl.height - o.y) / d.y;
-    if (ta > tb) { float tmp = ta; ta = tb; tb = tmp; }
-    t0 = fmaxf(t0, ta);
-    t1 = fminf(t1, tb);
-  }
-  float ox = o.x - fl.base[0], oz = o.z - fl.base[2];
-  if (fl.bounds == 1) {
-    // box |x|,|z| <= radius: two more slabs
-    float r = fl.radius;
-    if (fabsf(d.x) < 1e-8f) {
-      if (ox < -r || ox > r) return false;
-    } else {
-      float ta = (-r - ox) / d.x, tb = (r - ox) / d.x;
-      if (ta > tb) { float tmp = ta; ta = tb; tb = tmp; }
-      t0 = fmaxf(t0, ta);
-      t1 = fminf(t1, tb);
-    }
-    if (fabsf(d.z) < 1e-8f) {
-      if (oz < -r || oz > r) return false;
-    } else {
-      float ta = (-r - oz) / d.z, tb = (r - oz) / d.z;
-      if (ta > tb) { float tmp = ta; ta = tb; tb = tmp; }
-      t0 = fmaxf(t0, ta);
-      t1 = fminf(t1, tb);
-    }
-    return t0 < t1;
-  }
-  float a = d.x * d.x + d.z * d.z;
-  if (a < 1e-12f) {
-    if (ox * ox + oz * oz > fl.radius * fl.radius) return false;
-  } else {
-    float b = ox * d.x + oz * d.z;
-    float c = ox * ox + oz * oz - fl.radius * fl.radius;
-    float disc = b * b - a * c;
-    if (disc <= 0.0f) return false;
-    float s = sqrtf(disc);
-    t0 = fmaxf(t0, (-b - s) / a);
-    t1 = fminf(t1, (-b + s) / a);
-  }
-  return t0 < t1;
-}
-
-DEV int flameSteps(const DevFlame& fl, float len) {
-  if (fl.step <= 0.0f) return 32;
-  float n = ceilf(len / fl.step);
-  return (int)fminf(fmaxf(n, 1.0f), 65536.0f);
-}
-
-DEV vec3 marchFlames(const LaunchParams& P, vec3 o, vec3 d, float tEnd, Pcg& rng, vec3& trans) {
-  vec3 Lv = v3(0.0f);
-  for (int i = 0; i < P.numFlames; i++) {
-    const DevFlame& fl = P.flames[i];
-    float t0 = 1e-4f, t1 = tEnd;
-    if (!clipFlame(fl, o, d, t0, t1)) continue;
-    int steps = flameSteps(fl, t1 - t0);
-    float dt = (t1 - t0) / steps;
-    float jitter = rng.rnd();
-    float tl = 1.0f;
-    vec3 acc = v3(0.0f);
-    for (int k = 0; k < steps; k++) {
-      vec3 p = o + (t0 + (k + jitter) * dt) * d;
-      float sigma;
-      vec3 eps;
-      flameField(fl, p, sigma, eps);
-      acc += (tl * dt) * eps;
-      tl *= expf(-sigma * dt);
-      if (tl < 1e-4f) break;
-    }
-    Lv += trans * acc;
-    trans *= tl;
-  }
-  return Lv;
-}
-
-DEV float flameTransmittance(const LaunchParams& P, vec3 o, vec3 d, float tEnd, Pcg& rng, int skip) {
-  float tr = 1.0f;
-  for (int i = 0; i < P.numFlames; i++) {
-    if (i == skip) continue;
-    const DevFlame& fl = P.flames[i];
-    float t0 = 1e-4f, t1 = tEnd;
-    if (!clipFlame(fl, o, d, t0, t1)) continue;
-    int steps = flameSteps(fl, t1 - t0);
-    float dt = (t1 - t0) / steps;
-    float jitter = rng.rnd();
-    float tl = 1.0f;
-    for (int k = 0; k < steps; k++) {
-      vec3 p = o + (t0 + (k + jitter) * dt) * d;
-      float sigma;
-      vec3 eps;
-      flameField(fl, p, sigma, eps);
-      tl *= expf(-sigma * dt);
-      if (tl < 1e-4f) break;
-    }
-    tr *= tl;
-  }
-  return tr;
-}
-
-// ---------------------------------------------------------------------------
-// environment light: device/env_light.cuh:17-116
-// ---------------------------------------------------------------------------
-
-DEV vec2 envDirToUv(const LaunchParams& P, vec3 dir) {
-  float u = (atan2f(dir.z, dir.x) - P.envRotate + kPi) * (0.5f * kInvPi);
-  u -= floorf(u);
-  float v = acosf(clampf(dir.y, -1.0f, 1.0f)) * kInvPi;
-  return vec2{u, v};
-}
-DEV vec3 envEval(const LaunchParams& P, vec3 dir) {
-  vec2 uv = envDirToUv(P, dir);
-  float4 c = bilinear(P.envTex, P.envW, P.envH, uv.x, uv.y, false);
-  return vec3{c.x, c.y, c.z} * P.envIntensity;
-}
-DEV int findInterval(const float* cdf, int n, float u) {
-  int lo = 0, hi = n - 1;
-  while (lo + 1 < hi) {
-    int mid = (lo + hi) >> 1;
-    if (cdf[mid] <= u) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-struct LightSample {
-  vec3 wi;
-  float dist;
-  vec3 Li;
-  float pdf;
-  bool isDelta;
-  bool valid;
-};
-
-DEV LightSample sampleEnv(const LaunchParams& P, Pcg& rng) {
-  LightSample s;
-  s.valid = false;
-  s.isDelta = false;
-  s.pdf = 0.0f;
-  s.dist = 1e16f;
-  vec2 rn = rng.rnd2();
-  if (!P.envImportance) {
-    s.wi = uniformSphere(rn);
-    s.pdf = 0.25f * kInvPi;
-    s.Li = envEval(P, s.wi);
-    s.valid = true;
-    return s;
-  }
-  int W = P.envW, H = P.envH;
-  int row = findInterval(P.margCdf, H + 1, rn.x);
-  float dm = P.margCdf[row + 1] - P.margCdf[row];
-  float dv = dm > 0.0f ? (rn.x - P.margCdf[row]) / dm : 0.5f;
-  float v = (row + dv) / H;
-  const float* cdf = P.condCdf + (size_t)row * (W + 1);
-  int col = findInterval(cdf, W + 1, rn.y);
-  float dc = cdf[col + 1] - cdf[col];
-  float du = dc > 0.0f ? (rn.y - cdf[col]) / dc : 0.5f;
-  float u = (col + du) / W;
-  float theta = v * kPi;
-  float sinT = sinf(theta);
-  if (sinT < 1e-6f || dm <= 0.0f || dc <= 0.0f) return s;
-  float phi = (2.0f * kPi * u - kPi) + P.envRotate;
-  s.wi = vec3{sinT * cosf(phi), cosf(theta), sinT * sinf(phi)};
-  s.pdf = dm * dc * W * H / (2.0f * kPi * kPi * sinT);
-  s.Li = envEval(P, s.wi);
-  s.valid = s.pdf > 0.0f;
-  return s;
-}
-
-DEV float envPdf(const LaunchParams& P, vec3 dir) {
-  if (!P.envImportance) return 0.25f * kInvPi;
-  float sinT = sqrtf(fmaxf(0.0f, 1.0f - dir.y * dir.y));
-  if (!(sinT >= 1e-6f)) return 0.0f;  // also rejects NaN directions
-  vec2 uv = envDirToUv(P, dir);
-  int W = P.envW, H = P.envH;
-  // fmaxf drops NaN, so the texel index stays inside the image
-  int col = (int)fminf(fmaxf(uv.x * W, 0.0f), (float)(W - 1));
-  int row = (int)fminf(fmaxf(uv.y * H, 0.0f), (float)(H - 1));
-  float dm = P.margCdf[row + 1] - P.margCdf[row];
-  const float* cdf = P.condCdf + (size_t)row * (W + 1);
-  float dc = cdf[col + 1] - cdf[col];
-  return dm * dc * W * H / (2.0f * kPi * kPi * sinT);
-}
-
-// ---------------------------------------------------------------------------
-// BSDFs: device/bsdf.cuh:27-424
-// ---------------------------------------------------------------------------
-
-DEV bool bsdfIsDelta(const DevMaterial& m) {
-  if (m.kind == MT_WATER) return true;
-  if (m.kind == MT_DIELECTRIC || m.kind == MT_METAL) return m.roughness < 1e-3f;
-  return false;
-}
-DEV float ggxD(vec3 h, float alpha) {
-  float c2 = h.z * h.z;
-  float k = alpha * alpha * c2 + (1.0f - c2);
-  return alpha * alpha / (kPi * k * k);
-}
-DEV float ggxLambda(vec3 w, float alpha) {
-  float c2 = w.z * w.z;
-  if (c2 >= 1.0f) return 0.0f;
-  float t2 = (1.0f - c2) / c2;
-  return 0.5f * (-1.0f + sqrtf(1.0f + alpha * alpha * t2));
-}
-DEV float ggxG1(vec3 w, float alpha) { return 1.0f / (1.0f + ggxLambda(w, alpha)); }
-DEV float ggxG(vec3 wo, vec3 wi, float alpha) {
-  return 1.0f / (1.0f + ggxLambda(wo, alpha) + ggxLambda(wi, alpha));
-}
-// Heitz 2018 visible-normal sampling
-DEV vec3 ggxSampleVndf(vec3 wo, float alpha, vec2 u) {
-  vec3 vh = normalize(vec3{alpha * wo.x, alpha * wo.y, wo.z});
-  float l2 = vh.x * vh.x + vh.y * vh.y;
-  vec3 T1 = l2 > 0.0f ? vec3{-vh.y, vh.x, 0.0f} * (1.0f / sqrtf(l2)) : vec3{1.0f, 0.0f, 0.0f};
-  vec3 T2 = cross(vh, T1);
-  float r = sqrtf(u.x);
-  float phi = 2.0f * kPi * u.y;
-  float t1 = r * cosf(phi);
-  float t2 = r * sinf(phi);
-  float s = 0.5f * (1.0f + vh.z);
-  t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
-  vec3 nh = t1 * T1 + t2 * T2 + sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2)) * vh;
-  return normalize(vec3{alpha * nh.x, alpha * nh.y, fmaxf(1e-6f, nh.z)});
-}
-DEV float ggxPdf(vec3 wo, vec3 wi, float alpha) {
-  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-  vec3 h = normalize(wo + wi);
-  return ggxG1(wo, alpha) * ggxD(h, alpha) / (4.0f * wo.z);
-}
-DEV float fresnelDielectric(float cosI, float eta, float f0) {
-  float k = 1.0f - eta * eta * (1.0f - cosI * cosI);
-  if (k <= 0.0f) return 1.0f;
-  return schlick(eta < 1.0f ? cosI : sqrtf(k), f0);
-}
-DEV float ggxPdfTransmit(vec3 lo, vec3 li, vec3 h, float eta, float alpha) {
-  float loh = dot(lo, h), lih = dot(li, h);
-  if (lo.z <= 0.0f || li.z >= 0.0f || loh <= 0.0f || lih >= 0.0f) return 0.0f;
-  float denom = eta * loh + lih;
-  if (fabsf(denom) < 1e-6f) return 0.0f;
-  return ggxG1(lo, alpha) * ggxD(h, alpha) * loh / lo.z * fabsf(lih) / (denom * denom);
-}
-
-struct PlasticTerms {
-  float fSpec;
-  vec3 fDiff;
-  float pSpec, pDiff;
-};
-DEV PlasticTerms plasticTerms(const DevMaterial& m, vec3 albedo, vec3 lo, vec3 li) {
-  PlasticTerms t;
-  float rough = fmaxf(m.roughness, 1e-3f);
-  float alpha = rough * rough;
-  float f0 = (m.ior - 1.0f) / (m.ior + 1.0f);
-  f0 = f0 * f0;
-  vec3 h = normalize(lo + li);
-  float F = schlick(dot(lo, h), f0);
-  t.fSpec = F * ggxD(h, alpha) * ggxG(lo, li, alpha) / (4.0f * lo.z * li.z);
-  t.fDiff = albedo * (kInvPi * (1.0f - schlick(lo.z, f0)) * (1.0f - schlick(li.z, f0)));
-  t.pSpec = ggxPdf(lo, li, alpha);
-  t.pDiff = li.z * kInvPi;
-  return t;
-}
-
-DEV vec3 bsdfEval(const DevMaterial& m, vec3 albedo, vec3 wo, vec3 wi, vec3 n, float eta) {
-  float ci = dot(wi, n);
-  float co = dot(wo, n);
-  if (co <= 0.0f) return v3(0.0f);
-  bool roughGlass = m.kind == MT_DIELECTRIC && m.roughness >= 1e-3f;
-  if (ci <= 0.0f) {
-    if (!roughGlass || ci == 0.0f) return v3(0.0f);
-    float alpha = m.roughness * m.roughness;
-    Frame fr(n);
-    vec3 lo = fr.toLocal(wo), li = fr.toLocal(wi);
-    vec3 h = normalize(eta * lo + li);
-    if (h.z < 0.0f) h = -h;
-    float loh = dot(lo, h), lih = dot(li, h);
-    if (loh <= 0.0f || lih >= 0.0f) return v3(0.0f);
-    float denom = eta * loh + lih;
-    if (fabsf(denom) < 1e-6f) return v3(0.0f);
-    float f0 = (1.0f - eta) / (1.0f + eta);
-    f0 = f0 * f0;
-    float F = fresnelDielectric(loh, eta, f0);
-    float ft = eta * eta * (1.0f - F) * ggxD(h, alpha) * ggxG(lo, li, alpha) *
-               fabsf(loh * lih) / (lo.z * fabsf(li.z) * denom * denom);
-    return v3(ft);
-  }
-  if (m.kind == MT_LAMBERT) return albedo * kInvPi;
-  if (m.kind == MT_PLASTIC) {
-    Frame fr(n);
-    PlasticTerms t = plasticTerms(m, albedo, fr.toLocal(wo), fr.toLocal(wi));
-    return v3(t.fSpec) + t.fDiff;
-  }
-  if (m.kind == MT_METAL && m.roughness >= 1e-3f) {
-    float alpha = m.roughness * m.roughness;
-    Frame fr(n);
-    vec3 lo = fr.toLocal(wo), li = fr.toLocal(wi);
-    vec3 h = normalize(lo + li);
-    vec3 F = schlick3(dot(lo, h), albedo);
-    return F * (ggxD(h, alpha) * ggxG(lo, li, alpha) / (4.0f * lo.z * li.z));
-  }
-  if (roughGlass) {
-    float alpha = m.roughness * m.roughness;
-    Frame fr(n);
-    vec3 lo = fr.toLocal(wo), li = fr.toLocal(wi);
-    vec3 h = normalize(lo + li);
-    float f0 = (1.0f - eta) / (1.0f + eta);
-    f0 = f0 * f0;
-    float F = fresnelDielectric(dot(lo, h), eta, f0);
-    return v3(F * ggxD(h, alpha) * ggxG(lo, li, alpha) / (4.0f * lo.z * li.z));
-  }
-  return v3(0.0f);
-}
-
-DEV float bsdfPdf(const DevMaterial& m, vec3 wo, vec3 wi, vec3 n, float eta) {
-  float ci = dot(wi, n);
-  float co = dot(wo, n);
-  if (co <= 0.0f) return 0.0f;
-  bool roughGlass = m.kind == MT_DIELECTRIC && m.roughness >= 1e-3f;
-  if (ci <= 0.0f) {
-    if (!roughGlass || ci == 0.0f) return 0.0f;
-    float alpha = m.roughness * m.roughness;
-    Frame fr(n);
-    vec3 lo = fr.toLocal(wo), li = fr.toLocal(wi);
-    vec3 h = normalize(eta * lo + li);
-    if (h.z < 0.0f) h = -h;
-    float loh = dot(lo, h);
-    if (loh <= 0.0f) return 0.0f;
-    float f0 = (1.0f - eta) / (1.0f + eta);
-    f0 = f0 * f0;
-    float F = fresnelDielectric(loh, eta, f0);
-    return (1.0f - F) * ggxPdfTransmit(lo, li, h, eta, alpha);
-  }
-  if (m.kind == MT_LAMBERT) return ci * kInvPi;
-  if (m.kind == MT_PLASTIC) {
-    Frame fr(n);
-    PlasticTerms t = plasticTerms(m, v3(0.0f), fr.toLocal(wo), fr.toLocal(wi));
-    return 0.5f * (t.pSpec + t.pDiff);
-  }
-  if (m.kind == MT_METAL && m.roughness >= 1e-3f) {
-    Frame fr(n);
-    return ggxPdf(fr.toLocal(wo), fr.toLocal(wi), m.roughness * m.roughness);
-  }
-  if (roughGlass) {
-    float alpha = m.roughness * m.roughness;
-    Frame fr(n);
-    vec3 lo = fr.toLocal(wo), li = fr.toLocal(wi);
-    vec3 h = normalize(lo + li);
-    float f0 = (1.0f - eta) / (1.0f + eta);
-    f0 = f0 * f0;
-    float F = fresnelDielectric(dot(lo, h), eta, f0);
-    return F * ggxPdf(lo, li, alpha);
-  }
-  return 0.0f;
-}
-
-struct BsdfSample {
-  vec3 wi;
-  vec3 weight;
-  float pdf;
-  bool isDelta;
-  bool valid;
-};
-
-DEV BsdfSample bsdfSample(const DevMaterial& m, vec3 albedo, vec3 rayDir, vec3 n, bool frontface,
-                          float etaExt, Pcg& rng) {
-  BsdfSample s;
-  s.valid = false;
-  s.pdf = 0.0f;
-  s.isDelta = false;
-  s.wi = v3(0.0f);
-  s.weight = v3(0.0f);
-  vec3 wo = -rayDir;
-  switch (m.kind) {
-    case MT_LAMBERT: {
-      Frame fr(n);
-      vec3 li = cosineHemisphere(rng.rnd2());
-      s.wi = fr.toWorld(li);
-      s.pdf = fmaxf(li.z, 1e-8f) * kInvPi;
-      s.weight = albedo;
-      s.valid = li.z > 0.0f;
-      return s;
-    }
-    case MT_METAL: {
-      if (m.roughness < 1e-3f) {
-        s.wi = normalize(reflect(rayDir, n));
-        if (dot(s.wi, n) <= 0.0f) return s;
-        s.weight = schlick3(dot(s.wi, n), albedo);
-        s.isDelta = true;
-        s.valid = true;
-      } else {
-        float alpha = m.roughness * m.roughness;
-        Frame fr(n);
-        vec3 lo = fr.toLocal(wo);
-        if (lo.z <= 0.0f) return s;
-        vec3 h = ggxSampleVndf(lo, alpha, rng.rnd2());
-        vec3 li = reflect(-lo, h);
-        if (li.z <= 0.0f) return s;
-        vec3 F = schlick3(dot(lo, h), albedo);
-        s.wi = fr.toWorld(li);
-        s.weight = F * (ggxG(lo, li, alpha) / ggxG1(lo, alpha));
-        s.pdf = ggxPdf(lo, li, alpha);
-        s.valid = s.pdf > 0.0f;
-      }
-      return s;
-    }
-    case MT_WATER:
-    case MT_DIELECTRIC: {
-      vec3 refr;
-      float eta = frontface ? etaExt / m.ior : m.ior / etaExt;
-      float f0 = (1.0f - eta) / (1.0f + eta);
-      f0 = f0 * f0;
-      if (m.kind == MT_WATER || m.roughness < 1e-3f) {
-        s.isDelta = true;
-        if (refract(rayDir, n, eta, refr)) {
-          float c = eta < 1.0f ? -dot(rayDir, n) : -dot(refr, n);
-          float R = schlick(c, f0);
-          if (rng.rnd() < R) s.wi = normalize(reflect(rayDir, n));
-          else s.wi = normalize(refr);
-        } else {
-          s.wi = normalize(reflect(rayDir, n));
-        }
-        s.weight = v3(1.0f);
-        s.valid = true;
-        return s;
-      }
-      float alpha = m.roughness * m.roughness;
-      Frame fr(n);
-      vec3 lo = fr.toLocal(wo);
-      if (lo.z <= 0.0f) return s;
-      vec3 h = ggxSampleVndf(lo, alpha, rng.rnd2());
-      float loh = dot(lo, h);
-      if (loh <= 0.0f) return s;
-      vec3 li;
-      if (!refract(-lo, h, eta, refr)) {
-        li = reflect(-lo, h);
-        if (li.z <= 0.0f) return s;
-        s.pdf = ggxPdf(lo, li, alpha);
-      } else {
-        float c = eta < 1.0f ? loh : -dot(refr, h);
-        float R = schlick(c, f0);
-        if (rng.rnd() < R) {
-          li = reflect(-lo, h);
-          if (li.z <= 0.0f) return s;
-          s.pdf = R * ggxPdf(lo, li, alpha);
-        } else {
-          li = normalize(refr);
-          if (li.z >= 0.0f) return s;
-          s.pdf = (1.0f - R) * ggxPdfTransmit(lo, li, h, eta, alpha);
-        }
-      }
-      s.wi = fr.toWorld(li);
-      float wgt = ggxG(lo, li, alpha) / ggxG1(lo, alpha);
-      if (li.z < 0.0f) wgt *= eta * eta;
-      s.weight = v3(wgt);
-      s.valid = s.pdf > 0.0f;
-      return s;
-    }
-    case MT_PLASTIC: {
-      Frame fr(n);
-      vec3 lo = fr.toLocal(wo);
-      if (lo.z <= 0.0f) return s;
-      bool coat = rng.rnd() < 0.5f;
-      vec3 li;
-      if (coat) {
-        float rough = fmaxf(m.roughness, 1e-3f);
-        vec3 h = ggxSampleVndf(lo, rough * rough, rng.rnd2());
-        li = reflect(-lo, h);
-      } else {
-        li = cosineHemisphere(rng.rnd2());
-      }
-      if (li.z <= 0.0f) return s;
-      PlasticTerms t = plasticTerms(m, albedo, lo, li);
-      s.pdf = 0.5f * (t.pSpec + t.pDiff);
-      if (s.pdf <= 0.0f) return s;
-      s.wi = fr.toWorld(li);
-      s.weight = (v3(t.fSpec) + t.fDiff) * (li.z / s.pdf);
-      s.valid = true;
-      return s;
-    }
-    default:
-      return s;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// lights: device/light_sample.cuh:24-226
-// ---------------------------------------------------------------------------
-
-DEV LightSample sampleLight(const LaunchParams& P, const DevLight& lt, vec3 x, Pcg& rng) {
-  LightSample s;
-  s.valid = false;
-  s.isDelta = false;
-  s.pdf = 0.0f;
-  s.wi = v3(0.0f);
-  s.dist = 0.0f;
-  s.Li = v3(0.0f);
-  switch (lt.kind) {
-    case LT_RECT:
-    case LT_DISK: {
-      vec3 p;
-      float tu, tv;
-      vec3 lu = ld3(lt.u), lv = ld3(lt.v);
-      if (lt.kind == LT_RECT) {
-        float a = 2.0f * rng.rnd() - 1.0f;
-        float b = 2.0f * rng.rnd() - 1.0f;
-        p = ld3(lt.p) + a * lu + b * lv;
-        tu = (a + 1.0f) * 0.5f;
-        tv = (b + 1.0f) * 0.5f;
-      } else {
-        vec2 d = concentricDisk(rng.rnd2());
-        p = ld3(lt.p) + d.x * lu + d.y * lv;
-        tu = (atan2f(d.y, d.x) + kPi) / (2.0f * kPi);
-        tv = sqrtf(d.x * d.x + d.y * d.y);
-      }
-      vec3 to = p - x;
-      float d2 = len2(to);
-      if (d2 < 1e-10f) return s;
-      float dist = sqrtf(d2);
-      vec3 wi = to / dist;
-      float cosL = -dot(wi, ld3(lt.n));
-      if (!lt.twoSided && cosL <= 0.0f) return s;
-      cosL = fabsf(cosL);
-      if (cosL < 1e-6f) return s;
-      s.wi = wi;
-      s.dist = dist;
-      s.Li = ld3(lt.L);
-      if (lt.texId >= 0) {
-        float4 c = evalTexture(P, lt.texId, tu, tv);
-        s.Li = ld3(lt.L) * vec3{c.x, c.y, c.z};
-      }
-      s.pdf = d2 / (cosL * lt.area);
-      s.valid = true;
-      return s;
-    }
-    case LT_SPHERE: {
-      vec3 to = ld3(lt.p) - x;
-      float d2 = len2(to);
-      float r2 = lt.radius * lt.radius;
-      if (d2 <= r2 * 1.0001f) return s;
-      float dist = sqrtf(d2);
-      vec3 wc = to / dist;
-      float sin2Max = r2 / d2;
-      float cosMax = sqrtf(fmaxf(0.0f, 1.0f - sin2Max));
-      vec2 u = rng.rnd2();
-      float cosT = 1.0f - u.x * (1.0f - cosMax);
-      float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
-      float phi = 2.0f * kPi * u.y;
-      Frame fr(wc);
-      s.wi = fr.toWorld(vec3{sinT * cosf(phi), sinT * sinf(phi), cosT});
-      float b = dot(s.wi, to);
-      float det = fmaxf(0.0f, b * b - d2 + r2);
-      s.dist = b - sqrtf(det);
-      s.Li = ld3(lt.L);
-      s.pdf = 1.0f / (2.0f * kPi * (1.0f - cosMax));
-      s.valid = true;
-      return s;
-    }
-    case LT_MESH: {
-      float uc = rng.rnd();
-      const float* cdf = P.lightCdf + lt.mCdfOffset;
-      int lo = 0, hi = lt.mNumTris;
-      while (lo + 1 < hi) {
-        int mid = (lo + hi) >> 1;
-        if (cdf[mid] <= uc) lo = mid; else hi = mid;
-      }
-      const uint32_t* tri = P.indices + lt.mIdxOffset + 3 * lo;
-      uint32_t i0 = tri[0], i1 = tri[1], i2 = tri[2];
-      const float* wp = P.lightPos + 3 * lt.mPosOffset;
-      vec3 p0 = ld3i(wp, i0), p1 = ld3i(wp, i1), p2 = ld3i(wp, i2);
-      vec2 r = rng.rnd2();
-      float su = sqrtf(r.x);
-      float bx = su * (1.0f - r.y);
-      float by = su * r.y;
-      vec3 p = p0 * (1.0f - bx - by) + p1 * bx + p2 * by;
-      vec3 ngU = cross(p1 - p0, p2 - p0);
-      float ngL2 = len2(ngU);
-      if (ngL2 < 1e-20f) return s;
-      vec3 ng = ngU * (lt.mNgSign / sqrtf(ngL2));
-      vec3 to = p - x;
-      float d2 = len2(to);
-      if (d2 < 1e-10f) return s;
-      float dist = sqrtf(d2);
-      vec3 wi = to / dist;
-      float cosL = -dot(wi, ng);
-      if (!lt.twoSided && cosL <= 0.0f) return s;
-      cosL = fabsf(cosL);
-      if (cosL < 1e-6f) return s;
-      s.wi = wi;
-      s.dist = dist;
-      s.Li = ld3(lt.L);
-      if (lt.texId >= 0) {
-        float tu, tv;
-        if (lt.mUvOffset >= 0) {
-          const float* uv = P.uvs + 2 * lt.mUvOffset;
-          vec2 t0 = vec2{uv[2 * i0], uv[2 * i0 + 1]}, t1 = vec2{uv[2 * i1], uv[2 * i1 + 1]},
-               t2 = vec2{uv[2 * i2], uv[2 * i2 + 1]};
-          vec2 t = t0 * (1.0f - bx - by) + t1 * bx + t2 * by;
-          tu = t.x;
-          tv = t.y;
-        } else {
-          tu = bx;
-          tv = by;
-        }
-        float4 c = evalTexture(P, lt.texId, tu, tv);
-        s.Li = ld3(lt.L) * vec3{c.x, c.y, c.z};
-      }
-      s.pdf = d2 / (cosL * lt.area);
-      s.valid = true;
-      return s;
-    }
-    case LT_POINT: {
-      s.isDelta = true;
-      vec3 to = ld3(lt.p) - x;
-      float d2c = len2(to);
-      if (d2c < 1e-10f) return s;
-      vec3 wc = to / sqrtf(d2c);
-      vec2 d = concentricDisk(rng.rnd2()) * lt.radius;
-      Frame fr(wc);
-      vec3 p = ld3(lt.p) + d.x * fr.t + d.y * fr.b;
-      vec3 top = p - x;
-      float d2 = len2(top);
-      float dist = sqrtf(d2);
-      s.wi = top / dist;
-      s.dist = fmaxf(dist - 1e-4f, 1e-4f);
-      s.Li = ld3(lt.L) / d2;
-      s.pdf = 1.0f;
-      s.valid = true;
-      return s;
-    }
-    case LT_DISTANT: {
-      s.isDelta = true;
-      s.wi = -ld3(lt.dir);
-      s.dist = 1e16f;
-      s.Li = ld3(lt.L);
-      s.pdf = 1.0f;
-      s.valid = true;
-      return s;
-    }
-  }
-  return s;
-}
-
-DEV float lightPdfSolidAngle(const DevLight& lt, vec3 from, vec3 hitP, vec3 nHit) {
-  switch (lt.kind) {
-    case LT_RECT:
-    case LT_DISK:
-    case LT_MESH: {
-      vec3 to = hitP - from;
-      float d2 = len2(to);
-      if (d2 < 1e-10f) return 0.0f;
-      float dist = sqrtf(d2);
-      float cosL = fabsf(dot(to / dist, lt.kind == LT_MESH ? nHit : ld3(lt.n)));
-      if (cosL < 1e-6f) return 0.0f;
-      return d2 / (cosL * lt.area);
-    }
-    case LT_SPHERE: {
-      vec3 to = ld3(lt.p) - from;
-      float d2 = len2(to);
-      float r2 = lt.radius * lt.radius;
-      if (d2 <= r2 * 1.0001f) return 0.0f;
-      float cosMax = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
-      return 1.0f / (2.0f * kPi * (1.0f - cosMax));
-    }
-    default:
-      return 0.0f;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// geometry. Contraction is disabled in these functions so that the CPU
-// oracle, compiled without FMA contraction, reproduces every t bit.
-// ---------------------------------------------------------------------------
-
-// Canonical quadrics in object space (device/intersect.cuh:37-162): up to
-// two roots in (tmin, tmax); `which` selects root 0 or 1 of the pair.
-DEV int quadricRoots(int kind, vec3 o, vec3 d, float tmin, float tmax, float t[2]) {
-#pragma clang fp contract(off)
-  int n = 0;
-  switch (kind) {
-    case GK_SPHERE: {
-      // products written out here: the pragma above governs only
-      // expressions parsed in this function, not inlined helpers like dot()
-      float a = d.x * d.x + d.y * d.y + d.z * d.z;
-      float b = o.x * d.x + o.y * d.y + o.z * d.z;
-      float c = (o.x * o.x + o.y * o.y + o.z * o.z) - 1.0f;
-      float disc = b * b - a * c;
-      if (disc <= 0.0f) return 0;
-      float s = sqrtf(disc);
-      float r0 = (-b - s) / a, r1 = (-b + s) / a;
-      if (r0 > tmin && r0 < tmax) t[n++] = r0;
-      if (r1 > tmin && r1 < tmax) t[n++] = r1;
-      return n;
-    }
-    case GK_RECT:
-    case GK_DISK: {
-      if (d.y == 0.0f) return 0;
-      float tt = -o.y / d.y;
-      if (tt <= tmin || tt >= tmax) return 0;
-      float x = o.x + tt * d.x, z = o.z + tt * d.z;
-      if (kind == GK_RECT) {
-        if (fabsf(x) > 1.0f || fabsf(z) > 1.0f) return 0;
-      } else {
-        if (sqrtf(x * x + z * z) > 1.0f) return 0;
-      }
-      t[0] = tt;
-      return 1;
-    }
-    case GK_CYLINDER: {
-      float a = d.x * d.x + d.z * d.z;
-      if (a == 0.0f) return 0;
-      float b = o.x * d.x + o.z * d.z;
-      float c = o.x * o.x + o.z * o.z - 1.0f;
-      float disc = b * b - a * c;
-      if (disc <= 0.0f) return 0;
-      float s = sqrtf(disc);
-      float r[2] = {(-b - s) / a, (-b + s) / a};
-      for (int i = 0; i < 2; i++)
-        if (r[i] > tmin && r[i] < tmax && fabsf(o.y + r[i] * d.y) <= 1.0f) t[n++] = r[i];
-      return n;
-    }
-    case GK_PARABOLA: {
-      float A = 0.5f * (d.x * d.x + d.z * d.z);
-      float B = o.x * d.x + o.z * d.z - d.y;
-      float C = 0.5f * (o.x * o.x + o.z * o.z) - o.y;
-      float r[2];
-      int m = 0;
-      if (fabsf(A) < 1e-12f) {
-        if (B == 0.0f) return 0;
-        r[0] = -C / B;
-        m = 1;
-      } else {
-        float disc = B * B - 4.0f * A * C;
-        if (disc <= 0.0f) return 0;
-        float s = sqrtf(disc);
-        r[0] = (-B - s) / (2.0f * A);
-        r[1] = (-B + s) / (2.0f * A);
-        m = 2;
-      }
-      for (int i = 0; i < m; i++) {
-        if (r[i] > tmin && r[i] < tmax) {
-          float px = o.x + r[i] * d.x, pz = o.z + r[i] * d.z;
-          if (px * px + pz * pz <= 1.0f) t[n++] = r[i];
-        }
-      }
-      return n;
-    }
-  }
-  return 0;
-}
-
-// Object-space normal (canonical front side) and uv at a quadric hit
-// (device/intersect.cuh:382-500).
-DEV void quadricSurface(int kind, vec3 p, vec3& n, float& u, float& v) {
-  switch (kind) {
-    case GK_SPHERE:
-      n = normalize(p);
-      u = (atan2f(n.z, n.x) + kPi) / (2.0f * kPi);
-      v = (asinf(clampf(n.y, -1.0f, 1.0f)) + kPi / 2.0f) / kPi;
-      return;
-    case GK_RECT:
-      n = vec3{0.0f, 1.0f, 0.0f};
-      u = (p.x + 1.0f) * 0.5f;
-      v = (p.z + 1.0f) * 0.5f;
-      return;
-    case GK_DISK:
-      n = vec3{0.0f, 1.0f, 0.0f};
-      u = (atan2f(p.z, p.x) + kPi) / (2.0f * kPi);
-      v = sqrtf(p.x * p.x + p.z * p.z);
-      return;
-    case GK_CYLINDER:
-      n = normalize(vec3{p.x, 0.0f, p.z});
-      u = (atan2f(p.z, p.x) + kPi) / (2.0f * kPi);
-      v = (p.y + 1.0f) * 0.5f;
-      return;
-    default:  // parabola
-      n = normalize(vec3{p.x, -1.0f, p.z});
-      u = (atan2f(p.z, p.x) + kPi) / (2.0f * kPi);
-      v = sqrtf(p.x * p.x + p.z * p.z);
-      return;
-  }
-}
-
-// Per-ray constants of the watertight triangle test (Woop, Benthin, Wald,
-// "Watertight Ray/Triangle Intersection", JCGT 2013).
-struct RayTri {
-  int kx, ky, kz;
-  float Sx, Sy, Sz;
-};
-
-DEV RayTri rayTriSetup(vec3 d) {
-#pragma clang fp contract(off)
-  RayTri r;
-  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-  r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
-  r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
-  float dz = comp(d, r.kz);
-  if (dz < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
-  r.Sx = comp(d, r.kx) / dz;
-  r.Sy = comp(d, r.ky) / dz;
-  r.Sz = 1.0f / dz;
-  return r;
-}
-
-// Returns true and (t, b1, b2) when the triangle is hit in (tmin, tmax).
-// b1, b2 are the weights of v1, v2.
-DEV bool triHit(const RayTri& R, vec3 o, vec3 v0, vec3 v1, vec3 v2, float tmin, float tmax, float& t,
-                float& b1, float& b2) {
-#pragma clang fp contract(off)
-  vec3 A = v0 - o, B = v1 - o, C = v2 - o;
-  float Az0 = comp(A, R.kz), Bz0 = comp(B, R.kz), Cz0 = comp(C, R.kz);
-  float Ax = comp(A, R.kx) - R.Sx * Az0, Ay = comp(A, R.ky) - R.Sy * Az0;
-  float Bx = comp(B, R.kx) - R.Sx * Bz0, By = comp(B, R.ky) - R.Sy * Bz0;
-  float Cx = comp(C, R.kx) - R.Sx * Cz0, Cy = comp(C, R.ky) - R.Sy * Cz0;
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-    U = (float)(CxBy - CyBx);
-    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-    V = (float)(AxCy - AyCx);
-    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-    W = (float)(BxAy - ByAx);
-  }
-  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-  float det = U + V + W;
-  if (det == 0.0f) return false;
-  float Az = R.Sz * Az0, Bz = R.Sz * Bz0, Cz = R.Sz * Cz0;
-  float T = U * Az + V * Bz + W * Cz;
-  float rcp = 1.0f / det;
-  float tt = T * rcp;
-  if (!(tt > tmin && tt < tmax)) return false;
-  t = tt;
-  b1 = V * rcp;
-  b2 = W * rcp;
-  return true;
-}
-
-DEV vec3 xfPoint(const float* m, vec3 p) {
-#pragma clang fp contract(off)
-  return vec3{m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
-              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
-}
-DEV vec3 xfVector(const float* m, vec3 v) {
-#pragma clang fp contract(off)
-  return vec3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
-              m[8] * v.x + m[9] * v.y + m[10] * v.z};
-}
-// inverse-transpose applied to an object-space normal, given the inverse
-DEV vec3 xfNormal(const float* inv, vec3 n) {
-  return vec3{inv[0] * n.x + inv[4] * n.y + inv[8] * n.z, inv[1] * n.x + inv[5] * n.y + inv[9] * n.z,
-              inv[2] * n.x + inv[6] * n.y + inv[10] * n.z};
-}
 
 // Slab test against the 4 child boxes of one node. Returns per-child entry
 // distance, or +inf when missed. tFar is widened by 2*gamma(3) so boxes stay
@@ -1524,7 +432,7 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
 // ---------------------------------------------------------------------------
 
 DEV vec3 evalBackground(const LaunchParams& P, vec3 dir) {
-  if (P.bgKind == BG_ENVMAP) return envEval(P, dir);
+  if (P.bgKind == BG_ENVMAP) return envRadiance(P, dir);
   if (P.bgKind == BG_GRADIENT) {
     float t = 0.5f * (dir.y + 1.0f);
     return lerp(ld3(P.bgA), ld3(P.bgB), t);
@@ -1583,7 +491,7 @@ struct PathState {
 // What pathOnShadow and pathScatter need from the hit being shaded.
 struct SurfState {
   vec3 x, ns, albedo;
-  float etaRel, etaExt;
+  float etaRel;  // outside / inside IOR seen from the incoming ray
   uint32_t matId;
   bool frontface;
   // NEE light sample waiting for its shadow query
@@ -1657,10 +565,11 @@ DEV Pcg pathRng(const LaunchParams& P, const PathState& ps) {
 // BSDF sample, medium stack update, throughput, Russian roulette from depth
 // 4 with q = clamp(max beta, 0.05, 0.95) (device/programs.cu:524-557).
 DEV int pathScatter(const LaunchParams& P, PathState& ps, const SurfState& sf, Pcg& rng) {
-  const DevMaterial& mat = P.materials[sf.matId];
-  BsdfSample bs = bsdfSample(mat, sf.albedo, ps.d, sf.ns, sf.frontface, sf.etaExt, rng);
+  const Closure cl = makeClosure(P.materials[sf.matId], sf.albedo, sf.etaRel);
+  BsdfSample bs = sampleClosure(cl, ps.d, sf.ns, rng);
   if (!bs.valid) return PN_END;
-  if ((mat.kind == MT_WATER || mat.kind == MT_DIELECTRIC) && dot(bs.wi, sf.ns) < 0.0f) {
+  // a refracted ray enters (front face) or leaves (back face) the medium
+  if ((cl.lobe == LK_INTERFACE || cl.lobe == LK_GLASS) && dot(bs.wi, sf.ns) < 0.0f) {
     if (sf.frontface) {
       if (ps.medDepth < kMedMax) medSet(ps.medLo, ps.medHi, ps.medDepth, sf.matId);
       ps.medDepth++;
@@ -1719,12 +628,12 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
     if (maxc(sigma) > 0.0f) ps.beta *= vexp(-sigma * (isHit ? hit.t : 1e4f));
   }
   if (P.numFlames > 0) {
-    vec3 trans = v3(1.0f);
-    vec3 Lv = marchFlames(P, ps.o, ps.d, isHit ? hit.t : 1e16f, rng, trans);
+    vec3 Lv = v3(0.0f);
+    const float T = marchVolumes<true>(P, ps.o, ps.d, isHit ? hit.t : 1e16f, rng, -1, Lv);
     vec3 cv = ps.beta * Lv;
     if (ps.depth >= 1 && P.clampVal > 0.0f) cv = vmin(cv, v3(P.clampVal));
     ps.L += cv;
-    ps.beta *= trans;
+    ps.beta *= v3(T);
   }
   if (!isHit) {
     vec3 bg = evalBackground(P, ps.d);
@@ -1748,9 +657,9 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
   vec3 ns = sp.frontface ? sp.nFront : -sp.nFront;
   sf.x = ps.o + hit.t * ps.d;
   const DevMaterial& mat = P.materials[sf.matId];
-  sf.albedo = materialAlbedo(P, mat, sp.u, sp.v);
+  sf.albedo = surfaceColor(P, mat, sp.u, sp.v);
   if (mat.kind == MT_WATER && mat.waveAmp > 0.0f)
-    ns = waterNormal(sf.x, ns, -ps.d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
+    ns = rippleNormal(sf.x, ns, -ps.d, mat.waveAmp, mat.waveFreq, mat.waveOctaves);
   sf.ns = ns;
 
   if (ps.depth == 0)
@@ -1772,14 +681,15 @@ DEV int pathOnHit(const LaunchParams& P, PathState& ps, bool isHit, const Hit& h
     return PN_END;
   }
 
-  sf.etaExt = 1.0f;
+  // the IOR outside an interface is that of the enclosing medium
+  float etaExt = 1.0f;
   if (mat.kind == MT_DIELECTRIC || mat.kind == MT_WATER) {
     int outer = min(sp.frontface ? ps.medDepth : ps.medDepth - 1, kMedMax);
-    if (outer > 0) sf.etaExt = P.materials[medGet(ps.medLo, ps.medHi, outer - 1)].ior;
+    if (outer > 0) etaExt = P.materials[medGet(ps.medLo, ps.medHi, outer - 1)].ior;
   }
-  sf.etaRel = sp.frontface ? sf.etaExt / mat.ior : mat.ior / sf.etaExt;
+  sf.etaRel = sp.frontface ? etaExt / mat.ior : mat.ior / etaExt;
 
-  if (nStrat > 0 && !bsdfIsDelta(mat)) {
+  if (nStrat > 0 && !materialIsDelta(mat)) {
     int k = min((int)(rng.rnd() * nStrat), nStrat - 1);
     LightSample ls = (k < P.numLights) ? sampleLight(P, P.lights[k], sf.x, rng) : sampleEnv(P, rng);
     float cosS = ls.valid ? dot(ls.wi, ns) : 0.0f;
@@ -1816,17 +726,15 @@ DEV void shadowRay(const SurfState& sf, vec3& so, vec3& wi, float& tmax) {
 // MIS-weighted light contribution of the suspended NEE sample before the
 // shadow query's transmittance is applied (device/programs.cu:481-524).
 // Draws no random numbers.
+// One fused closure evaluation gives both the BSDF value and its density.
 DEV vec3 neeUnshadowed(const LaunchParams& P, const PathState& ps, const SurfState& sf) {
   const int nStrat = neeStrategies(P);
-  const DevMaterial& mat = P.materials[sf.matId];
-  float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
-  vec3 f = bsdfEval(mat, sf.albedo, -ps.d, sf.wi, sf.ns, sf.etaRel);
-  float w = 1.0f;
-  if (!sf.isDelta) {
-    float pdfB = bsdfPdf(mat, -ps.d, sf.wi, sf.ns, sf.etaRel);
-    w = pdfLe / (pdfLe + pdfB);
-  }
-  return ps.beta * f * fabsf(sf.cosS) * sf.Li * w / pdfLe;
+  const Closure cl = makeClosure(P.materials[sf.matId], sf.albedo, sf.etaRel);
+  const Frame fr(sf.ns);
+  const LobeEval e = evalClosure(cl, fr.toLocal(-ps.d), fr.toLocal(sf.wi));
+  const float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
+  const float w = sf.isDelta ? 1.0f : pdfLe / (pdfLe + e.pdf);
+  return ps.beta * e.f * fabsf(sf.cosS) * sf.Li * w / pdfLe;
 }
 
 // Whether the random numbers a bounce draws after its shadow query depend on
@@ -1844,7 +752,7 @@ DEV int pathOnShadow(const LaunchParams& P, PathState& ps, const SurfState& sf, 
       float tmaxS;
       shadowRay(sf, so, wi, tmaxS);
       int skip = (sf.k < P.numLights) ? P.lights[sf.k].flameId : -1;
-      tr = tr * flameTransmittance(P, so, wi, tmaxS, rng, skip);
+      tr = tr * volumeTransmittance(P, so, wi, tmaxS, rng, skip);
     }
     vec3 c = neeUnshadowed(P, ps, sf);
     c = c * tr;
@@ -1952,7 +860,7 @@ DEV void renderWavePooled(const LaunchParams& P, SlotPixel slotPixel, uint32_t* 
           if (maxc(tr) > 0.0f) {
             Pcg rng = pathRng(P, ps);
             const int skip = (sf.k < P.numLights) ? P.lights[sf.k].flameId : -1;
-            tr = tr * flameTransmittance(P, so, wi, tmaxS, rng, skip);
+            tr = tr * volumeTransmittance(P, so, wi, tmaxS, rng, skip);
             ps.rngState = rng.state;
             vec3 c = cNee * tr;
             if (clampNee) c = vmin(c, v3(P.clampVal));
@@ -2191,7 +1099,7 @@ enum WfPlane {
   WP_RES0,    // closest: t, inst, prim, b1 | shadow: transmittance.xyz, -
   WP_RES1,    // closest: b2, -, -, -
   WP_SX,      // surface x.xyz, etaRel
-  WP_SNS,     // ns.xyz, etaExt
+  WP_SNS,     // ns.xyz, -
   WP_SALB,    // albedo.xyz, matId | frontface << 16 | isDelta << 17 | transmitNee << 18
   WP_SWI,     // wi.xyz, pdf
   WP_SLI,     // Li.xyz, dist
@@ -2250,7 +1158,6 @@ DEV void wfLoadSurf(const WfParams& W, int slot, SurfState& sf) {
   sf.x = f4xyz(a);
   sf.etaRel = a.w;
   sf.ns = f4xyz(b);
-  sf.etaExt = b.w;
   sf.albedo = f4xyz(c);
   uint32_t m = wbits(c);
   sf.matId = m & 0xFFFFu;
@@ -2269,7 +1176,7 @@ DEV void wfStoreSurf(const WfParams& W, int slot, const SurfState& sf) {
   uint32_t m = sf.matId | ((sf.frontface ? 1u : 0u) << 16) | ((sf.isDelta ? 1u : 0u) << 17) |
                ((sf.transmitNee ? 1u : 0u) << 18);
   wfSt(W, WP_SX, slot, f4(sf.x, sf.etaRel));
-  wfSt(W, WP_SNS, slot, f4(sf.ns, sf.etaExt));
+  wfSt(W, WP_SNS, slot, f4(sf.ns, 0.0f));
   wfSt(W, WP_SALB, slot, f4i(sf.albedo, m));
   wfSt(W, WP_SWI, slot, f4(sf.wi, sf.pdf));
   wfSt(W, WP_SLI, slot, f4(sf.Li, sf.dist));
@@ -2589,10 +1496,48 @@ __global__ __launch_bounds__(kBlock) void k_shadow(LaunchParams P, const float* 
 __global__ void k_env_bins(LaunchParams P, const float* u, int n, int* rowOut, int* colOut) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int row = findInterval(P.margCdf, P.envH + 1, u[2 * i]);
-  int col = findInterval(P.condCdf + (size_t)row * (P.envW + 1), P.envW + 1, u[2 * i + 1]);
+  const int row = envRowBin(P, u[2 * i]);
+  const int col = envColBin(P, row, u[2 * i + 1]);
   rowOut[i] = row;
   colOut[i] = col;
+}
+
+// Light samples from n points: sample i draws from init(seed0 + i, seq).
+// out: 12 floats per sample (wi, dist, Li, pdf, flags, draws, MIS-side pdf
+// at the sampled point or NaN for mesh / delta lights, pad).
+__global__ void k_light_sample(LaunchParams P, int light, const float* xs, int n, uint64_t seed0, uint64_t seq,
+                               float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Pcg rng = Pcg::make(seed0 + (uint64_t)i, seq);
+  const uint64_t s0 = rng.state;
+  const vec3 x = vec3{xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
+  const DevLight& lt = P.lights[light];
+  const LightSample s = sampleLight(P, lt, x, rng);
+  // draws taken: PCG steps between the two states (at most a handful)
+  Pcg walk;
+  walk.state = s0;
+  walk.inc = rng.inc;
+  int draws = 0;
+  while (walk.state != rng.state && draws < 16) {
+    walk.next();
+    draws++;
+  }
+  float back = __builtin_nanf("");
+  if (s.valid && !s.isDelta && lt.kind != LT_MESH) back = lightPdfSolidAngle(lt, x, x + s.wi * s.dist, v3(0.0f));
+  float* o = out + 12 * (size_t)i;
+  o[0] = s.wi.x;
+  o[1] = s.wi.y;
+  o[2] = s.wi.z;
+  o[3] = s.dist;
+  o[4] = s.Li.x;
+  o[5] = s.Li.y;
+  o[6] = s.Li.z;
+  o[7] = s.valid ? s.pdf : 0.0f;
+  o[8] = (float)((s.valid ? 1 : 0) | (s.isDelta ? 2 : 0));
+  o[9] = (float)draws;
+  o[10] = back;
+  o[11] = 0.0f;
 }
 
 __global__ void k_flame_field(LaunchParams P, int flame, const float* pts, int n, float* sigmaOut, float* emOut) {
@@ -2663,6 +1608,12 @@ hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* 
 
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s) {
   hipLaunchKernelGGL(k_env_bins, dim3((n + 255) / 256), dim3(256), 0, s, P, u, n, row, col);
+  return hipGetLastError();
+}
+
+hipError_t launchLightSample(const LaunchParams& P, int light, const float* xs, int n, uint64_t seed0, uint64_t seq,
+                            float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_light_sample, dim3((n + 255) / 256), dim3(256), 0, s, P, light, xs, n, seed0, seq, out);
   return hipGetLastError();
 }
 

@@ -28,6 +28,8 @@ hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool bru
                        int* inst, int* prim, int* counters, hipStream_t s);
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
 hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
+hipError_t launchLightSample(const LaunchParams& P, int light, const float* xs, int n, uint64_t seed0, uint64_t seq,
+                            float* out, hipStream_t s);
 hipError_t launchFlameField(const LaunchParams& P, int flame, const float* pts, int n, float* sigma, float* em,
                             hipStream_t s);
 hipError_t launchPcg(uint64_t seed, uint64_t seq, int n, uint32_t* out, hipStream_t s);
@@ -255,6 +257,37 @@ inline bool buildEnvCdf(const HostImage& img, std::vector<float>& marg, std::vec
   return !degenerate;
 }
 
+// 16-ary lookup table of a CDF of n entries (the device's cdfBin16): its
+// keys cdf[1..n-2] in 16-float leaf blocks, over a mid level holding the
+// first key of every leaf block after the first, under a top block holding
+// the first entry of every mid block after the first; unused slots are +inf.
+// Returns false when the top level would need more than one block.
+struct CdfSearchShape {
+  int leaf, mid;
+  int blocks() const { return 1 + mid + leaf; }
+};
+inline CdfSearchShape cdfSearchShape(int n) {
+  const int keys = n > 2 ? n - 2 : 0;
+  CdfSearchShape sh;
+  sh.leaf = keys > 0 ? (keys + 15) / 16 : 1;
+  sh.mid = sh.leaf > 1 ? (sh.leaf - 1 + 15) / 16 : 1;
+  return sh;
+}
+inline bool buildCdfSearch(const float* cdf, int n, float* out) {
+  const CdfSearchShape sh = cdfSearchShape(n);
+  if (sh.mid - 1 > 16) return false;
+  const int keys = n > 2 ? n - 2 : 0;
+  const float inf = __builtin_inff();
+  for (int i = 0; i < 16 * sh.blocks(); i++) out[i] = inf;
+  float* top = out;
+  float* mid = out + 16;
+  float* leaf = out + 16 * (1 + sh.mid);
+  for (int k = 0; k < keys; k++) leaf[k] = cdf[1 + k];
+  for (int b = 1; b < sh.leaf; b++) mid[b - 1] = leaf[16 * b];
+  for (int c = 1; c < sh.mid; c++) top[c - 1] = mid[16 * c];
+  return true;
+}
+
 inline Aabb quadricBox(int kind) {
   const float e = 1e-4f;
   Aabb b;
@@ -372,7 +405,8 @@ struct sundog_session {
   int spillRows = 0;
   DevBuf textures, texels, materials, lights, flames, instances, nodes, tris, meshes;
   DevBuf positions, indices, normals, uvs, lightPos, lightCdf;
-  DevBuf envTex, margCdf, condCdf;
+  DevBuf envTex, margCdf, condCdf, envSearch;
+  int envSearchShape[4] = {0, 0, 0, 0};
   DevBuf accum, albedo, normal, rayCounter, workCounters, spill, sampleBuf;
   size_t spillThreads = 0;
   // wavefront variant
@@ -405,7 +439,7 @@ struct sundog_session {
   uint64_t deviceBytes() const {
     const DevBuf* all[] = {&textures, &texels, &materials, &lights, &flames, &instances, &nodes, &tris,
                            &meshes, &positions, &indices, &normals, &uvs, &lightPos, &lightCdf, &envTex,
-                           &margCdf, &condCdf, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill,
+                           &margCdf, &condCdf, &envSearch, &accum, &albedo, &normal, &rayCounter, &workCounters, &spill,
                            &sampleBuf, &wfState, &wfClosest, &wfShadow, &wfBuckets, &wfCnt};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->bytes;
@@ -723,6 +757,19 @@ struct sundog_session {
       envTex.upload(et, stream);
       margCdf.upload(marg, stream);
       condCdf.upload(cond, stream);
+      const int ew = prep.env.width, eh = prep.env.height;
+      const CdfSearchShape ms = cdfSearchShape(eh + 1), rs = cdfSearchShape(ew + 1);
+      std::vector<float> table(16 * ((size_t)ms.blocks() + (size_t)eh * rs.blocks()));
+      bool fits = buildCdfSearch(marg.data(), eh + 1, table.data());
+      for (int r = 0; fits && r < eh; r++)
+        fits = buildCdfSearch(&cond[(size_t)r * (ew + 1)], ew + 1, &table[16 * ((size_t)ms.blocks() + (size_t)r * rs.blocks())]);
+      if (fits) {
+        envSearch.upload(table, stream);
+        envSearchShape[0] = ms.blocks();
+        envSearchShape[1] = ms.mid;
+        envSearchShape[2] = rs.blocks();
+        envSearchShape[3] = rs.mid;
+      }
     }
 
     // film
@@ -772,6 +819,11 @@ struct sundog_session {
     p.envRotate = S.env.rotateDeg * kPi / 180.0f;
     p.envIntensity = S.env.intensity;
     p.envImportance = envImportance;
+    p.envSearch = envSearch.p ? envSearch.as<float4>() : nullptr;
+    p.envSearchMargBlocks = envSearchShape[0];
+    p.envSearchMargMid = envSearchShape[1];
+    p.envSearchRowBlocks = envSearchShape[2];
+    p.envSearchRowMid = envSearchShape[3];
     p.textures = textures.as<DevTexture>();
     p.texels = texels.as<float4>();
     p.materials = materials.as<DevMaterial>();

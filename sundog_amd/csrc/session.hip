@@ -314,6 +314,23 @@ extern "C" int sundog_session_env_bins(sundog_session* s, const float* u, int n,
   });
 }
 
+extern "C" int sundog_session_light_samples(sundog_session* s, int light, const float* xs, int n, uint64_t seed0,
+                                           uint64_t seq, float* out) {
+  return guarded([&] {
+    if (!s || !xs || !out) throw std::runtime_error("light samples: null argument");
+    if (light < 0 || light >= s->P.numLights) throw std::runtime_error("light samples: light id out of range");
+    if (n <= 0) return SUNDOG_OK;
+    DevBuf dx, dout;
+    dx.alloc((size_t)n * 12);
+    dout.alloc((size_t)n * 48);
+    HIP_OK(hipMemcpyAsync(dx.p, xs, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_OK(launchLightSample(s->P, light, dx.as<float>(), n, seed0, seq, dout.as<float>(), s->stream));
+    HIP_OK(hipMemcpyAsync(out, dout.p, (size_t)n * 48, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return SUNDOG_OK;
+  });
+}
+
 extern "C" int sundog_session_flame_field(sundog_session* s, int flame, const float* pts, int n, float* sigma_out,
                                           float* emission_out) {
   return guarded([&] {

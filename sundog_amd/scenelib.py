@@ -591,6 +591,19 @@ class Session(object):
                    self.lib)
         return row, col
 
+    def light_samples(self, light, xs, seed0=1, seq=7):
+        """Device light samples of light `light` from (n, 3) points, sample i
+        from the stream init(seed0 + i, seq): the oracle's light_sample keys."""
+        xs = np.ascontiguousarray(xs, np.float32).reshape(-1, 3)
+        n = xs.shape[0]
+        out = np.zeros((n, 12), np.float32)
+        _lib.check(self.lib.sundog_session_light_samples(self.ptr, int(light), _fptr(xs), n, int(seed0), int(seq),
+                                                         _fptr(out)), self.lib)
+        flags = out[:, 8].astype(np.int32)
+        return {"wi": out[:, 0:3].copy(), "dist": out[:, 3].copy(), "Li": out[:, 4:7].copy(), "pdf": out[:, 7].copy(),
+                "valid": (flags & 1) != 0, "delta": (flags & 2) != 0, "draws": out[:, 9].astype(np.int32),
+                "pdf_back": out[:, 10].copy()}
+
 
 def device_pcg32(seed, seq, n, lib=None):
     lib = lib or _lib.load()

@@ -1,0 +1,193 @@
+"""GPU probes against the CPU oracle through the C ABI: light sampling for
+every light kind, the ray counter, closest hits on every quadric kind, and
+textured surfaces and emitters.
+
+Light samples: both sides draw from the same PCG32 streams, so flags and
+draw counts must be equal and directions, distances, radiance and pdfs
+agree to the transcendental functions' last bits. Ray counter: radiance
+traces + traced shadow rays equal the oracle's counts on the same pixels
+(device/programs.cu:376, :493). Quadrics: (t bits, instance, root) equal to
+brute force on both sides.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import sundog_amd as sd
+from sundog_amd import configs
+from sundog_amd.scenelib import rotate_x, rotate_y, rotate_z, scale, translate
+
+pytestmark = pytest.mark.gpu
+
+
+def _light_scene(asset_dir):
+    s = sd.Scene()
+    s.render(width=16, height=16, spp=1, max_depth=2)
+    s.camera(lookfrom=[0, 1, 6], lookat=[0, 0, 0], vfov=40)
+    s.background_solid(color=[0, 0, 0])
+    s.texture("stripes", "checker", a=[1.0, 0.5, 0.25], b=[0.25, 0.5, 1.0], scale=[3, 2])
+    s.emissive("lamp", color=[3.0, 2.0, 1.0], intensity=1.0)
+    s.emissive("tlamp", texture="stripes", intensity=2.0)
+    s.emissive("glow", color=[1.0, 1.5, 2.0], intensity=1.0, two_sided=True)
+    obj = os.path.join(asset_dir, "probe_ico1.obj")
+    if not os.path.exists(obj):
+        v, f = configs.icosphere(1, amplitude=0.0, seed=1)
+        configs.write_obj(obj, v, f)
+    s.mesh("ico", obj)
+    # light list order: area lights in object order, then explicit lights
+    s.add("rect", "lamp", scale(0.5, 1, 0.3), rotate_x(180), translate(0.2, 2.0, -0.1))       # 0 rect
+    s.add("disk", "lamp", scale(0.4, 1, 0.4), rotate_x(160), translate(-0.3, 2.2, 0.1))       # 1 disk
+    s.add("sphere", "lamp", scale(0.3), translate(0.4, 2.0, -0.2))                            # 2 sphere
+    s.add("rect", "tlamp", scale(0.6, 1, 0.4), rotate_z(170), translate(-0.8, 2.1, 0.3))      # 3 textured rect
+    s.add("mesh:ico", "glow", scale(0.3, 0.2, 0.4), rotate_y(30), translate(1.0, 2.3, 0.5))   # 4 mesh
+    s.point_light(position=[0.5, 2.0, 0.0], intensity=[8.0, 4.0, 2.0], radius=0.25)          # 5 point
+    s.distant_light(direction=[0.3, -1.0, 0.2], radiance=[1.0, 2.0, 3.0])                    # 6 distant
+    return s
+
+
+@pytest.mark.parametrize("light", range(7))
+def test_light_samples_match_oracle(orc, asset_dir, light):
+    sc = _light_scene(asset_dir).build()
+    sess = sc.session()
+    flat = sc.flatten()
+    rng = np.random.default_rng(light)
+    xs = np.stack([rng.uniform(-1.5, 1.5, 4096), rng.uniform(-0.5, 0.5, 4096), rng.uniform(-1.5, 1.5, 4096)], 1)
+    g = sess.light_samples(light, xs, seed0=101, seq=9)
+    c = orc.light_sample(flat, light, xs, seed0=101, seq=9)
+    assert g["valid"].mean() > 0.3
+    assert np.array_equal(g["valid"], c["valid"])
+    assert np.array_equal(g["delta"], c["delta"])
+    assert np.array_equal(g["draws"], c["draws"])
+    ok = c["valid"]
+    np.testing.assert_allclose(g["wi"][ok], c["wi"][ok], rtol=0, atol=2e-5)
+    np.testing.assert_allclose(g["dist"][ok], c["dist"][ok], rtol=2e-5)
+    np.testing.assert_allclose(g["Li"][ok], c["Li"][ok], rtol=2e-5, atol=1e-7)
+    np.testing.assert_allclose(g["pdf"][ok], c["pdf"][ok], rtol=1e-4)
+    mis = ok & np.isfinite(c["pdf_back"])
+    np.testing.assert_allclose(g["pdf_back"][mis], c["pdf_back"][mis], rtol=1e-4)
+    # the sampler's own identity on the device: sample pdf = MIS-side pdf
+    np.testing.assert_allclose(g["pdf_back"][mis], g["pdf"][mis], rtol=2e-4)
+
+
+@pytest.mark.parametrize("name", ["cornell", "instanced", "volumes"])
+def test_ray_counter_matches_oracle(orc, asset_dir, name):
+    if name == "cornell":
+        w, h, spp = 64, 64, 16
+        sc = configs.cornell(width=w, height=h, spp=spp).build()
+    elif name == "instanced":
+        w, h, spp = 1920, 1080, 1
+        sc = configs.instanced(asset_dir, width=w, height=h, spp=spp).build()
+    else:
+        w, h, spp = 1280, 720, 1
+        sc = configs.volumes(asset_dir, width=w, height=h, spp=spp).build()
+    print("ray counter: %s %dx%d %d spp" % (name, w, h, spp), flush=True)
+    sess = sc.session()
+    sess.count_rays(True)
+    sess.render(0, spp)
+    sess.sync()
+    st = sess.stats()
+    flat = sc.flatten()
+    _, _, _, cst = orc.render(flat, w, h, 0, spp)
+    expect = cst["radiance_rays"] + cst["shadow_rays"]
+    got = st["rays_traced"]
+    print("rays: gpu %d oracle %d (radiance %d, shadow %d)" % (got, expect, cst["radiance_rays"],
+                                                               cst["shadow_rays"]), flush=True)
+    assert st["paths"] == cst["paths"] == w * h * spp
+    if name == "cornell":
+        assert got == expect
+    else:
+        # the streams are shared, so the counts part only where a float
+        # difference flips a branch of a few paths
+        assert abs(got - expect) <= 2e-5 * expect + 2, (got, expect)
+
+
+def _quadric_scene():
+    s = sd.Scene()
+    s.render(width=16, height=16, spp=1, max_depth=2)
+    s.camera(lookfrom=[0, 0, 8], lookat=[0, 0, 0], vfov=40)
+    s.background_solid(color=[0, 0, 0])
+    s.lambert("a", color=[0.5, 0.5, 0.5])
+    s.lambert("b", color=[0.2, 0.6, 0.3])
+    shapes = ["disk", "cylinder", "parabola", "sphere", "rect"]
+    rng = np.random.default_rng(3)
+    for i in range(40):
+        sh = shapes[i % len(shapes)]
+        sx, sy, sz = rng.uniform(0.2, 0.8, 3)
+        s.add(sh, "a" if i % 2 else "b", scale(sx, sy, sz), rotate_x(rng.uniform(0, 360)),
+              rotate_y(rng.uniform(0, 360)), translate(*rng.uniform(-2.5, 2.5, 3)),
+              material_back="b" if i % 3 else None)
+    return s
+
+
+def test_quadric_closest_hit_bit_exact(orc):
+    sc = _quadric_scene().build()
+    sess = sc.session()
+    flat = sc.flatten()
+    rng = np.random.default_rng(4)
+    n = 8192
+    o = rng.uniform(-3.5, 3.5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = 1e16
+    t_g, i_g, p_g = sess.trace(rays)
+    t_gb, i_gb, p_gb = sess.trace(rays, brute=True)
+    t_c, i_c, p_c = orc.trace(flat, rays, brute=True)
+    hit = i_g >= 0
+    assert hit.mean() > 0.3
+    # every quadric kind is hit, on its first and second root
+    kinds = np.array([k % 5 for k in range(40)])
+    for k in range(5):
+        assert np.isin(i_g[hit], np.nonzero(kinds == k)[0]).sum() > 50, k
+    assert (p_g[hit] == 1).sum() > 100          # second roots (pass-through back faces)
+    for t, i, p in ((t_gb, i_gb, p_gb), (t_c, i_c, p_c)):
+        assert np.array_equal(i_g, i)
+        assert np.array_equal(p_g, p)
+        assert np.array_equal(t_g.view(np.uint32), t.view(np.uint32))
+
+
+def _write_ppm(path, rgb8):
+    h, w = rgb8.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"P6\n%d %d\n255\n" % (w, h))
+        f.write(np.ascontiguousarray(rgb8, np.uint8).tobytes())
+
+
+def test_textured_surfaces_and_emitter_match_oracle(orc, asset_dir):
+    img = os.path.join(asset_dir, "probe_tex.ppm")
+    yy, xx = np.mgrid[0:24, 0:32]
+    rgb = np.stack([(xx * 8) % 256, (yy * 10) % 256, ((xx + yy) * 5) % 256], -1).astype(np.uint8)
+    _write_ppm(img, rgb)
+    w, h, spp = 96, 64, 64
+    s = sd.Scene()
+    s.render(width=w, height=h, spp=spp, max_depth=6, seed=5)
+    s.camera(lookfrom=[0, 1.2, 4.5], lookat=[0, 0.3, 0], vfov=45)
+    s.background_gradient(horizon=[0.3, 0.3, 0.35], zenith=[0.05, 0.05, 0.1])
+    s.texture("chk", "checker", a=[0.9, 0.8, 0.7], b=[0.1, 0.2, 0.3], scale=[8, 8])
+    s.texture("grd", "grid", a=[0.6, 0.6, 0.6], b=[0.05, 0.05, 0.05], scale=[5, 5], width=0.06)
+    s.texture("pic", "image", file=img, srgb=True)
+    s.lambert("floor", texture="chk")
+    s.plastic("wall", texture="grd", roughness=0.3)
+    s.metal("ball", texture="pic", roughness=0.2)
+    s.lambert("cyl", texture="pic")
+    s.emissive("panel", texture="chk", intensity=4.0)
+    s.add("rect", "floor", scale(3))
+    s.add("rect", "wall", scale(3, 1, 1.5), rotate_x(90), translate(0, 1.5, -2))
+    s.add("sphere", "ball", scale(0.5), translate(-0.8, 0.5, 0))
+    s.add("cylinder", "cyl", scale(0.35, 0.5, 0.35), translate(0.8, 0.5, 0.3))
+    s.add("rect", "panel", scale(0.6, 1, 0.3), rotate_x(180), translate(0, 2.2, 0))
+    sc = s.build()
+    sess = sc.session()
+    sess.render_all()
+    gpu, _, _ = sess.download()
+    ref, _, _, _ = orc.render(sc.flatten(), w, h, 0, spp)
+    assert np.isfinite(gpu).all()
+    err = sd.rel_rmse(gpu, ref)
+    close = np.abs(gpu[..., :3] - ref[..., :3]) <= 1e-4 * np.maximum(1.0, np.abs(ref[..., :3]))
+    print("textured: relRMSE %.3g, %.4f of values within 1e-4" % (err, close.mean()), flush=True)
+    assert err < 1e-3
+    assert close.mean() > 0.98

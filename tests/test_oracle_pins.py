@@ -413,3 +413,77 @@ def test_distant_light_on_lambert_plane_closed_form(orc):
     b, _, _, _ = orc.render(flat, 16, 16, 0, 2)
     expect = 0.5 / math.pi * 2.0 * math.cos(theta)
     np.testing.assert_allclose(b[..., :3], expect, rtol=2e-5)
+
+
+def _quadric_roots64(kind, o, d):
+    """Float64 roots of the canonical disk, cylinder and parabola (object
+    space, rays o + t d), filtered to the surface patch; NaN for none; also
+    the distance of the hit to the patch's boundary (for excluding grazing
+    cases)."""
+    n = len(o)
+    best = np.full(n, np.nan)
+    margin = np.full(n, np.inf)
+    if kind == "disk":
+        t = -o[:, 1] / d[:, 1]
+        p = o + t[:, None] * d
+        r2 = p[:, 0] ** 2 + p[:, 2] ** 2
+        ok = (t > 0) & (r2 <= 1.0)
+        best[ok] = t[ok]
+        margin = np.abs(r2 - 1.0)
+        return best, margin
+    if kind == "cylinder":
+        a = d[:, 0] ** 2 + d[:, 2] ** 2
+        b = o[:, 0] * d[:, 0] + o[:, 2] * d[:, 2]
+        c = o[:, 0] ** 2 + o[:, 2] ** 2 - 1.0
+        disc = b * b - a * c
+        roots = [(-b - np.sqrt(np.maximum(disc, 0))) / a, (-b + np.sqrt(np.maximum(disc, 0))) / a]
+        ycheck = lambda t: np.abs(o[:, 1] + t * d[:, 1])
+    else:  # parabola y = (x^2 + z^2) / 2 over the unit disk
+        A = 0.5 * (d[:, 0] ** 2 + d[:, 2] ** 2)
+        B = o[:, 0] * d[:, 0] + o[:, 2] * d[:, 2] - d[:, 1]
+        C = 0.5 * (o[:, 0] ** 2 + o[:, 2] ** 2) - o[:, 1]
+        disc = B * B - 4 * A * C
+        roots = [(-B - np.sqrt(np.maximum(disc, 0))) / (2 * A), (-B + np.sqrt(np.maximum(disc, 0))) / (2 * A)]
+        ycheck = lambda t: (o[:, 0] + t * d[:, 0]) ** 2 + (o[:, 2] + t * d[:, 2]) ** 2
+    for t in roots:
+        ok = (disc > 0) & (t > 0) & (ycheck(t) <= 1.0) & np.isnan(best)
+        best[ok] = t[ok]
+        margin = np.minimum(margin, np.where(disc > 0, np.abs(ycheck(t) - 1.0), np.inf))
+    margin = np.minimum(margin, np.abs(disc))
+    return best, margin
+
+
+@pytest.mark.parametrize("kind", ["disk", "cylinder", "parabola"])
+def test_disk_cylinder_parabola_hit_distances_closed_form(orc, kind):
+    sx, sy, sz, c = 0.8, 1.3, 0.6, np.array([0.4, -0.3, -6.0])
+    # the disk is turned to face the rays (+z): world = T R S p
+    steps = [scale(sx, sy, sz)] + ([rotate_x(90)] if kind == "disk" else []) + [translate(*c)]
+    def objs(s):
+        s.lambert("a", color=[0.5, 0.5, 0.5])
+        s.add(kind, "a", *steps, material_back="a")
+    flat = _scene(objs).flatten()
+    rng = np.random.default_rng({"disk": 1, "cylinder": 2, "parabola": 3}[kind])
+    n = 4000
+    o = rng.uniform(-2.0, 2.0, (n, 3)) + np.array([0.0, 0.0, 0.0])
+    tgt = c + rng.uniform(-1.2, 1.2, (n, 3)) * np.array([sx, sy, sz])
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = 1e16
+    t, inst, prim = orc.trace(flat, rays)
+    # the ray in object space: t is unchanged by the affine map
+    sc = np.array([sx, sy, sz])
+    o32, d32 = rays[:, 0:3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+    ow, dw = o32 - c, d32
+    if kind == "disk":  # undo rotate_x(90): (x, y, z) -> (x, z, -y)
+        ow = np.stack([ow[:, 0], ow[:, 2], -ow[:, 1]], 1)
+        dw = np.stack([dw[:, 0], dw[:, 2], -dw[:, 1]], 1)
+    t64, margin = _quadric_roots64(kind, ow / sc, dw / sc)
+    clear = margin > 1e-3
+    hit64 = ~np.isnan(t64)
+    assert (clear & hit64).sum() > 500 and (clear & ~hit64).sum() > 200
+    np.testing.assert_array_equal((inst >= 0)[clear], hit64[clear])
+    both = clear & hit64
+    np.testing.assert_allclose(t[both], t64[both], rtol=2e-5)

@@ -62,9 +62,15 @@ def test_light_samples_match_oracle(orc, asset_dir, light):
     assert np.array_equal(g["draws"], c["draws"])
     ok = c["valid"]
     np.testing.assert_allclose(g["wi"][ok], c["wi"][ok], rtol=0, atol=2e-5)
-    np.testing.assert_allclose(g["dist"][ok], c["dist"][ok], rtol=2e-5)
+    # the sphere light's distance b - sqrt(b^2 - d^2 + r^2) cancels for
+    # near-tangent cone samples, amplifying last-bit direction differences
+    np.testing.assert_allclose(g["dist"][ok], c["dist"][ok], rtol=3e-4 if light == 2 else 2e-5)
     np.testing.assert_allclose(g["Li"][ok], c["Li"][ok], rtol=2e-5, atol=1e-7)
-    np.testing.assert_allclose(g["pdf"][ok], c["pdf"][ok], rtol=1e-4)
+    # area pdfs d^2 / (cos * A) grow as 1 / cos at grazing emitter angles,
+    # where last-bit direction differences are amplified the same way
+    graze = ok & (c["pdf"] > 1e4)
+    np.testing.assert_allclose(g["pdf"][ok & ~graze], c["pdf"][ok & ~graze], rtol=1e-4)
+    np.testing.assert_allclose(g["pdf"][graze], c["pdf"][graze], rtol=1e-3)
     mis = ok & np.isfinite(c["pdf_back"])
     np.testing.assert_allclose(g["pdf_back"][mis], c["pdf_back"][mis], rtol=1e-4)
     # the sampler's own identity on the device: sample pdf = MIS-side pdf
@@ -115,9 +121,11 @@ def _quadric_scene():
     for i in range(40):
         sh = shapes[i % len(shapes)]
         sx, sy, sz = rng.uniform(0.2, 0.8, 3)
-        s.add(sh, "a" if i % 2 else "b", scale(sx, sy, sz), rotate_x(rng.uniform(0, 360)),
-              rotate_y(rng.uniform(0, 360)), translate(*rng.uniform(-2.5, 2.5, 3)),
-              material_back="b" if i % 3 else None)
+        # every third object lets its front face through, so rays that enter
+        # it stop on its back face: the second root
+        front = None if i % 3 == 0 else ("a" if i % 2 else "b")
+        s.add(sh, front, scale(sx, sy, sz), rotate_x(rng.uniform(0, 360)),
+              rotate_y(rng.uniform(0, 360)), translate(*rng.uniform(-2.5, 2.5, 3)), material_back="b")
     return s
 
 
@@ -138,12 +146,12 @@ def test_quadric_closest_hit_bit_exact(orc):
     t_gb, i_gb, p_gb = sess.trace(rays, brute=True)
     t_c, i_c, p_c = orc.trace(flat, rays, brute=True)
     hit = i_g >= 0
-    assert hit.mean() > 0.3
+    assert hit.mean() > 0.1
     # every quadric kind is hit, on its first and second root
     kinds = np.array([k % 5 for k in range(40)])
     for k in range(5):
         assert np.isin(i_g[hit], np.nonzero(kinds == k)[0]).sum() > 50, k
-    assert (p_g[hit] == 1).sum() > 100          # second roots (pass-through back faces)
+    assert (p_g[hit] == 1).sum() > 30           # second roots (behind pass-through front faces)
     for t, i, p in ((t_gb, i_gb, p_gb), (t_c, i_c, p_c)):
         assert np.array_equal(i_g, i)
         assert np.array_equal(p_g, p)

@@ -730,8 +730,7 @@ DEV void shadowRay(const SurfState& sf, vec3& so, vec3& wi, float& tmax) {
 DEV vec3 neeUnshadowed(const LaunchParams& P, const PathState& ps, const SurfState& sf) {
   const int nStrat = neeStrategies(P);
   const Closure cl = makeClosure(P.materials[sf.matId], sf.albedo, sf.etaRel);
-  const Frame fr(sf.ns);
-  const LobeEval e = evalClosure(cl, fr.toLocal(-ps.d), fr.toLocal(sf.wi));
+  const LobeEval e = evalClosureAt(cl, sf.ns, -ps.d, sf.wi);
   const float pdfLe = (sf.isDelta ? 1.0f : sf.pdf) / nStrat;
   const float w = sf.isDelta ? 1.0f : pdfLe / (pdfLe + e.pdf);
   return ps.beta * e.f * fabsf(sf.cosS) * sf.Li * w / pdfLe;

@@ -493,13 +493,16 @@ struct Closure {
 
 // etaRel: outside / inside IOR as seen from the incoming ray (dielectrics
 // and water), frontface ? etaExt / ior : ior / etaExt.
+SDEV float dielectricF0(float eta) {
+  const float r = (1.0f - eta) / (1.0f + eta);
+  return r * r;
+}
 SDEV Closure makeClosure(const DevMaterial& m, vec3 tint, float etaRel) {
   Closure c;
   c.tint = tint;
   c.eta = etaRel;
   c.alpha = m.roughness * m.roughness;
-  const float r = (1.0f - etaRel) / (1.0f + etaRel);
-  c.f0 = r * r;
+  c.f0 = 0.0f;  // conductors take F0 from the tint, lambert has none
   switch (m.kind) {
     case MT_LAMBERT:
       c.lobe = LK_DIFFUSE;
@@ -509,9 +512,11 @@ SDEV Closure makeClosure(const DevMaterial& m, vec3 tint, float etaRel) {
       break;
     case MT_DIELECTRIC:
       c.lobe = m.roughness < kSmoothRoughness ? LK_INTERFACE : LK_GLASS;
+      c.f0 = dielectricF0(etaRel);
       break;
     case MT_WATER:  // always smooth: its waves come from the ripple normal
       c.lobe = LK_INTERFACE;
+      c.f0 = dielectricF0(etaRel);
       break;
     case MT_PLASTIC: {
       c.lobe = LK_COATED;
@@ -649,6 +654,18 @@ SDEV LobeEval evalClosure(const Closure& c, vec3 lo, vec3 li) {
     default:
       return noEval();
   }
+}
+
+// The same from world directions wo, wi at shading normal n; the lambert
+// lobe needs only the two cosines, every other lobe the shading frame.
+SDEV LobeEval evalClosureAt(const Closure& c, vec3 n, vec3 wo, vec3 wi) {
+  if (c.lobe == LK_DIFFUSE) {
+    if (dot(wo, n) <= 0.0f) return noEval();
+    const float ci = dot(wi, n);
+    return ci <= 0.0f ? noEval() : LobeEval{c.tint * kInvPi, ci * kInvPi};
+  }
+  const Frame fr(n);
+  return evalClosure(c, fr.toLocal(wo), fr.toLocal(wi));
 }
 
 struct BsdfSample {
@@ -999,23 +1016,36 @@ SDEV int quadricRoots(int kind, vec3 o, vec3 d, float tmin, float tmax, float t[
 SDEV float azimuthU(float x, float z) { return (atan2f(z, x) + kPi) / (2.0f * kPi); }
 
 // Object-space normal of the canonical front side and the surface uv at a
-// quadric hit point p.
+// quadric hit point p, one branch per kind (each with its own azimuth: a
+// shared or selected one changes the traversal loop's register allocation
+// for the worse).
 SDEV void quadricSurface(int kind, vec3 p, vec3& n, float& u, float& v) {
-  if (kind == GK_RECT || kind == GK_DISK) {
-    n = vec3{0.0f, 1.0f, 0.0f};
-    u = kind == GK_RECT ? (p.x + 1.0f) * 0.5f : azimuthU(p.x, p.z);
-    v = kind == GK_RECT ? (p.z + 1.0f) * 0.5f : sqrtf(p.x * p.x + p.z * p.z);
-    return;
+  switch (kind) {
+    case GK_RECT:
+      n = vec3{0.0f, 1.0f, 0.0f};
+      u = (p.x + 1.0f) * 0.5f;
+      v = (p.z + 1.0f) * 0.5f;
+      break;
+    case GK_SPHERE:
+      n = normalize(p);
+      u = azimuthU(n.x, n.z);
+      v = (asinf(clampf(n.y, -1.0f, 1.0f)) + kPi / 2.0f) / kPi;
+      break;
+    case GK_DISK:
+      n = vec3{0.0f, 1.0f, 0.0f};
+      u = azimuthU(p.x, p.z);
+      v = sqrtf(p.x * p.x + p.z * p.z);
+      break;
+    case GK_CYLINDER:
+      n = normalize(vec3{p.x, 0.0f, p.z});
+      u = azimuthU(p.x, p.z);
+      v = (p.y + 1.0f) * 0.5f;
+      break;
+    default:  // parabola
+      n = normalize(vec3{p.x, -1.0f, p.z});
+      u = azimuthU(p.x, p.z);
+      v = sqrtf(p.x * p.x + p.z * p.z);
   }
-  if (kind == GK_SPHERE) {
-    n = normalize(p);
-    u = azimuthU(n.x, n.z);
-    v = (asinf(clampf(n.y, -1.0f, 1.0f)) + kPi / 2.0f) / kPi;
-    return;
-  }
-  n = normalize(kind == GK_CYLINDER ? vec3{p.x, 0.0f, p.z} : vec3{p.x, -1.0f, p.z});
-  u = azimuthU(p.x, p.z);
-  v = kind == GK_CYLINDER ? (p.y + 1.0f) * 0.5f : sqrtf(p.x * p.x + p.z * p.z);
 }
 
 // Watertight ray / triangle test (Woop, Benthin, Wald, JCGT 2013): the ray

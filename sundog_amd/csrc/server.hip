@@ -88,6 +88,9 @@ struct sundog_server {
   int64_t nextId = 0;
   bool stop = false;
   bool started = false;
+  // a serial render or the autotuner runs on the caller's thread on the
+  // server stream: no batch may run meanwhile (guarded by mu)
+  bool exclusive = false;
   std::string workerError;
   std::thread worker;
   // stats
@@ -152,8 +155,9 @@ struct sundog_server {
     applyTuning();
   }
 
-  // Knobs that reallocate the launch buffers (spill, per-sample results)
-  // may only change while no request is in flight; the caller holds mu.
+  // Launch-shape knobs (buffer sizes, tile, pass length, graph passes) may
+  // only change while no request is in flight, so the worker never reads
+  // one mid-change; the caller holds mu.
   void requireIdle(const char* what) const {
     if (!active.empty() || !waiting.empty())
       throw std::runtime_error(std::string("tuning: ") + what + " cannot change while requests are in flight");
@@ -386,13 +390,23 @@ struct sundog_server {
         }
       }
     } catch (const std::exception& e) {
+      // let whatever was queued on the stream end before its films go
+      (void)hipStreamSynchronize(stream);
       std::lock_guard<std::mutex> lk(mu);
       workerError = e.what();
-      for (ServerRequest* r : active) {
+      auto failOne = [&](ServerRequest* r) {
         r->status = REQ_FAILED;
+        r->tDone = now();
+        r->accum.release();
+        r->albedo.release();
+        r->normal.release();
         completed.push_back(r->id);
-      }
+      };
+      for (ServerRequest* r : active) failOne(r);
+      for (ServerRequest* r : waiting) failOne(r);
       active.clear();
+      waiting.clear();
+      used = 0;
     }
     idleCv.notify_all();
   }
@@ -402,6 +416,26 @@ struct sundog_server {
     ensureDevice();
     started = true;
     worker = std::thread([this] { loop(); });
+  }
+};
+
+// Claims the server stream for a call that drives it from the caller's
+// thread (serial render, autotuner): refused while requests are queued or
+// running, and submits are refused until it ends. Snapshots the tuning the
+// call reads.
+struct ExclusiveUse {
+  sundog_server& s;
+  int serialWaves = 0;
+  ExclusiveUse(sundog_server& srv, const char* who) : s(srv) {
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.active.empty() || !s.waiting.empty() || s.exclusive)
+      throw std::runtime_error(std::string(who) + ": the server is busy");
+    s.exclusive = true;
+    serialWaves = s.serialWaves;
+  }
+  ~ExclusiveUse() {
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.exclusive = false;
   }
 };
 
@@ -446,8 +480,10 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
     if (!s || !key) throw std::runtime_error("server: null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     std::string k = key;
+    if (s->exclusive) throw std::runtime_error("tuning: a serial render or the autotuner holds the server");
     if (k == "tile") {
       if (value != 8 && value != 16 && value != 32) throw std::runtime_error("tuning: tile must be 8, 16 or 32");
+      s->requireIdle("tile");
       s->tileSize = value;
     } else if (k == "waves") {
       if (value < 2 || value > 5) throw std::runtime_error("tuning: waves must be 2, 3, 4 or 5");
@@ -466,6 +502,7 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
       if (s->stream) s->applyTuning();
     } else if (k == "graph_passes") {
       if (value < 1 || value > 64) throw std::runtime_error("tuning: graph_passes must be 1..64");
+      s->requireIdle("graph_passes");
       s->graphPasses = value;
     } else if (k == "serial_waves") {
       if (value != 0 && (value < 2 || value > 5))
@@ -473,6 +510,7 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
       s->serialWaves = value;
     } else if (k == "pass_spp") {
       if (value <= 0) throw std::runtime_error("tuning: pass_spp must be positive");
+      s->requireIdle("pass_spp");
       s->passSpp = value;
     } else {
       throw std::runtime_error("tuning: unknown key '" + k + "'");
@@ -498,6 +536,7 @@ extern "C" int64_t sundog_server_submit(sundog_server* s, int scene_id, int widt
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (!s->workerError.empty()) throw std::runtime_error("server failed: " + s->workerError);
+      if (s->exclusive) throw std::runtime_error("submit: a serial render or the autotuner holds the server");
       id = s->nextId++;
       r->id = id;
       r->tSubmit = s->now();
@@ -629,11 +668,8 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
   return guardedS([&] {
     if (!s) throw std::runtime_error("serial: null handle");
     if (scene_id < 0 || scene_id >= (int)s->scenes.size()) throw std::runtime_error("serial: scene id out of range");
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (!s->active.empty() || !s->waiting.empty())
-        throw std::runtime_error("serial: the batching loop is busy");
-    }
+    ExclusiveUse hold(*s, "serial");
+    const int serialWaves = hold.serialWaves;
     s->ensureDevice();
     ServerRequest r;
     r.scene = scene_id;
@@ -672,10 +708,10 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
       P.sampleOffset = done;
       P.sppThisLaunch = std::min(16, spp - done);
       bool trial = false, gTrial = false;
-      const int w = s->serialWaves != 0 ? s->serialWaves : pick.pick(trial);
+      const int w = serialWaves != 0 ? serialWaves : pick.pick(trial);
       int groups = 1;
       if (P.sppThisLaunch <= kPoolSamples) {
-        if (trial || (s->serialWaves == 0 && pick.chosen == 0))
+        if (trial || (serialWaves == 0 && pick.chosen == 0))
           groups = sampleGroupsFor(tiles, P.sppThisLaunch, s->numCUs);
         else
           groups = gpick.pick(gTrial, P.sppThisLaunch);
@@ -708,6 +744,7 @@ extern "C" int sundog_server_autotune(sundog_server* s, int scene_id, int width,
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->started) throw std::runtime_error("autotune: call before the first submit");
     }
+    ExclusiveUse hold(*s, "autotune");
     s->ensureDevice();
     ServerRequest r;
     r.scene = scene_id;

@@ -13,7 +13,11 @@ sides). The roofline entry prices the render kernel's algorithmic traversal
 FLOPs (12 per box test, 40 per triangle test, 30 per quadric test, 30 per
 instance transform; counted by an instrumented pass) against the FP32
 vector peak; the kernel time comes from HIP events on the session stream.
-cpu_baseline times the CPU oracle on a bounded sample of the same frame.
+cpu_baseline times the CPU oracle on a bounded sample of the same frame,
+on the host cores given to this job. A short run of the serving workload
+(four resident scenes, Poisson arrivals, continuous batching in the
+persistent kernel) adds the served and serial-megakernel rates, their
+ratio and the p50 / p99 request latencies under "serving".
 """
 
 import argparse
@@ -27,13 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "paths/s & Mrays/s at 1080p; p50/p99 request latency at ≤1% RMSE"
-# MI355X FP32 vector peak without packed FP32 (256 CUs x 64 lanes x 2 flops
-# per FMA x 2.4 GHz); the published 157.3 TFLOP/s counts packed-FP32 FMAs,
-# which this build does not emit.
-FP32_PEAK_TFLOPS = 78.6
-FP32_PEAK_PACKED_TFLOPS = 157.3
+# MI355X published FP32 vector peak (157.3 TFLOP/s counts packed-FP32 FMAs)
+# and the non-packed issue peak (256 CUs x 64 lanes x 2 flops per FMA x
+# 2.4 GHz); this build emits no packed FP32, so the second bounds it.
+FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_NONPACKED_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_render_instanced_1080p.json")
+TRAFFIC_PROFILE_NAME = "profiles/k_render_instanced_1080p.json"
+TRAFFIC_PROFILE = os.path.join(ROOT, TRAFFIC_PROFILE_NAME)
 
 
 def counter_profile(kernel):
@@ -89,6 +94,11 @@ def main():
                     help="host binned-SAH build (default), the GPU LBVH rebuild of both levels, or the GPU "
                          "LBVH for the meshes with a host SAH top level over GPU instance boxes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0: the job's CPU share, OMP_NUM_THREADS capped by the affinity mask)")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests of the serving run reported under 'serving' (0: skip it)")
+    ap.add_argument("--serving-qps", type=float, default=4.0)
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
     ap.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
@@ -222,7 +232,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
             "traffic": measured_traffic(kname) if args.workload == "instanced" else None,
-            "peak_packed_fp32": FP32_PEAK_PACKED_TFLOPS,
+            "traffic_source": TRAFFIC_PROFILE_NAME if args.workload == "instanced" else None,
+            "peak_nonpacked_fp32": FP32_PEAK_NONPACKED_TFLOPS,
+            "frac_nonpacked_fp32": achieved_tflops / FP32_PEAK_NONPACKED_TFLOPS,
             "kernel": kname,
             "kernel_ms_avg": kernel_ms_avg,
             "algorithmic_flops_per_query": flops_per_query,
@@ -244,8 +256,8 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         flat = scene.flatten()
-        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = min(ncpu, 16)
+        host = host_cores()
+        threads = args.cpu_threads or host["share"]
         _, _, _, cst = oracle.render(flat, W, H, 0, args.cpu_spp, tile_stride=args.cpu_tile_stride,
                                      threads=threads)
         tiles = ((W + 15) // 16) * ((H + 15) // 16)
@@ -254,13 +266,73 @@ def main():
             "unit": "paths/s",
             "cores": cst["threads"],
             "kind": "port",
+            "nproc": host["nproc"],
+            "affinity_cores": host["affinity"],
+            "omp_num_threads": host["omp"],
+            "cpu_model": host["model"],
+            "threads_rule": "the job's CPU share: OMP_NUM_THREADS (16 per GPU on this pool) capped by the affinity mask",
             "sample": "every %dth 16x16 tile of the same %dx%d frame (%d of %d tiles), samples 0..%d; "
                       "%.1f s; %.2f Mrays/s" % (args.cpu_tile_stride, W, H, (tiles + args.cpu_tile_stride - 1)
                                                // args.cpu_tile_stride, tiles, args.cpu_spp - 1,
                                                cst["seconds"],
                                                (cst["radiance_rays"] + cst["shadow_rays"]) / cst["seconds"] / 1e6),
         }
+    if args.serving_requests > 0:
+        out["serving"] = serving_run(args)
     print(json.dumps(out), flush=True)
+
+
+def host_cores():
+    """The host's core counts and model, and this job's CPU share."""
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = aff
+    if omp and omp.isdigit() and int(omp) > 0:
+        share = min(aff, int(omp))
+    return {"nproc": nproc, "affinity": aff, "omp": omp, "model": model, "share": share}
+
+
+def serving_run(args):
+    """The serving workload at a short request count: four resident scenes
+    (Cornell; instanced icospheres with 4,096 and 16,384 instances; volumes +
+    environment), seeded Poisson arrivals, resolutions {256^2, 512^2,
+    1024x576, 1920x1080}, spp log-uniform in [16, 1024]; the same requests
+    then rendered one at a time by the per-request megakernel."""
+    from sundog_amd import serving
+    scenes = serving.build_scenes(args.asset_dir)
+    handles = [s.build() for s in scenes]
+    srv = serving.Server(budget_bytes=64 << 30, pass_spp=16, tile=16)
+    sids = [srv.add_scene(h) for h in handles]
+    srv.tune("graph_passes", 4)
+    tuned = srv.autotune(sids[1], 1920, 1080, 32)
+    reqs = serving.make_requests(args.serving_requests, args.serving_qps, seed=1)
+    total = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+    print("serving: %d requests, %.3g paths" % (len(reqs), total), flush=True)
+    done, makespan = serving.run(reqs, srv, sids)
+    lat = [d["t_done_ms"] - d["t_submit_ms"] for d in done if d["status"] == 2]
+    ser_ms = 0.0
+    for r in reqs:
+        ms, _ = srv.render_serial(sids[r["scene"]], r["width"], r["height"], r["spp"], r["seed"])
+        ser_ms += ms
+    served = total / makespan
+    serial = total / (ser_ms * 1e-3)
+    srv.close()
+    return {"requests": len(reqs), "completed": len(lat), "qps": args.serving_qps, "paths": total,
+            "makespan_s": makespan, "served_paths_per_s": served, "serial_paths_per_s": serial,
+            "speedup_vs_serial": served / serial, "p50_ms": serving.percentile(lat, 50),
+            "p99_ms": serving.percentile(lat, 99),
+            "launch_shape": {k: tuned[k] for k in ("tile", "waves", "stack", "blocks_per_cu", "graph_passes")
+                             if k in tuned}}
 
 
 if __name__ == "__main__":

@@ -8,4 +8,4 @@ scene-program driver.
 
 from ._lib import load, last_error, SundogError  # noqa: F401
 from .scenelib import (Scene, Session, SceneHandle, SceneError, scale, rotate_x, rotate_y,  # noqa: F401
-                       rotate_z, translate, device_pcg32, probe, rel_rmse)
+                       rotate_z, translate, rigid_body, static_body, device_pcg32, probe, rel_rmse)

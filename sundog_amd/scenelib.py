@@ -72,6 +72,23 @@ def _put(d, k, v):
         d[k] = v
 
 
+# per-object physics bodies (recorded and passed to the ABI, which keeps them
+# and renders baked poses: settling is not simulated)
+def rigid_body(density=OMIT, velocity=OMIT, angular_velocity=OMIT, friction=OMIT, restitution=OMIT):
+    b = {"dynamic": True}
+    for k, v in (("density", density), ("velocity", velocity), ("angular_velocity", angular_velocity),
+                 ("friction", friction), ("restitution", restitution)):
+        _put(b, k, v)
+    return b
+
+
+def static_body(thickness=OMIT, friction=OMIT, restitution=OMIT):
+    b = {}
+    for k, v in (("thickness", thickness), ("friction", friction), ("restitution", restitution)):
+        _put(b, k, v)
+    return b
+
+
 class Scene(object):
     def __init__(self):
         self._render = {}
@@ -83,8 +100,22 @@ class Scene(object):
         self._objects = []
         self._lights = []
         self._flames = []
+        self._physics = None
 
     # ---- blocks ---------------------------------------------------------------
+    def physics(self, gravity=OMIT, timestep=OMIT, max_time=OMIT, friction=OMIT, restitution=OMIT,
+                solver_iterations=OMIT, sleep_threshold=OMIT, stop_time=OMIT):
+        if solver_iterations is not OMIT and (not isinstance(solver_iterations, (list, tuple))
+                                              or len(solver_iterations) != 2):
+            raise SceneError("physics: solver_iterations must be [posIters, velIters]")
+        p = {}
+        for k, v in (("gravity", gravity), ("timestep", timestep), ("max_time", max_time),
+                     ("friction", friction), ("restitution", restitution),
+                     ("solver_iterations", solver_iterations), ("sleep_threshold", sleep_threshold),
+                     ("stop_time", stop_time)):
+            _put(p, k, v)
+        self._physics = p
+
     def render(self, width=OMIT, height=OMIT, spp=OMIT, max_depth=OMIT, clamp=OMIT,
                seed=OMIT, exposure=OMIT, transparent_shadows=OMIT):
         for k, v in (("width", width), ("height", height), ("spp", spp),
@@ -165,7 +196,7 @@ class Scene(object):
 
     # ---- objects and lights ---------------------------------------------------
     def add(self, shape, material, *steps, **kw):
-        unknown = set(kw) - {"material_back", "cutout", "nee"}
+        unknown = set(kw) - {"material_back", "cutout", "nee", "physics"}
         if unknown:
             raise SceneError("add: unknown keyword(s) %s" % ", ".join(sorted(unknown)))
         if shape not in _SHAPES and not str(shape).startswith("mesh:"):
@@ -173,8 +204,8 @@ class Scene(object):
         o = {"shape": shape, "material": material}
         if steps:
             o["transform"] = list(steps)
-        for k in ("material_back", "cutout", "nee"):
-            if k in kw:
+        for k in ("material_back", "cutout", "nee", "physics"):
+            if k in kw and kw[k] is not OMIT:
                 o[k] = kw[k]
         self._objects.append(o)
         return o
@@ -223,6 +254,12 @@ class Scene(object):
         p.append(("set_render", fint(r, "width"), fint(r, "height"), fint(r, "spp"),
                   fint(r, "max_depth"), fnum(r, "clamp"), fint(r, "seed"),
                   fnum(r, "exposure"), ftri(r, "transparent_shadows")))
+        if self._physics is not None:
+            ph = self._physics
+            it = ph.get("solver_iterations")
+            p.append(("set_physics", fvec(ph, "gravity"), fnum(ph, "timestep"), fnum(ph, "max_time"),
+                      fnum(ph, "friction"), fnum(ph, "restitution"), int(it[0]) if it else -1,
+                      int(it[1]) if it else -1, fnum(ph, "sleep_threshold"), fnum(ph, "stop_time")))
         c = self._camera
         p.append(("set_camera", fvec(c, "lookfrom"), fvec(c, "lookat"), fvec(c, "up"),
                   fnum(c, "vfov"), fnum(c, "aperture"), fnum(c, "focus_dist")))
@@ -303,7 +340,14 @@ class Scene(object):
                 else:
                     steps.append(({"rotate_x": 2, "rotate_y": 3, "rotate_z": 4}[k],
                                   float(v), 0.0, 0.0))
-            p.append(("add_object", gk, mesh_id, front, back, cutout, steps, ftri(o, "nee")))
+            body = None
+            if "physics" in o:
+                pb = o["physics"]
+                body = (1 if pb.get("dynamic") else 0, fnum(pb, "density"), 1 if "velocity" in pb else 0,
+                        fvec(pb, "velocity") or [0.0, 0.0, 0.0], 1 if "angular_velocity" in pb else 0,
+                        fvec(pb, "angular_velocity") or [0.0, 0.0, 0.0], fnum(pb, "friction"),
+                        fnum(pb, "restitution"), fnum(pb, "thickness"))
+            p.append(("add_object", gk, mesh_id, front, back, cutout, steps, ftri(o, "nee"), body))
         for i, fl in enumerate(self._flames):
             p.append(("add_flame", [float(v) for v in fl["base"]], float(fl["height"]),
                       float(fl["radius"]), fnum(fl, "intensity"), fnum(fl, "sigma"),
@@ -345,6 +389,9 @@ def apply_program(program, lib=None):
             if name == "set_render":
                 w, ht, spp, md, cl, seed, ex, ts = args
                 rc = lib.sundog_set_render(h, w, ht, spp, md, d(cl), seed, d(ex), ts)
+            elif name == "set_physics":
+                g, ts_, mt, fr, re, pi, vi, sl, st = args
+                rc = lib.sundog_set_physics(h, _d3(g), d(ts_), d(mt), d(fr), d(re), pi, vi, d(sl), d(st))
             elif name == "set_camera":
                 lf, la, up, vf, ap, fd = args
                 rc = lib.sundog_set_camera(h, _d3(lf), _d3(la), _d3(up), vf, ap, fd)
@@ -387,12 +434,18 @@ def apply_program(program, lib=None):
                 rc = lib.sundog_add_mesh(h, args[0].encode(), args[1],
                                          args[2].encode() if args[2] else None)
             elif name == "add_object":
-                gk, mesh_id, front, back, cutout, steps, nee = args
+                gk, mesh_id, front, back, cutout, steps, nee = args[:7]
+                body = args[7] if len(args) > 7 else None
                 arr = (_lib.XformStep * max(len(steps), 1))()
                 for i, (k, a, b, c) in enumerate(steps):
                     arr[i] = _lib.XformStep(k, a, b, c)
+                pb = None
+                if body is not None:
+                    dyn, dens, hv, vel, hav, avel, fr, re, th = body
+                    pb = _lib.PhysicsBody(dyn, dens, hv, hav, (ctypes.c_double * 3)(*vel),
+                                          (ctypes.c_double * 3)(*avel), fr, re, th)
                 rc = lib.sundog_add_object(h, gk, mesh_id, front, back, cutout, arr, len(steps),
-                                           nee, None)
+                                           nee, ctypes.byref(pb) if pb is not None else None)
             elif name == "add_flame":
                 base, ht, rad, inten, sig, ns, seed, li = args
                 rc = lib.sundog_add_flame(h, _d3(base), ht, rad, inten, sig, ns, seed, li)

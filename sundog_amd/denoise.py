@@ -285,6 +285,30 @@ def measure(args):
     return res
 
 
+def train_steps(args):
+    """Training steps (forward, backward, Adam) on synthetic crops: log-HDR
+    inputs and targets with per-pixel noise; reports ms per step and the
+    step's FLOPs (forward + backward-data + backward-weights = 3x forward)."""
+    rng = np.random.default_rng(3)
+    n, c = args.batch, args.crop
+    base = rng.gamma(2.0, 0.5, (n, c, c, 3)).astype(np.float32)
+    inp = np.concatenate([base * rng.gamma(4.0, 0.25, (n, c, c, 3)), rng.uniform(0, 1, (n, c, c, 3)),
+                          rng.normal(0, 0.5, (n, c, c, 3))], axis=-1).astype(np.float32)
+    den = Denoiser(seed=1)
+    for _ in range(2):
+        den.train_step(inp, base, 1e-3)
+    _, fwd_flop = den.time_forward(n, c, c, iters=1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        den.train_step(inp, base, 1e-3)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    flop = 3.0 * fwd_flop
+    out = {"train_step_ms": ms, "batch": n, "crop": c, "step_gflop": flop / 1e9,
+           "tflops": flop / (ms * 1e-3) / 1e12, "mfma_frac_bf16_dense": flop / (ms * 1e-3) / 2.5e15}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -317,7 +341,14 @@ def main(argv=None):
     tm.add_argument("--height", type=int, default=1088)
     tm.add_argument("--width", type=int, default=1920)
     tm.add_argument("--iters", type=int, default=10)
+    ts = sub.add_parser("train-steps", help="time training steps on synthetic crops (a short program to profile)")
+    ts.add_argument("--steps", type=int, default=20)
+    ts.add_argument("--batch", type=int, default=8)
+    ts.add_argument("--crop", type=int, default=128)
     args = ap.parse_args(argv)
+
+    if args.cmd == "train-steps":
+        return train_steps(args)
 
     if args.cmd == "time":
         den = Denoiser(seed=1)

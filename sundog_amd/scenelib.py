@@ -12,7 +12,10 @@ methods select the march / ripple variants of this build:
 resident in HBM and is rendered in progressive passes.
 """
 
+import argparse
 import ctypes
+import os
+import sys
 import math
 
 import numpy as np
@@ -369,6 +372,27 @@ class Scene(object):
         """Replay the call program; returns a SceneHandle."""
         return SceneHandle(apply_program(self.program(base_dir), lib), lib)
 
+    def run(self, out, argv=None, base_dir=None):
+        """Renders this scene from a scene file's __main__ block, with the
+        reference's command line (scenes/scenelib.py:685-719, :898-929):
+        --out, --spp, --size WxH, --seed, --denoise / --no-denoise,
+        --opaque-shadows, --clamp, --physics-time, --stats, --aov-albedo,
+        --aov-normal, --probe, --quiet. Exits 0 on success, 1 with
+        'sundog: fatal: <message>' on stderr otherwise."""
+        opts = parse_args(list(sys.argv[1:] if argv is None else argv), out)
+        if opts.probe:
+            sys.exit(print_probe())
+        if base_dir is None:
+            base_dir = os.path.dirname(os.path.abspath(sys.argv[0])) or "."
+        sys.stdout.flush()
+        try:
+            h = self.build(base_dir)
+            h.render_file(opts.out, **render_overrides(opts, os.path.basename(sys.argv[0]) or "(scene)"))
+        except (SceneError, _lib.SundogError, OSError) as e:
+            sys.stderr.write("sundog: fatal: %s\n" % e)
+            sys.exit(1)
+        sys.exit(0)
+
 
 def _d3(v):
     return (ctypes.c_double * 3)(*v) if v is not None else None
@@ -665,6 +689,77 @@ def device_pcg32(seed, seq, n, lib=None):
     return out
 
 
+def parse_args(args, default_out):
+    """The scene-file command line (the reference's flags and defaults:
+    -1 / NaN / None leave the scene's own setting)."""
+    ap = argparse.ArgumentParser(prog=os.path.basename(sys.argv[0]),
+                                 description="sundog scene: running this file renders it", allow_abbrev=False)
+    ap.add_argument("--out", default=default_out, metavar="FILE")
+    ap.add_argument("--spp", type=int, default=-1, metavar="N")
+    ap.add_argument("--size", default=None, metavar="WxH")
+    ap.add_argument("--seed", type=int, default=-1, metavar="N")
+    ap.add_argument("--denoise", dest="denoise", action="store_const", const=1, default=-1)
+    ap.add_argument("--no-denoise", dest="denoise", action="store_const", const=0)
+    ap.add_argument("--opaque-shadows", dest="transparent_shadows", action="store_const", const=0, default=-1)
+    ap.add_argument("--clamp", type=float, default=NAN, metavar="F")
+    ap.add_argument("--physics-time", dest="physics_time", type=float, default=NAN, metavar="F")
+    ap.add_argument("--stats", default=None, metavar="FILE.json")
+    ap.add_argument("--aov-albedo", dest="aov_albedo", default=None)
+    ap.add_argument("--aov-normal", dest="aov_normal", default=None)
+    ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    ns = ap.parse_args(args)
+    ns.width = ns.height = -1
+    if ns.size:
+        w, sep, h = ns.size.partition("x")
+        if not sep or not w.isdigit() or not h.isdigit():
+            ap.error("--size expects WxH")
+        ns.width, ns.height = int(w), int(h)
+    return ns
+
+
+def render_overrides(opts, scene_name):
+    """sundog_render_options fields from parsed scene-file flags."""
+    o = {"spp": opts.spp, "width": opts.width, "height": opts.height, "seed": opts.seed,
+         "denoise": opts.denoise, "transparent_shadows": opts.transparent_shadows, "clamp": opts.clamp,
+         "physics_time": opts.physics_time, "quiet": 1 if opts.quiet else 0, "scene_name": scene_name}
+    if opts.stats:
+        o["stats_path"] = opts.stats
+    if opts.aov_albedo:
+        o["aov_albedo_path"] = opts.aov_albedo
+    if opts.aov_normal:
+        o["aov_normal_path"] = opts.aov_normal
+    return o
+
+
+def print_probe(lib=None):
+    """Prints the device report of --probe; returns the exit status."""
+    lib = lib or _lib.load()
+    di = _lib.DeviceInfo()
+    if lib.sundog_probe(ctypes.byref(di)) != 0:
+        sys.stderr.write("sundog: fatal: %s\n" % _lib.last_error(lib))
+        return 1
+    sys.stdout.write("GPU:            %s\n"
+                     "Compute:        %d.%d\n"
+                     "VRAM:           %d MB\n"
+                     "HIP driver:     %d\n"
+                     "HIP runtime:    %d\n"
+                     % (di.name.decode(), di.cc_major, di.cc_minor, di.total_mem_mb, di.driver_version,
+                        di.runtime_version))
+    return 0
+
+
+def run_scene_file(path, argv):
+    """Runs a scene file written against the reference's scenelib (it does
+    `from scenelib import ...`) with this library's builder in its place,
+    as its __main__: `python -m sundog_amd.scenelib FILE.py [flags]`."""
+    sys.modules["scenelib"] = sys.modules[__name__]
+    sys.argv = [path] + list(argv)
+    with open(path) as f:
+        code = compile(f.read(), path, "exec")
+    exec(code, {"__name__": "__main__", "__file__": path})
+
+
 def probe(lib=None):
     lib = lib or _lib.load()
     di = _lib.DeviceInfo()
@@ -691,3 +786,10 @@ def rel_rmse(a, b):
     a = np.asarray(a, np.float64)[..., :3]
     b = np.asarray(b, np.float64)[..., :3]
     return math.sqrt(float(((a - b) ** 2).sum()) / max(float((b ** 2).sum()), 1e-300))
+
+
+if __name__ == "__main__":
+    if len(sys.argv) < 2:
+        sys.stderr.write("usage: python -m sundog_amd.scenelib SCENE.py [scene flags]\n")
+        sys.exit(2)
+    run_scene_file(sys.argv[1], sys.argv[2:])

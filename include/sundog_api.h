@@ -276,6 +276,20 @@ SUNDOG_API void* sundog_session_stream(sundog_session*);
 SUNDOG_API int sundog_session_trace(sundog_session*, const float* rays, int n,
                                     int brute, float* t_out, int32_t* inst_out,
                                     int32_t* prim_out);
+/* The closest-hit query above for rays and outputs already in device
+ * memory, allocating nothing: no counterpart in the reference; it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). d_rays: n x 8 floats; d_t, d_inst,
+ * d_prim: n entries each, filled as sundog_session_trace fills its outputs.
+ * Asynchronous on the session's stream (sundog_session_stream): synchronize
+ * before reading the outputs. The traversal stack's overflow lives in a
+ * static scratch array of SUNDOG_TRACE_SCRATCH_BYTES per GPU, shared by every
+ * call on that GPU: calls that overlap on one GPU race. A call whose rays
+ * would need more overflow than the array holds (deep trees times n) is
+ * refused with an error that names the largest n the scene allows. */
+#define SUNDOG_TRACE_SCRATCH_BYTES (32u << 20)
+SUNDOG_API int sundog_session_trace_noalloc(sundog_session*, const float* d_rays, int n,
+                                            float* d_t, int32_t* d_inst, int32_t* d_prim);
 /* Shadow transmittance of n segments (same ray layout; tmax = segment end)
  * including the transparent-shadow accumulation; out: n x 3 floats. */
 SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,

@@ -277,6 +277,77 @@ extern "C" int sundog_session_trace(sundog_session* s, const float* rays, int n,
   });
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free closest-hit query
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Overflow region of the traversal stacks for sundog_session_trace_noalloc:
+// a static array on every GPU the library runs on, held until the process
+// ends. All calls on one GPU share it (and leave their contents in it), so
+// calls that overlap on one GPU race. Nothing in it needs to start at zero:
+// a stack entry is always written before it is read.
+constexpr size_t kTraceScratchBytes = SUNDOG_TRACE_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kTraceScratchBytes];
+
+constexpr int kMaxGpus = 64;
+std::mutex g_scratchMu;
+void* g_scratchPtr[kMaxGpus] = {};
+
+// Device address of GPU dev's scratch array, looked up once per GPU with
+// that GPU current. In host code the array's name is only a stand-in, never
+// a device address, so the address from this lookup is what kernels get.
+void* traceScratch(int dev) {
+  if (dev < 0 || dev >= kMaxGpus) throw std::runtime_error("trace_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    int prev = 0;
+    HIP_OK(hipGetDevice(&prev));
+    if (prev != dev) HIP_OK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != dev) HIP_OK(hipSetDevice(prev));
+    if (e != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("trace_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratchPtr[dev] = ptr;
+  }
+  return g_scratchPtr[dev];
+}
+
+}  // namespace
+
+extern "C" int sundog_session_trace_noalloc(sundog_session* s, const float* d_rays, int n, float* d_t,
+                                            int32_t* d_inst, int32_t* d_prim) {
+  return guarded([&] {
+    if (!s || !d_rays || !d_t || !d_inst || !d_prim) throw std::runtime_error("trace_noalloc: null argument");
+    if (n <= 0) return SUNDOG_OK;
+    // the query kernel keeps stackLdsEntries() entries per lane in LDS;
+    // deeper entries of this scene's trees go to the scratch array, one
+    // column per thread of the grid
+    const size_t threads = ((size_t)n + 255) / 256 * 256;
+    const int rows = s->spillRowsFor(stackLdsEntries());
+    const size_t need = (size_t)rows * threads * kStackEntryBytes;
+    if (need > kTraceScratchBytes) {
+      const size_t maxRays = kTraceScratchBytes / ((size_t)rows * kStackEntryBytes) / 256 * 256;
+      char msg[256];
+      snprintf(msg, sizeof msg,
+               "trace_noalloc: %d rays need %zu bytes of traversal-stack scratch, more than the %zu bytes "
+               "available; at most %zu rays per call on this scene",
+               n, need, kTraceScratchBytes, maxRays);
+      throw std::runtime_error(msg);
+    }
+    int dev = 0;
+    HIP_OK(hipStreamGetDevice(s->stream, &dev));
+    LaunchParams P = s->P;
+    P.spill = static_cast<uint32_t*>(traceScratch(dev));
+    P.spillStride = rows > 0 ? (int)threads : 0;
+    P.spillRows = rows;
+    HIP_OK(launchTrace(P, d_rays, n, false, s->numInstances, d_t, d_inst, d_prim, nullptr, s->stream));
+    return SUNDOG_OK;
+  });
+}
+
 extern "C" int sundog_session_shadow(sundog_session* s, const float* rays, int n, float* tr_out) {
   return guarded([&] {
     if (!s || !rays || !tr_out) throw std::runtime_error("shadow: null argument");

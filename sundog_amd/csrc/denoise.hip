@@ -520,70 +520,206 @@ static void launchConv(int nt, dim3 grid, hipStream_t s, const void* in, const u
 }
 
 // ---------------------------------------------------------------------------
-// backward-weights: partial[k][co][t][ci] = sum over pixel chunk k of
-// dy[p][co] * x[p + off(t)][ci]; one wave per (co tile, ci tile, tap, chunk)
+// backward-weights (implicit GEMM: M = Cout, N = 3 taps x 32 input channels
+// per wave, K = output pixels):
+//   dW[co][t][ci] = sum_p dy[p][co] * x[p + off(t)][ci],  db[co] = sum_p dy[p][co]
+//
+// Output pixels are walked in tiles of kTC columns x (64 / kTC) rows of one
+// image; columns and rows past the image edge carry dy = 0, so any H x W
+// works. A block owns one 32-channel input slice and a run of consecutive
+// tiles (its K chunk) and writes FP32 partial sums, one row of
+// [Cout][9][Cin] weights followed by [Cout] biases per chunk; the chunks are
+// summed afterwards in a fixed order (deterministic).
+//
+// Per tile the block stages in LDS
+//   dys[co][64 px]               dy rounded to BF16, pixel-contiguous per co
+//   xs[dx + 1][ci][halo row][c]  the input slice shifted by dx = -1, 0, +1
+// so that each MFMA operand a lane needs (8 pixels of one channel, for both
+// A = dy^T and B = x) is one aligned 16-byte LDS read. Both strides carry 16
+// bytes of padding, which spreads every 8 lanes of a 16-byte access over
+// distinct banks. Wave w = 3 * coTile + tapRow forms the 32x32 tiles of its
+// co tile for the three taps of its tap row (dy = tapRow - 1): per 16-pixel
+// K step one A read and three B reads feed three MFMAs. The next tile's
+// global loads are issued into registers before the current tile's MFMAs.
+// The block's threads hold fixed co channels while staging dy (the block has
+// 6 * Cout threads), so each sums its own dy values in FP32 for the bias in
+// a fixed order, and the six per-channel sums are added in thread order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_conv_dw(const float* __restrict__ dy, const uint16_t* __restrict__ x,
-                                                float* __restrict__ partial, int N, int H, int W, int Cin,
-                                                int Cout, int chunk) {
-  const int lane = threadIdx.x;
-  const int co0 = blockIdx.x * kTile;
-  const int ciTiles = (Cin + kTile - 1) / kTile;
-  const int ci0 = (blockIdx.y % ciTiles) * kTile;
-  const int t = blockIdx.y / ciTiles;
-  const int k = blockIdx.z;
-  const int P = N * H * W;
-  const int row = lane & 31, half = lane >> 5;
-  const int dy_ = t / 3 - 1, dx_ = t % 3 - 1;
-  const int pBeg = k * chunk, pEnd = min(P, pBeg + chunk);
-  const int co = co0 + row, ci = ci0 + row;
-  const bool ciOk = ci < Cin;
-  f32x16 acc = {};
-  for (int q0 = pBeg; q0 < pEnd; q0 += 16) {
-    u16x8 a, b;
+template <int kTC, int kNco>
+struct DwTile {
+  static constexpr int kRows = 64 / kTC;            // output rows per tile
+  static constexpr int kHalo = kRows + 2;           // input rows per tile
+  static constexpr int kSegs = kTC / 8;             // 8-pixel segments per row
+  static constexpr int kXStride = kHalo * kTC + 8;  // BF16 per (dx, ci) plane
+  static constexpr int kDyStride = 64 + 8;          // BF16 per co row
+  static constexpr int kCout = 32 * kNco;
+  static constexpr int kThreads = 192 * kNco;       // 3 waves per co tile = 6 * kCout
+  static constexpr int kDyTasks = (kCout * 8 + kThreads - 1) / kThreads;
+  static constexpr int kXItems = 32 * kHalo * kSegs;
+  static constexpr int kXTasks = (kXItems + kThreads - 1) / kThreads;
+};
+
+struct DwArgs {
+  const float* dy;     // [N][H][W][Cout] FP32
+  const uint16_t* x;   // [N][H][W][Cin] BF16
+  float* partial;      // [chunks][Cout * 9 * Cin + Cout]
+  int N, H, W, Cin, Cout;
+  int tilesX, tilesY, tiles, tilesPerChunk;
+};
+
+template <int kTC, int kNco>
+struct DwRegs {
+  using T = DwTile<kTC, kNco>;
+  float dy[T::kDyTasks][8];
+  uint16_t x[T::kXTasks][10];  // input columns c0 - 1 .. c0 + 8 of one segment
+};
+
+template <int kTC, int kNco>
+__device__ __forceinline__ void dwLoad(const DwArgs& A, int tile, int ci0, DwRegs<kTC, kNco>& R) {
+  using T = DwTile<kTC, kNco>;
+  const int tid = threadIdx.x;
+  const int perImage = A.tilesX * A.tilesY;
+  const int n = tile / perImage, rem = tile - n * perImage;
+  const int y0 = (rem / A.tilesX) * T::kRows, x0 = (rem % A.tilesX) * kTC;
+#pragma unroll
+  for (int i = 0; i < T::kDyTasks; i++) {
+    const int task = tid + i * T::kThreads;
+    const int co = task % T::kCout, seg = task / T::kCout;
+    const int y = y0 + seg / T::kSegs, xc = x0 + 8 * (seg % T::kSegs);
+    const bool rowOk = task < T::kCout * 8 && y < A.H;
+    const float* src = A.dy + ((size_t)(n * A.H + (rowOk ? y : 0)) * A.W) * T::kCout + co;
+#pragma unroll
+    for (int j = 0; j < 8; j++) R.dy[i][j] = rowOk && xc + j < A.W ? src[(size_t)(xc + j) * T::kCout] : 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < T::kXTasks; i++) {
+    const int task = tid + i * T::kThreads;
+    const int cil = task % 32, rest = task / 32;
+    const int hr = rest % T::kHalo, cs = rest / T::kHalo;
+    const int ci = ci0 + cil, y = y0 - 1 + hr, xc = x0 + 8 * cs - 1;
+    const bool rowOk = task < T::kXItems && ci < A.Cin && y >= 0 && y < A.H;
+    const uint16_t* src = A.x + ((size_t)(n * A.H + (rowOk ? y : 0)) * A.W) * A.Cin + (rowOk ? ci : 0);
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+      const int c = xc + j;
+      R.x[i][j] = rowOk && c >= 0 && c < A.W ? src[(size_t)c * A.Cin] : (uint16_t)0;
+    }
+  }
+}
+
+template <int kTC, int kNco>
+__device__ __forceinline__ void dwStore(const DwRegs<kTC, kNco>& R, uint16_t* dys, uint16_t* xs, float& bias) {
+  using T = DwTile<kTC, kNco>;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < T::kDyTasks; i++) {
+    const int task = tid + i * T::kThreads;
+    if (task >= T::kCout * 8) continue;
+    const int co = task % T::kCout, seg = task / T::kCout;
+    u16x8 v;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      int q = q0 + 8 * half + j;
-      float av = 0.0f;
-      uint16_t bv = 0;
-      if (q < pEnd) {
-        av = dy[(size_t)q * Cout + co];
-        int n = q / (H * W), yx = q - n * H * W;
-        int y = yx / W, xx = yx - (yx / W) * W;
-        int sy = y + dy_, sx = xx + dx_;
-        if (ciOk && sy >= 0 && sy < H && sx >= 0 && sx < W) bv = x[(((size_t)n * H + sy) * W + sx) * Cin + ci];
-      }
-      a[j] = f2bf(av);
-      b[j] = bv;
+      v[j] = f2bf(R.dy[i][j]);
+      bias += R.dy[i][j];
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc, 0, 0, 0);
+    *(u16x8*)(dys + co * T::kDyStride + seg * 8) = v;
+  }
+#pragma unroll
+  for (int i = 0; i < T::kXTasks; i++) {
+    const int task = tid + i * T::kThreads;
+    if (task >= T::kXItems) continue;
+    const int cil = task % 32, rest = task / 32;
+    const int hr = rest % T::kHalo, cs = rest / T::kHalo;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      u16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = R.x[i][j + d];
+      *(u16x8*)(xs + (d * 32 + cil) * T::kXStride + hr * kTC + 8 * cs) = v;
+    }
+  }
+}
+
+template <int kTC, int kNco>
+__global__ __launch_bounds__(192 * kNco) void k_conv_dw(DwArgs A) {
+  using T = DwTile<kTC, kNco>;
+  __shared__ __align__(16) uint16_t dys[T::kCout * T::kDyStride];
+  __shared__ __align__(16) uint16_t xs[3 * 32 * T::kXStride];
+  __shared__ float biasPart[T::kThreads];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = lane & 31, half = lane >> 5;
+  const int coTile = wave / 3, tapRow = wave % 3;
+  const int chunk = blockIdx.x, ci0 = blockIdx.y * 32;
+  const int t0 = chunk * A.tilesPerChunk, t1 = min(A.tiles, t0 + A.tilesPerChunk);
+  f32x16 acc[3] = {};
+  float bias = 0.0f;
+  DwRegs<kTC, kNco> R;
+  dwLoad<kTC, kNco>(A, t0, ci0, R);
+  for (int tile = t0; tile < t1; tile++) {
+    if (tile > t0) __syncthreads();  // the previous tile's operands are consumed
+    dwStore<kTC, kNco>(R, dys, xs, bias);
+    __syncthreads();
+    if (tile + 1 < t1) dwLoad<kTC, kNco>(A, tile + 1, ci0, R);
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+      const int s = 2 * ks + half;  // this half-wave's 8-pixel segment
+      const int r = s / T::kSegs, cs = s % T::kSegs;
+      const u16x8 a = *(const u16x8*)(dys + (coTile * 32 + row) * T::kDyStride + s * 8);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const u16x8 b = *(const u16x8*)(xs + (d * 32 + row) * T::kXStride + (r + tapRow) * kTC + 8 * cs);
+        acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[d], 0, 0, 0);
+      }
+    }
   }
   // D[i = co offset][j = ci offset]; lane holds column j = row
+  const size_t nW = (size_t)T::kCout * 9 * A.Cin;
+  float* dst = A.partial + (size_t)chunk * (nW + T::kCout);
+  const int ci = ci0 + row;
+  if (ci < A.Cin) {
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    int i = 8 * (r / 4) + 4 * half + (r % 4);
-    if (ciOk)
-      partial[(((size_t)k * Cout + co0 + i) * 9 + t) * Cin + ci] = acc[r];
+    for (int d = 0; d < 3; d++) {
+      const int t = 3 * tapRow + d;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int i = 8 * (r / 4) + 4 * half + (r % 4);
+        dst[((size_t)(coTile * 32 + i) * 9 + t) * A.Cin + ci] = acc[d][r];
+      }
+    }
+  }
+  if (blockIdx.y == 0) {  // thread tid staged channel tid % kCout
+    biasPart[tid] = bias;
+    __syncthreads();
+    if (tid < T::kCout) {
+      float s = biasPart[tid];
+#pragma unroll
+      for (int g = 1; g < 6; g++) s += biasPart[g * T::kCout + tid];
+      dst[nW + tid] = s;
+    }
   }
 }
 
-// dW[e] = sum_k partial[k][e] in chunk order
-__global__ void k_reduce_chunks(const float* __restrict__ partial, float* __restrict__ out, int chunks, size_t n) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+// out[e] = sum over chunks k of partial[k][e] (rows of n floats) in a fixed
+// order: eight interleaved chunk groups summed in ascending chunk order, then
+// the group sums in ascending group order. A block covers 32 elements.
+__global__ __launch_bounds__(256) void k_reduce_chunks(const float* __restrict__ partial, float* __restrict__ out,
+                                                       int chunks, size_t n) {
+  __shared__ float part[8][32];
+  const int lane = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const size_t e = (size_t)blockIdx.x * 32 + lane;
   float s = 0.0f;
-  for (int k = 0; k < chunks; k++) s += partial[(size_t)k * n + e];
-  out[e] = s;
-}
-
-// db[co] = sum_p dy[p][co], fixed order: per-block partial sums then reduce
-__global__ void k_bias_grad(const float* __restrict__ dy, float* __restrict__ partial, int P, int C, int rowsPerBlock) {
-  int co = threadIdx.x;
-  if (co >= C) return;
-  int p0 = blockIdx.x * rowsPerBlock, p1 = min(P, p0 + rowsPerBlock);
-  float s = 0.0f;
-  for (int p = p0; p < p1; p++) s += dy[(size_t)p * C + co];
-  partial[(size_t)blockIdx.x * C + co] = s;
+  if (e < n) {
+#pragma unroll 4
+    for (int k = g; k < chunks; k += 8) s += partial[(size_t)k * n + e];
+  }
+  part[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && e < n) {
+    float t = part[0][lane];
+#pragma unroll
+    for (int q = 1; q < 8; q++) t += part[q][lane];
+    out[e] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -776,6 +912,10 @@ __global__ void k_to_bf16_layouts(const float* __restrict__ w, uint16_t* __restr
 #define DN_OK(expr) HIP_OK(expr)
 
 static inline unsigned blocks(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// target block count of a backward-weights launch (split over pixel chunks)
+constexpr int kDwBlocks = 512;
+constexpr size_t kDwPartialBytes = 16u << 20;
 
 struct Conv {
   int cin, cout;  // cin padded to a multiple of 16, cout to a multiple of 32
@@ -1029,27 +1169,48 @@ struct sundog_denoiser {
   // dW, db for conv k given dy (FP32, already relu-masked) and its input x
   void convGradW(int k, const float* dy, const uint16_t* x, int h, int w) {
     const Conv& c = net.convs[k];
-    int P = N * h * w;
-    int chunk = 4096;
-    int chunks = (P + chunk - 1) / chunk;
-    size_t nW = (size_t)c.cout * 9 * c.cin;
-    size_t need = std::max((size_t)chunks * nW, (size_t)((P + 255) / 256) * c.cout) * 4;
+    if (c.cout % 32 || c.cout > 160) throw std::runtime_error("denoiser: backward-weights needs Cout % 32 == 0, <= 160");
+    // tile width: the least padding past the image edge, wider on ties
+    int tc = 32;
+    for (int cand : {16, 8}) {
+      auto waste = [&](int t) { return (w + t - 1) / t * t - w; };
+      if (waste(cand) < waste(tc)) tc = cand;
+    }
+    DwArgs A;
+    A.dy = dy;
+    A.x = x;
+    A.N = N;
+    A.H = h;
+    A.W = w;
+    A.Cin = c.cin;
+    A.Cout = c.cout;
+    A.tilesX = (w + tc - 1) / tc;
+    A.tilesY = (h + 64 / tc - 1) / (64 / tc);
+    A.tiles = N * A.tilesX * A.tilesY;
+    const int ciTiles = (c.cin + 31) / 32;
+    // about kDwBlocks blocks, each over at least one tile, with at most
+    // kDwPartialBytes of partial sums
+    const size_t row = (size_t)c.cout * 9 * c.cin + c.cout;
+    const int maxChunks = (int)std::max<size_t>(1, kDwPartialBytes / (row * 4));
+    A.tilesPerChunk = std::max({1, (A.tiles * ciTiles + kDwBlocks - 1) / kDwBlocks,
+                                (A.tiles + maxChunks - 1) / maxChunks});
+    const int chunks = (A.tiles + A.tilesPerChunk - 1) / A.tilesPerChunk;
+    const size_t need = (size_t)chunks * row * 4;
     if (need > partialCap) {
       partial.alloc(need);
       partialCap = need;
     }
-    int ciTiles = (c.cin + 31) / 32;
-    dim3 grid(c.cout / 32, ciTiles * 9, chunks);
-    hipLaunchKernelGGL(k_conv_dw, grid, dim3(64), 0, stream, dy, x, partial.as<float>(), N, h, w, c.cin, c.cout,
-                       chunk);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(nW)), dim3(256), 0, stream, partial.as<float>(),
-                       net.grads.as<float>() + c.wOff, chunks, nW);
-    int rows = 256;
-    int nb = (P + rows - 1) / rows;
-    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(std::max(64, c.cout)), 0, stream, dy, partial.as<float>(), P,
-                       c.cout, rows);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(c.cout)), dim3(256), 0, stream, partial.as<float>(),
-                       net.grads.as<float>() + c.bOff, nb, (size_t)c.cout);
+    A.partial = partial.as<float>();
+    const dim3 grid(chunks, ciTiles);
+    const int nco = c.cout / 32;
+#define SD_DW(TC, NCO) \
+  if (tc == TC && nco == NCO) hipLaunchKernelGGL((k_conv_dw<TC, NCO>), grid, dim3(192 * NCO), 0, stream, A); else
+#define SD_DW_TC(TC) SD_DW(TC, 1) SD_DW(TC, 2) SD_DW(TC, 3) SD_DW(TC, 4) SD_DW(TC, 5)
+    SD_DW_TC(32) SD_DW_TC(16) SD_DW_TC(8) throw std::runtime_error("denoiser: no backward-weights variant");
+#undef SD_DW_TC
+#undef SD_DW
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(row, 32)), dim3(256), 0, stream, partial.as<float>(),
+                       net.grads.as<float>() + c.wOff, chunks, row);
     HIP_OK(hipGetLastError());
   }
 

@@ -15,8 +15,8 @@
 //
 // Activations are NHWC BF16, gradients NHWC FP32, master weights FP32 with
 // Adam; BF16 weight copies are kept in two layouts, [co][tap][ci] for the
-// forward pass and [ci][tap][co] for backward-data, so every MFMA operand a
-// lane needs is 16 contiguous bytes.
+// forward pass and [ci][8 - tap][co] for backward-data, so every MFMA operand
+// a lane needs is 16 contiguous bytes.
 //
 // Convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (FP32
 // accumulate): forward M = pixels, N = Cout, K = 9*Cin; backward-data M =
@@ -66,153 +66,22 @@ constexpr int kTile = 32;  // MFMA 32x32x16
 // ---------------------------------------------------------------------------
 // forward / backward-data convolution (implicit GEMM)
 // ---------------------------------------------------------------------------
-// in: [P][Cin] (BF16, or FP32 when kInF32), w: [Cout][9][Cin] BF16.
-// out[p][co] = bias[co] + sum_t sum_ci in[p + tapOff(t)][ci] * w[co][t][ci];
-// tapSign = +1 for forward, -1 for backward-data (the transposed conv reads
-// the neighbour on the opposite side). Cin % 16 == 0, Cout % (32 * kNT) == 0.
-// The network runs its backward-data pass here (FP32 gradient input); its
-// forward layers run the spatially tiled k_conv_halo below.
+// out[p][co] = bias[co] + sum_t sum_ci in[p + off(t)][ci] * w[co][t][ci],
+// off(t) = (t / 3 - 1, t % 3 - 1). The backward-data pass is the same
+// kernel over the FP32 gradient (HS_F32) with the transposed weights, taps
+// mirrored ([ci][8 - t][co], as off(8 - t) = -off(t)).
 //
-// A block is 4 waves x 32 pixels (M = 128) by 32 * kNT output channels (N).
-// K = 9 * Cin runs tap-major (k = tap * Cin + ci, the weights' own row
-// order) in chunks of ConvChunk<kNT>::kK elements: each chunk's weight slab
-// (N x chunk, the B operand of all four waves) is staged once per block in
-// LDS, double-buffered: the next chunk is prefetched into registers while
-// the current one feeds the MFMAs, then stored to the other buffer before
-// the chunk's single barrier. Chunks are longer for fewer output tiles (128
-// K elements for one or two tiles, 64 beyond, so the slab stays at 16-40 KB
-// per block). Each wave's activation fragment (its 32 pixels x
-// 16 channels of one tap, the A operand) is loaded from global memory once
-// per K step and reused by its kNT MFMAs. Vector v of a slab row r is stored
-// at v ^ (r & 7) (rows hold a multiple of 8 16-byte vectors) so the 32 rows
-// a half-wave reads land in 8 distinct 16-byte bank groups.
-template <int kNT>
-struct ConvChunk {
-  static constexpr int kK = kNT <= 2 ? 128 : 64;  // K elements per chunk
-  static constexpr int kVecs = kK / 8;                                 // 16-byte vectors per slab row
-  static constexpr int kRegs = kNT * 32 * kVecs / 256;                 // slab vectors per thread
-};
-
-template <int kNT>
-__device__ __forceinline__ void convLoadChunk(const uint16_t* __restrict__ w, int co0, int k0, int kn, int Ktot,
-                                              u16x8 (&reg)[ConvChunk<kNT>::kRegs]) {
-  constexpr int kVecs = ConvChunk<kNT>::kVecs;
-#pragma unroll
-  for (int k = 0; k < ConvChunk<kNT>::kRegs; k++) {
-    const int idx = (int)threadIdx.x + k * 256;
-    const int r = idx / kVecs, v = idx - r * kVecs;
-    reg[k] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    if (v * 8 < kn) reg[k] = *(const u16x8*)(w + (size_t)(co0 + r) * Ktot + k0 + v * 8);
-  }
-}
-
-template <int kNT>
-__device__ __forceinline__ void convStoreChunk(u16x8* sB, const u16x8 (&reg)[ConvChunk<kNT>::kRegs]) {
-  constexpr int kVecs = ConvChunk<kNT>::kVecs;
-#pragma unroll
-  for (int k = 0; k < ConvChunk<kNT>::kRegs; k++) {
-    const int idx = (int)threadIdx.x + k * 256;
-    const int r = idx / kVecs, v = idx - r * kVecs;
-    sB[r * kVecs + (v ^ (r & 7))] = reg[k];
-  }
-}
-
-template <bool kInF32, bool kOutF32, int kNT>
-__global__ __launch_bounds__(256) void k_conv(const void* __restrict__ inV, const uint16_t* __restrict__ w,
-                                              const float* __restrict__ bias, void* __restrict__ outV, int N,
-                                              int H, int W, int Cin, int Cout, int tapSign, int relu) {
-  constexpr int kK = ConvChunk<kNT>::kK, kVecs = ConvChunk<kNT>::kVecs;
-  __shared__ u16x8 sB[2][kNT * 32 * kVecs];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int P = N * H * W;
-  const int m0 = (blockIdx.x * 4 + wave) * kTile;
-  const int co0 = blockIdx.y * kTile * kNT;
-  // a wave past the last pixel still stages weights and meets every barrier
-  const bool waveActive = m0 < P;
-  const int row = lane & 31, half = lane >> 5;
-  const int p = m0 + row;
-  const bool pv = p < P;
-  const int n = pv ? p / (H * W) : 0;
-  const int yx = pv ? p - n * H * W : 0;
-  const int y = yx / W, x = yx - (yx / W) * W;
-  f32x16 acc[kNT];
-#pragma unroll
-  for (int j = 0; j < kNT; j++) acc[j] = f32x16{};
-  const int Ktot = 9 * Cin;
-  const int numChunks = (Ktot + kK - 1) / kK;
-  u16x8 reg[ConvChunk<kNT>::kRegs];
-  convLoadChunk<kNT>(w, co0, 0, min(kK, Ktot), Ktot, reg);
-  convStoreChunk<kNT>(sB[0], reg);
-  __syncthreads();
-  int tap = 0, ci = 0;  // (tap, channel) of the next K step
-  for (int q = 0; q < numChunks; q++) {
-    const int k0 = q * kK, kn = min(kK, Ktot - k0);
-    if (q + 1 < numChunks) convLoadChunk<kNT>(w, co0, k0 + kK, min(kK, Ktot - k0 - kK), Ktot, reg);
-    if (waveActive) {
-      const u16x8* slab = sB[q & 1];
-      for (int kk = 0; kk < kn; kk += 16) {
-        const int sy = y + tapSign * (tap / 3 - 1), sx = x + tapSign * (tap % 3 - 1);
-        const bool valid = pv && sy >= 0 && sy < H && sx >= 0 && sx < W;
-        const size_t src = ((size_t)n * H + sy) * W + sx;
-        u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (valid) {
-          if (kInF32) {
-            const float* ip = (const float*)inV + src * Cin + ci + 8 * half;
-            float4 f0 = *(const float4*)ip, f1 = *(const float4*)(ip + 4);
-            a = u16x8{f2bf(f0.x), f2bf(f0.y), f2bf(f0.z), f2bf(f0.w),
-                      f2bf(f1.x), f2bf(f1.y), f2bf(f1.z), f2bf(f1.w)};
-          } else {
-            a = *(const u16x8*)((const uint16_t*)inV + src * Cin + ci + 8 * half);
-          }
-        }
-        const int v = (kk >> 3) + half;
-#pragma unroll
-        for (int j = 0; j < kNT; j++) {
-          const int r = j * 32 + row;
-          u16x8 b = slab[r * kVecs + (v ^ (r & 7))];
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(asBf(a), asBf(b), acc[j], 0, 0, 0);
-        }
-        ci += 16;
-        if (ci == Cin) {
-          ci = 0;
-          tap++;
-        }
-      }
-    }
-    if (q + 1 < numChunks) convStoreChunk<kNT>(sB[(q + 1) & 1], reg);
-    __syncthreads();
-  }
-  if (!waveActive) return;
-#pragma unroll
-  for (int j = 0; j < kNT; j++) {
-    const int co = co0 + j * kTile + row;  // D column = lane % 32
-    const float bv = bias ? bias[co] : 0.0f;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      int i = 8 * (r / 4) + 4 * half + (r % 4);
-      int pp = m0 + i;
-      if (pp < P) {
-        float v = acc[j][r] + bv;
-        if (relu) v = fmaxf(v, 0.0f);
-        if (kOutF32) ((float*)outV)[(size_t)pp * Cout + co] = v;
-        else ((uint16_t*)outV)[(size_t)pp * Cout + co] = f2bf(v);
-      }
-    }
-  }
-}
-
-// Forward convolution (BF16 input), tiled in space: a work item is 8 x 16
+// Tiled in space: a work item is 8 x 16
 // output pixels of one image (wave w: rows 2w and 2w+1, 32 pixels) by one
 // block of kNT x 32 output channels; a layer has Cout / (32 kNT) channel
 // blocks, and the channel blocks of one tile are consecutive items, so they
-// run together and read the same halo from L2. Per chunk of up to 32 input channels the block stages the
-// 10 x 18 input halo (zero outside the image; each halo row is one
+// run together and read the same halo from L2. Per chunk of up to 32 input
+// channels the block stages the 10 x 18 input halo (zero outside the image; each halo row is one
 // contiguous run of pixels in memory) and all nine taps of the chunk's
 // weights in LDS, then runs the chunk's 9 x (chunk / 16) K steps from LDS
 // only: the nine taps of a pixel read the same halo instead of issuing nine
 // scattered 16-byte global loads per fragment. K runs chunk-major (chunk,
-// tap, channel); the sum is the same as the tap-major kernel's up to FP32
+// tap, channel); each output is the same sum as the direct one up to FP32
 // rounding order.
 //
 // Blocks are persistent: block b takes items b, b + gridDim.x, ... (so the
@@ -246,6 +115,7 @@ enum HaloSrc {
   HS_PLAIN = 0,  // in[P][Cin]
   HS_POOL = 1,   // 2x2 average of in[N][2H][2W][Cin], rounded to BF16 as k_pool_fwd does
   HS_UPCAT = 2,  // channels < ca: in[N][H/2][W/2][ca] (nearest up-sample); the rest: in2[P][Cin - ca]
+  HS_F32 = 3,    // inF[P][Cin] FP32 (gradients), rounded to BF16 as it is staged
 };
 enum HaloOut {
   HO_BF16 = 0,   // out[P][Cout] BF16
@@ -256,6 +126,7 @@ enum HaloOut {
 struct HaloArgs {
   const uint16_t* in;
   const uint16_t* in2;
+  const float* inF;
   const uint16_t* w;
   const float* bias;
   void* out;
@@ -284,6 +155,10 @@ __device__ __forceinline__ u16x8 haloPixel(const HaloArgs& A, int n, int gy, int
       return *(const u16x8*)(A.in + (((size_t)n * (A.H / 2) + gy / 2) * (A.W / 2) + gx / 2) * A.ca + c);
     const int cb = A.Cin - A.ca;
     return *(const u16x8*)(A.in2 + (((size_t)n * A.H + gy) * A.W + gx) * cb + (c - A.ca));
+  } else if (kSrc == HS_F32) {
+    const float* b = A.inF + (((size_t)n * A.H + gy) * A.W + gx) * A.Cin + c;
+    const float4 lo = *(const float4*)b, hi = *(const float4*)(b + 4);
+    return u16x8{f2bf(lo.x), f2bf(lo.y), f2bf(lo.z), f2bf(lo.w), f2bf(hi.x), f2bf(hi.y), f2bf(hi.z), f2bf(hi.w)};
   } else {
     return *(const u16x8*)(A.in + (((size_t)n * A.H + gy) * A.W + gx) * A.Cin + c);
   }
@@ -492,31 +367,6 @@ template <int kOut, int kNT, int kSrc>
 static void launchHalo(const HaloArgs& A, hipStream_t s) {
   const int g = std::min(A.numItems, haloResident<kOut, kNT, kSrc>());
   hipLaunchKernelGGL((k_conv_halo<kOut, kNT, kSrc>), dim3(g), dim3(256), 0, s, A);
-}
-
-// Output-channel tiles per wave for a convolution with `tiles` 32-wide tiles:
-// the largest divisor up to 5 (more accumulators cost occupancy: 5 tiles take
-// ~172 registers, 2 waves per SIMD).
-static int convTilesPerWave(int tiles) {
-  for (int k : {5, 4, 3, 2})
-    if (tiles % k == 0) return k;
-  return 1;
-}
-
-template <bool kInF32, bool kOutF32>
-static void launchConv(int nt, dim3 grid, hipStream_t s, const void* in, const uint16_t* w, const float* b, void* out,
-                       int N, int H, int W, int Cin, int Cout, int tapSign, int relu) {
-#define SD_CONV(K)                                                                                             \
-  case K:                                                                                                      \
-    hipLaunchKernelGGL((k_conv<kInF32, kOutF32, K>), grid, dim3(256), 0, s, in, w, b, out, N, H, W, Cin, Cout, \
-                       tapSign, relu);                                                                         \
-    break;
-  switch (nt) {
-    SD_CONV(1) SD_CONV(2) SD_CONV(3) SD_CONV(4) SD_CONV(5)
-    default:
-      throw std::runtime_error("denoiser: unsupported tile count");
-  }
-#undef SD_CONV
 }
 
 // ---------------------------------------------------------------------------
@@ -902,7 +752,7 @@ __global__ void k_to_bf16_layouts(const float* __restrict__ w, uint16_t* __restr
   int co = i / ((size_t)9 * Cin);
   uint16_t b = f2bf(w[i]);
   wf[i] = b;
-  wt[((size_t)ci * 9 + t) * Cout + co] = b;
+  wt[((size_t)ci * 9 + (8 - t)) * Cout + co] = b;  // tap 8 - t has the opposite offset
 }
 
 // ---------------------------------------------------------------------------
@@ -1085,6 +935,7 @@ struct sundog_denoiser {
     HaloArgs A;
     A.in = in;
     A.in2 = in2;
+    A.inF = nullptr;
     A.w = c.wf.as<uint16_t>();
     A.bias = net.params.as<float>() + c.bOff;
     A.out = out;
@@ -1214,14 +1065,31 @@ struct sundog_denoiser {
     HIP_OK(hipGetLastError());
   }
 
-  // dx = conv^T(dy); dx may be null (network input)
+  // dx = conv^T(dy): the forward kernel over dy (FP32, rounded to BF16 as
+  // it is staged) with the transposed, tap-mirrored weights [ci][8 - t][co]
   void convGradX(int k, const float* dy, float* dx, int h, int w) {
     const Conv& c = net.convs[k];
-    int P = N * h * w;
     if (c.cin % 32) throw std::runtime_error("denoiser: backward-data needs Cin % 32 == 0");
-    const int nt = convTilesPerWave(c.cin / 32);
-    dim3 grid((P + 127) / 128, c.cin / (32 * nt));
-    launchConv<true, true>(nt, grid, stream, dy, c.wt.as<uint16_t>(), nullptr, dx, N, h, w, c.cout, c.cin, -1, 0);
+    HaloArgs A;
+    A.in = nullptr;
+    A.in2 = nullptr;
+    A.inF = dy;
+    A.w = c.wt.as<uint16_t>();
+    A.bias = nullptr;
+    A.out = dx;
+    A.raw = nullptr;
+    A.H = h;
+    A.W = w;
+    A.Cin = c.cout;
+    A.Cout = c.cin;
+    A.ca = 0;
+    A.relu = 0;
+    const int ht = (c.cin / 32) % 2 == 0 ? 2 : 1;
+    A.numItems = N * ((w + kHaloTW - 1) / kHaloTW) * ((h + kHaloTH - 1) / kHaloTH) * (c.cin / (32 * ht));
+    if (ht == 2)
+      launchHalo<HO_F32, 2, HS_F32>(A, stream);
+    else
+      launchHalo<HO_F32, 1, HS_F32>(A, stream);
     HIP_OK(hipGetLastError());
   }
 

@@ -727,11 +727,36 @@ __global__ void k_unpack_output(const float* __restrict__ fin, float* __restrict
   out[i] = expm1f(fin[i]);
 }
 
-// Adam on FP32 masters, then the two BF16 layouts. Weights of one conv are
-// [co][t][ci] (Cin padded), followed by bias[co].
+// Where each convolution's weights sit in the flat parameter vector
+// ([co][t][ci], Cin padded, followed by bias[co]) and its two BF16 copies.
+struct WeightLayout {
+  size_t wOff, wEnd;  // weights: [wOff, wEnd)
+  int cin, cout;
+  uint16_t* wf;       // [co][t][ci]
+  uint16_t* wt;       // [ci][8 - t][co]
+};
+
+// the BF16 copies of parameter i (with value w) if it is a weight; the
+// table is in parameter order
+__device__ __forceinline__ void storeLayouts(const WeightLayout* __restrict__ L, int count, size_t i, float w) {
+  for (int k = 0; k < count; k++) {
+    if (i < L[k].wOff) return;  // the previous convolution's bias
+    if (i < L[k].wEnd) {
+      const size_t e = i - L[k].wOff;
+      const int cin = L[k].cin;
+      const int ci = (int)(e % cin), t = (int)((e / cin) % 9), co = (int)(e / ((size_t)9 * cin));
+      const uint16_t b = f2bf(w);
+      L[k].wf[e] = b;
+      L[k].wt[((size_t)ci * 9 + (8 - t)) * L[k].cout + co] = b;  // tap 8 - t has the opposite offset
+      return;
+    }
+  }
+}
+
+// Adam on the FP32 masters, then the BF16 layouts of the updated weights
 __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                        const float* __restrict__ g, size_t n, float lr, float b1, float b2, float eps, float c1,
-                       float c2) {
+                       float c2, const WeightLayout* __restrict__ L, int count) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float gi = g[i];
@@ -739,20 +764,15 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __re
   float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  storeLayouts(L, count, i, pi);
 }
 
-__global__ void k_to_bf16_layouts(const float* __restrict__ w, uint16_t* __restrict__ wf, uint16_t* __restrict__ wt,
-                                  int Cin, int Cout) {
+__global__ void k_to_bf16_layouts(const float* __restrict__ p, size_t n, const WeightLayout* __restrict__ L,
+                                  int count) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t n = (size_t)Cout * 9 * Cin;
-  if (i >= n) return;
-  int ci = i % Cin;
-  int t = (i / Cin) % 9;
-  int co = i / ((size_t)9 * Cin);
-  uint16_t b = f2bf(w[i]);
-  wf[i] = b;
-  wt[((size_t)ci * 9 + (8 - t)) * Cout + co] = b;  // tap 8 - t has the opposite offset
+  if (i < n) storeLayouts(L, count, i, p[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -778,6 +798,7 @@ struct Net {
   std::vector<Conv> convs;
   size_t nParams = 0;
   DevBuf params, grads, adamM, adamV;
+  DevBuf layouts;  // WeightLayout per conv, in parameter order
   int step = 0;
 
   void addConv(int cin, int cout, bool relu) {
@@ -859,20 +880,21 @@ struct sundog_denoiser {
     n.adamV.alloc(n.nParams * 4);
     HIP_OK(hipMemsetAsync(n.adamM.p, 0, n.nParams * 4, stream));
     HIP_OK(hipMemsetAsync(n.adamV.p, 0, n.nParams * 4, stream));
+    std::vector<WeightLayout> table;
     for (Conv& c : n.convs) {
       c.wf.alloc((size_t)c.cout * 9 * c.cin * 2);
       c.wt.alloc((size_t)c.cout * 9 * c.cin * 2);
+      table.push_back({c.wOff, c.wOff + (size_t)c.cout * 9 * c.cin, c.cin, c.cout, c.wf.as<uint16_t>(),
+                       c.wt.as<uint16_t>()});
     }
+    n.layouts.upload(table, stream);
     syncWeights();
     HIP_OK(hipStreamSynchronize(stream));
   }
 
   void syncWeights() {
-    for (Conv& c : net.convs) {
-      size_t n = (size_t)c.cout * 9 * c.cin;
-      hipLaunchKernelGGL(k_to_bf16_layouts, dim3(blocks(n)), dim3(256), 0, stream,
-                         net.params.as<float>() + c.wOff, c.wf.as<uint16_t>(), c.wt.as<uint16_t>(), c.cin, c.cout);
-    }
+    hipLaunchKernelGGL(k_to_bf16_layouts, dim3(blocks(net.nParams)), dim3(256), 0, stream, net.params.as<float>(),
+                       net.nParams, net.layouts.as<WeightLayout>(), (int)net.convs.size());
     HIP_OK(hipGetLastError());
   }
 
@@ -1200,8 +1222,8 @@ struct sundog_denoiser {
     float c1 = 1.0f - powf(b1, (float)net.step), c2 = 1.0f - powf(b2, (float)net.step);
     hipLaunchKernelGGL(k_adam, dim3(blocks(net.nParams)), dim3(256), 0, stream, net.params.as<float>(),
                        net.adamM.as<float>(), net.adamV.as<float>(), net.grads.as<float>(), net.nParams, lr, b1, b2,
-                       1e-8f, c1, c2);
-    syncWeights();
+                       1e-8f, c1, c2, net.layouts.as<WeightLayout>(), (int)net.convs.size());
+    HIP_OK(hipGetLastError());
   }
 };
 

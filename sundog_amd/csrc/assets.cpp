@@ -263,6 +263,14 @@ HostImage loadTextureImage(const std::string& path, bool srgb) {
       fclose(f);
       throw std::runtime_error("texture '" + path + "': truncated PFM");
     }
+    if (scale > 0.0f) {  // a positive scale marks big-endian samples
+      for (float& v : raw) {
+        uint32_t u;
+        memcpy(&u, &v, 4);
+        u = __builtin_bswap32(u);
+        memcpy(&v, &u, 4);
+      }
+    }
     img.width = w;
     img.height = h;
     img.rgba.resize((size_t)w * h * 4);

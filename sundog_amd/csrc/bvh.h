@@ -57,4 +57,15 @@ struct Bvh4 {
 // prims: one box per primitive. maxLeaf: primitives per leaf (>= 1).
 Bvh4 buildBvh4(const std::vector<Aabb>& prims, int maxLeaf);
 
+// Points in structure-of-arrays form, padded to a multiple of 4 by repeating
+// the last point.
+struct SoaPoints {
+  std::vector<float> x, y, z;
+};
+SoaPoints toSoa(const std::vector<vec3>& pts);
+
+// Bounds of the points under `a`, each coordinate computed as Affine::point
+// computes it (so the box equals the one grown from a.point(p)).
+Aabb transformedBounds(const Affine& a, const SoaPoints& pts);
+
 }  // namespace sdk

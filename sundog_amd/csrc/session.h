@@ -338,6 +338,10 @@ inline Aabb transformBox(const Affine& m, const Aabb& b) {
 inline std::vector<Aabb> objectWorldBoxes(const Scene& S, const std::vector<HostMesh>& meshes,
                                           const std::vector<Aabb>& meshBounds) {
   std::vector<Aabb> out(S.objects.size());
+  std::vector<SoaPoints> soa(meshes.size());
+  for (const Object& o : S.objects)
+    if (o.geomKind == GK_MESH && soa[o.meshId].x.empty() && !meshes[o.meshId].positions.empty())
+      soa[o.meshId] = toSoa(meshes[o.meshId].positions);
   auto work = [&](size_t begin, size_t end) {
     for (size_t i = begin; i < end; i++) {
       const Object& o = S.objects[i];
@@ -345,14 +349,11 @@ inline std::vector<Aabb> objectWorldBoxes(const Scene& S, const std::vector<Host
         out[i] = transformBox(o.xform, quadricBox(o.geomKind));
         continue;
       }
-      const std::vector<vec3>& pts = meshes[o.meshId].positions;
-      if (pts.empty()) {
+      if (meshes[o.meshId].positions.empty()) {
         out[i] = transformBox(o.xform, meshBounds[o.meshId]);
         continue;
       }
-      Aabb r;
-      for (const vec3& p : pts) r.grow(o.xform.point(p));
-      out[i] = padTransformed(r);
+      out[i] = padTransformed(transformedBounds(o.xform, soa[o.meshId]));
     }
   };
   const size_t n = out.size();

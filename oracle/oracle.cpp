@@ -1138,9 +1138,9 @@ TriRay triSetup(V d) {
   r.ky = (r.kx + 1) % 3;
   float dz = at(d, r.kz);
   if (dz < 0.0f) std::swap(r.kx, r.ky);
-  r.Sx = at(d, r.kx) / dz;
-  r.Sy = at(d, r.ky) / dz;
-  r.Sz = 1.0f / dz;
+  r.Sz = 1.0f / dz;  // shear factors as products with 1 / d_z, as the GPU computes them
+  r.Sx = at(d, r.kx) * r.Sz;
+  r.Sy = at(d, r.ky) * r.Sz;
   return r;
 }
 bool triIntersect(const TriRay& R, V o, V v0, V v1, V v2, float tmin, float tmax, float& t, float& b1,

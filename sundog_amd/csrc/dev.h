@@ -150,6 +150,7 @@ struct LaunchParams {
   int32_t spillRows;
   int32_t stackLds;          // LDS stack entries per lane of the megakernel
   uint32_t tlasRoot;         // packed reference of the top-level root
+  int32_t instFlags;         // OR of every instance's flags (INST_MASKED, INST_TRANSMISSIVE)
 };
 
 // Persistent tile-queue work item: one tile of one request for one pass.

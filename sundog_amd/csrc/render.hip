@@ -26,8 +26,11 @@ constexpr int kStackLds = 16;  // 8-byte (reference, entry distance) LDS entries
 constexpr int kStackLdsMegakernel = 12;
 
 // Slab test against the 4 child boxes of one node. Returns per-child entry
-// distance, or +inf when missed. tFar is widened by 2*gamma(3) so boxes stay
-// conservative under rounding (Ize, "Robust BVH Ray Traversal", JCGT 2013).
+// distance, or +inf when missed. Each slab distance carries the rounding of
+// the subtraction and the product and the <= 1 ulp error of the hardware
+// reciprocal that makes `inv` (4u relative in all); tFar is widened by twice
+// that, 8u, and a little more, so boxes stay conservative under rounding
+// (Ize, "Robust BVH Ray Traversal", JCGT 2013, with gamma(4) for gamma(3)).
 DEV void boxes4(const BvhNode4& nd, vec3 org, vec3 inv, float tmin, float tmax, float dist[4]) {
   const float4* q = reinterpret_cast<const float4*>(&nd);
   float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
@@ -41,7 +44,7 @@ DEV void boxes4(const BvhNode4& nd, vec3 org, vec3 inv, float tmin, float tmax, 
     float t0z = (lzs[i] - org.z) * inv.z, t1z = (hzs[i] - org.z) * inv.z;
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
     float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-    dist[i] = (tn <= tf * 1.0000004f) ? tn : __builtin_inff();
+    dist[i] = (tn <= tf * 1.0000006f) ? tn : __builtin_inff();
   }
 }
 
@@ -179,14 +182,19 @@ DEV RayState makeRay(vec3 o, vec3 d) {
   RayState r;
   r.o = o;
   r.d = d;
-  r.inv = vec3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  // only the slab test reads it, whose widening covers the reciprocal's
+  // error; a zero component gives an infinite slope as with a division
+  r.inv = vec3{__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z)};
   return r;
 }
 
 // Closest hit with masking. Ties in t resolve to the smaller (instance,
-// primitive) so the answer does not depend on traversal order.
-DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
-                      vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
+// primitive) so the answer does not depend on traversal order. kMasked =
+// false compiles the query for scenes without masked instances (no
+// pass-through faces, no cutouts), where maskAccepts always accepts.
+template <bool kMasked>
+DEV bool traceClosestT(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
+                       vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
   hit.t = tmax;
   hit.inst = 0x7fffffff;
   hit.prim = 0x7fffffff;
@@ -194,7 +202,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
   bool found = false;
   RayState world = makeRay(o, d);
   RayState obj = world;
-  RayTri rt = rayTriSetup(d);
+  RayTri rt{2, 0, 1, 0.0f, 0.0f, 1.0f};  // set on entering a mesh instance, before any triangle test
   int curInst = -1;
   bool inBlas = false;
   const int base = st.sp;
@@ -246,7 +254,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
       float ts[2];
       int n = quadricRoots(in.geomKind, oo, dd, tmin, candBound(hit.t), ts);
       for (int k = 0; k < n; k++) {
-        if (closer(ts[k], ii, k, hit) && maskAccepts(P, in, k, 0.0f, 0.0f, oo, dd, ts[k])) {
+        if (closer(ts[k], ii, k, hit) && (!kMasked || maskAccepts(P, in, k, 0.0f, 0.0f, oo, dd, ts[k]))) {
           hit.t = ts[k];
           hit.inst = ii;
           hit.prim = k;
@@ -267,7 +275,7 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
         float t, b1, b2;
         if (triHit(rt, obj.o, f4xyz(a), f4xyz(b), f4xyz(c), tmin, candBound(hit.t), t, b1, b2)) {
           int prim = __float_as_int(a.w);
-          if (closer(t, curInst, prim, hit) && maskAccepts(P, in, prim, b1, b2, obj.o, obj.d, t)) {
+          if (closer(t, curInst, prim, hit) && (!kMasked || maskAccepts(P, in, prim, b1, b2, obj.o, obj.d, t))) {
             hit.t = t;
             hit.inst = curInst;
             hit.prim = prim;
@@ -280,8 +288,9 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
         }
       }
     }
-    // pop; an entry whose box starts beyond the current closest hit (the
-    // same widened comparison as the box test) cannot hold a closer hit
+    // pop; an entry whose box starts beyond the current closest hit cannot
+    // hold a closer hit (the entry distance is within 4u of exact, the
+    // widening here is 6.7u)
     while (true) {
       if (st.sp == base) return found;
       uint2 e = st.pop();
@@ -298,14 +307,17 @@ DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tm
 // Shadow transmittance along (tmin, tmax) (device/programs.cu:70-85,
 // :247-292): opaque => 0; with transparent shadows, each glass / water
 // crossing multiplies (1 - F) and adds its signed sigma * t; every primitive
-// is visited once, so each crossing counts once.
-DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmax, WorkCount& wc) {
+// is visited once, so each crossing counts once. kSurfaces = false compiles
+// the query for scenes without masked or transmissive instances, where every
+// primitive hit occludes.
+template <bool kSurfaces>
+DEV vec3 traceShadowT(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmax, WorkCount& wc) {
   const float tmin = 0.0f;
   float trF = 1.0f;
   vec3 tau = v3(0.0f);
   RayState world = makeRay(o, d);
   RayState obj = world;
-  RayTri rt = rayTriSetup(d);
+  RayTri rt{2, 0, 1, 0.0f, 0.0f, 1.0f};  // set on entering a mesh instance, before any triangle test
   int curInst = -1;
   bool inBlas = false;
   const int base = st.sp;
@@ -373,6 +385,10 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
           prim = k;
         }
         if (!h) continue;
+        if (!kSurfaces) {
+          occluded = true;
+          break;
+        }
         uint32_t side;
         ShadePoint sp;
         bool needSp = (in.flags & (INST_MASKED | INST_TRANSMISSIVE)) != 0;
@@ -425,6 +441,18 @@ DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tma
   vec3 tr = v3(trF);
   if (maxc(tau) > 0.0f) tr = tr * vexp(-tau);
   return tr;
+}
+
+// The queries for a launch's scene: the specialisations above when no
+// instance needs surface tests inside traversal.
+DEV bool traceClosest(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmin, float tmax, Hit& hit,
+                      vec3& oObjOut, vec3& dObjOut, WorkCount& wc) {
+  if (P.instFlags & INST_MASKED) return traceClosestT<true>(P, st, o, d, tmin, tmax, hit, oObjOut, dObjOut, wc);
+  return traceClosestT<false>(P, st, o, d, tmin, tmax, hit, oObjOut, dObjOut, wc);
+}
+DEV vec3 traceShadow(const LaunchParams& P, Stack& st, vec3 o, vec3 d, float tmax, WorkCount& wc) {
+  if (P.instFlags & (INST_MASKED | INST_TRANSMISSIVE)) return traceShadowT<true>(P, st, o, d, tmax, wc);
+  return traceShadowT<false>(P, st, o, d, tmax, wc);
 }
 
 // ---------------------------------------------------------------------------

@@ -842,6 +842,8 @@ struct sundog_session {
     p.lightPos = lightPos.as<float>();
     p.lightCdf = lightCdf.as<float>();
     p.tlasRoot = 0;
+    p.instFlags = 0;
+    for (const DevInstance& d : instHost) p.instFlags |= d.flags;
     reset();
     HIP_OK(hipStreamSynchronize(stream));
     buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

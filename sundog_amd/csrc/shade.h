@@ -1049,7 +1049,10 @@ SDEV void quadricSurface(int kind, vec3 p, vec3& n, float& u, float& v) {
 }
 
 // Watertight ray / triangle test (Woop, Benthin, Wald, JCGT 2013): the ray
-// is sheared so it runs along +z; per-ray constants below.
+// is sheared so it runs along +z; per-ray constants below. The shear
+// factors are the reciprocal 1 / d_z times d_x and d_y (one division where
+// the paper divides three times); every triangle a ray meets is sheared by
+// the same factors, so shared edges still evaluate alike.
 struct RayTri {
   int kx, ky, kz;
   float Sx, Sy, Sz;
@@ -1068,9 +1071,9 @@ SDEV RayTri rayTriSetup(vec3 d) {
     r.kx = r.ky;
     r.ky = k;
   }
-  r.Sx = comp(d, r.kx) / dz;
-  r.Sy = comp(d, r.ky) / dz;
   r.Sz = 1.0f / dz;
+  r.Sx = comp(d, r.kx) * r.Sz;
+  r.Sy = comp(d, r.ky) * r.Sz;
   return r;
 }
 

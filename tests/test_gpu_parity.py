@@ -79,6 +79,32 @@ def test_closest_hit_bit_exact(scene_name, orc, asset_dir):
         assert np.array_equal(t_bvh.view(np.uint32), t.view(np.uint32))
 
 
+def test_non_finite_rays_miss(orc, asset_dir):
+    """Rays with a NaN or infinite origin or direction hit nothing: closest
+    hit = miss and shadow = unoccluded, on the GPU BVH, GPU brute force and
+    the oracle alike. (NaN slabs pass every box test, so such a query visits
+    every node; no primitive test accepts it. The scene is small.)"""
+    sc = configs.instanced(asset_dir, width=32, height=18, spp=1, instances=24, subdiv=2).build()
+    sess = sc.session()
+    flat = sc.flatten()
+    rays = _rays_looking_at(64, 21, [-60, 2, -60], [60, 30, 60], [-50, 0, -50], [50, 3, 50])
+    bad = rays.copy()
+    bad[0:16, 4] = np.nan            # NaN direction component
+    bad[16:32, 0:3] = np.nan         # NaN origin
+    bad[32:48, 1] = np.inf           # infinite origin component
+    bad[48:64, 4:7] = [np.inf, 0.0, 0.0]
+    t, i, p = sess.trace(bad)
+    t_bf, i_bf, _ = sess.trace(bad, brute=True)
+    t_c, i_c, _ = orc.trace(flat, bad, brute=True)
+    assert (i == -1).all() and (p == -1).all() and np.isinf(t).all()
+    assert (i_bf == -1).all() and (i_c == -1).all()
+    # the finite rays of the same batch still hit
+    t2, i2, _ = sess.trace(rays)
+    assert (i2 >= 0).sum() > 16
+    bad[:, 7] = 50.0
+    assert np.array_equal(sess.shadow(bad), np.ones((64, 3), np.float32))
+
+
 def test_shadow_transmittance_matches_oracle(orc, asset_dir):
     sc = configs.volumes(asset_dir, width=32, height=18, spp=1, env_w=64, env_h=32).build()
     sess = sc.session()

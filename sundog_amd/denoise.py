@@ -190,7 +190,7 @@ def render_pair(scene_fn, noisy_spp, ref_spp, seed):
     return make_input(b, a, n), rb[..., :3].copy()
 
 
-def dataset(asset_dir, n_variants, width, height, noisy_spp, ref_spp, first_variant=0, seed=11):
+def dataset(asset_dir, n_variants, width, height, noisy_spp, ref_spp, first_variant=0, seed=11, verbose=False):
     xs, ys = [], []
     for v in range(first_variant, first_variant + n_variants):
         # the three benchmark frame families in turn: Cornell, volumes, instanced
@@ -203,14 +203,29 @@ def dataset(asset_dir, n_variants, width, height, noisy_spp, ref_spp, first_vari
         x, y = render_pair(fn, noisy_spp, ref_spp, seed + 101 * v)
         xs.append(x)
         ys.append(y)
+        if verbose:
+            print("dataset: variant %d (%s) %dx%d rendered" % (v, ("cornell", "volumes", "instanced")[v % 3],
+                                                             width, height), flush=True)
     return np.stack(xs), np.stack(ys)
 
 
-def random_crops(xs, ys, rng, batch, crop):
+def crop_list(n, n_images, height, width, crop, seed=5):
+    """A fixed training set of n crops: (image, top row, left column)."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.integers(0, n_images, n), rng.integers(0, height - crop + 1, n),
+                     rng.integers(0, width - crop + 1, n)], axis=1)
+
+
+def random_crops(xs, ys, rng, batch, crop, picks=None):
+    """A training batch: random crops, or the given (image, row, column)
+    crops; each flipped horizontally at random."""
     n, h, w, _ = xs.shape
-    bi = rng.integers(0, n, batch)
-    oy = rng.integers(0, h - crop + 1, batch)
-    ox = rng.integers(0, w - crop + 1, batch)
+    if picks is None:
+        bi = rng.integers(0, n, batch)
+        oy = rng.integers(0, h - crop + 1, batch)
+        ox = rng.integers(0, w - crop + 1, batch)
+    else:
+        bi, oy, ox = picks[:, 0], picks[:, 1], picks[:, 2]
     cx = np.stack([xs[b, y:y + crop, x:x + crop] for b, y, x in zip(bi, oy, ox)])
     cy = np.stack([ys[b, y:y + crop, x:x + crop] for b, y, x in zip(bi, oy, ox)])
     flip = rng.integers(0, 2, batch).astype(bool)
@@ -221,11 +236,19 @@ def random_crops(xs, ys, rng, batch, crop):
     return cx, cy
 
 
-def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0):
+def train(den, xs, ys, steps, batch=4, crop=64, lr=2e-3, seed=0, log_every=0, crops=None):
+    """Adam steps with a decaying learning rate; with `crops` (crop_list)
+    the batches walk that fixed set in a new random order every epoch."""
     rng = np.random.default_rng(seed)
     losses = []
+    order = np.zeros(0, np.int64)
     for i in range(steps):
-        cx, cy = random_crops(xs, ys, rng, batch, crop)
+        picks = None
+        if crops is not None:
+            if len(order) < batch:
+                order = np.concatenate([order, rng.permutation(len(crops))])
+            picks, order = crops[order[:batch]], order[batch:]
+        cx, cy = random_crops(xs, ys, rng, batch, crop, picks)
         cur_lr = lr * (0.1 ** (i / max(steps, 1)))
         losses.append(den.train_step(cx, cy, cur_lr))
         if log_every and (i + 1) % log_every == 0:
@@ -240,14 +263,20 @@ def measure(args):
     at --width x --height: --input-spp noisy + denoise against a plain
     --target-spp render."""
     t0 = time.time()
-    xa, ya = dataset(args.asset_dir, args.variants // 2, args.size, args.size, 8, args.ref_spp)
-    xb, yb = dataset(args.asset_dir, args.variants - args.variants // 2, args.size, args.size, 16, args.ref_spp,
-                     first_variant=args.variants // 2)
-    xs, ys = np.concatenate([xa, xb]), np.concatenate([ya, yb])
+    if args.noisy_spp > 0:
+        xs, ys = dataset(args.asset_dir, args.variants, args.size, args.size, args.noisy_spp, args.ref_spp,
+                         verbose=True)
+    else:  # half the variants at 8 spp, half at 16
+        xa, ya = dataset(args.asset_dir, args.variants // 2, args.size, args.size, 8, args.ref_spp, verbose=True)
+        xb, yb = dataset(args.asset_dir, args.variants - args.variants // 2, args.size, args.size, 16, args.ref_spp,
+                         first_variant=args.variants // 2, verbose=True)
+        xs, ys = np.concatenate([xa, xb]), np.concatenate([ya, yb])
     data_s = time.time() - t0
+    crops = crop_list(args.crops, len(xs), args.size, args.size, args.crop) if args.crops > 0 else None
     den = Denoiser(seed=1)
     t1 = time.time()
-    losses = train(den, xs, ys, args.steps, batch=8, crop=args.crop, lr=1e-3, log_every=max(args.steps // 10, 1))
+    losses = train(den, xs, ys, args.steps, batch=8, crop=args.crop, lr=1e-3, log_every=max(args.steps // 10, 1),
+                   crops=crops)
     train_s = time.time() - t1
     if args.out:
         den.save(args.out)
@@ -275,7 +304,9 @@ def measure(args):
         print("%s: noisy %.4f denoised %.4f" % (name, frames[name]["rel_rmse_noisy"],
                                                  frames[name]["rel_rmse_denoised"]), flush=True)
     ms, fl = den.time_forward(1, (h + 15) // 16 * 16, (w + 15) // 16 * 16, iters=5)
-    res = {"train_steps": args.steps, "train_s": train_s, "data_s": data_s,
+    res = {"variants": args.variants, "render_size": args.size, "noisy_spp": args.noisy_spp or "8 and 16",
+           "ref_spp": args.ref_spp, "crops": args.crops or "random", "crop": args.crop,
+           "train_steps": args.steps, "train_s": train_s, "data_s": data_s,
            "final_loss": float(np.mean(losses[-50:])),
            "frame": [w, h], "input_spp": args.input_spp, "target_spp": args.target_spp, "target_render_s": ref_s,
            "held_out": frames,
@@ -325,11 +356,14 @@ def main(argv=None):
     t.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
     ms = sub.add_parser("measure", help="train, then denoise a held-out 1080p frame and compare with a "
                                         "plain high-spp render of it")
-    ms.add_argument("--steps", type=int, default=6000)
-    ms.add_argument("--variants", type=int, default=24)
-    ms.add_argument("--size", type=int, default=256)
+    ms.add_argument("--steps", type=int, default=7500)
+    ms.add_argument("--variants", type=int, default=36)
+    ms.add_argument("--size", type=int, default=1024)
     ms.add_argument("--ref-spp", type=int, default=4096)
-    ms.add_argument("--crop", type=int, default=128)
+    ms.add_argument("--noisy-spp", type=int, default=8, help="input spp of the training renders (0: half 8, half 16)")
+    ms.add_argument("--crops", type=int, default=20000,
+                    help="a fixed set of this many training crops (0: fresh random crops every step)")
+    ms.add_argument("--crop", type=int, default=256)
     ms.add_argument("--width", type=int, default=1920)
     ms.add_argument("--height", type=int, default=1080)
     ms.add_argument("--input-spp", type=int, default=16)

@@ -47,3 +47,33 @@ def test_converged_parity_4096spp(name, w, h, stride, orc, asset_dir):
     print("%s %dx%d @ %d spp: relRMSE %.5f" % (name, w, h, SPP, err))
     assert np.isfinite(gpu).all()
     assert err < 0.01, err
+
+
+@pytest.mark.parametrize("name,stride", [("cornell", 4), ("instanced", 1021), ("volumes", 601)])
+def test_converged_parity_4096spp_full_configs(name, stride, orc, asset_dir):
+    """The benchmark configurations at full size: the GPU renders the whole
+    frame at 4,096 spp (the Cornell box 256x256, the 4,096-instance frame
+    1920x1080, the volumes frame 1280x720 with its 2048x1024 sky); the
+    oracle renders every `stride`-th 16x16 tile (64, 8 and 6 tiles)."""
+    if name == "cornell":
+        scene = configs.cornell(spp=SPP)
+        w, h = 256, 256
+    elif name == "instanced":
+        scene = configs.instanced(asset_dir, spp=SPP)
+        w, h = 1920, 1080
+    else:
+        scene = configs.volumes(asset_dir, spp=SPP)
+        w, h = 1280, 720
+    sc = scene.build()
+    sess = sc.session()
+    sess.render_all(chunk=16)   # 16-spp passes: the session splits them into sample groups
+    gpu, _, _ = sess.download()
+    sess.close()
+    print("%s %dx%d @ %d spp rendered on the GPU" % (name, w, h, SPP), flush=True)
+    ref, _, _, _ = orc.render(sc.flatten(), w, h, 0, SPP, tile_stride=stride)
+    m = _tile_mask(w, h, stride)
+    err = sd.rel_rmse(gpu[m], ref[m])
+    print("%s %dx%d @ %d spp: relRMSE %.5f over %d px" % (name, w, h, SPP, err, int(m.sum())), flush=True)
+    assert np.isfinite(gpu).all()
+    assert m.sum() >= 6 * 256
+    assert err < 0.01, err

@@ -14,8 +14,6 @@
 
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
-
 #include <algorithm>
 #include <stdexcept>
 #include <vector>
@@ -404,6 +402,170 @@ __global__ __launch_bounds__(256) void k_instance_boxes(const float* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// exclusive prefix sum of uint32 (device-wide): 2,048-element blocks scanned
+// in LDS, their totals scanned the same way (recursively), then added back
+// ---------------------------------------------------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;  // consecutive elements per thread
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_block(const uint32_t* __restrict__ in,
+                                                            uint32_t* __restrict__ out,
+                                                            uint32_t* __restrict__ blockSums, int n) {
+  __shared__ uint32_t part[kScanThreads];
+  const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  // inclusive scan of the per-thread sums (Hillis-Steele)
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint32_t x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (blockSums && threadIdx.x == kScanThreads - 1) blockSums[blockIdx.x] = part[kScanThreads - 1];
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ blockOffsets, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += blockOffsets[i / kScanChunk];
+}
+
+// out[i] = in[0] + ... + in[i - 1]. Temporary buffers go to `pool`, which
+// must outlive the queued kernels.
+void exclusiveScan(const uint32_t* in, uint32_t* out, int n, std::vector<DevBuf>& pool, hipStream_t s) {
+  const int blocks = (n + kScanChunk - 1) / kScanChunk;
+  if (blocks <= 1) {
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kScanThreads), 0, s, in, out, (uint32_t*)nullptr, n);
+    HIP_OK(hipGetLastError());
+    return;
+  }
+  pool.emplace_back();
+  pool.back().alloc((size_t)blocks * 4);
+  uint32_t* sums = pool.back().as<uint32_t>();
+  pool.emplace_back();
+  pool.back().alloc((size_t)blocks * 4);
+  uint32_t* offsets = pool.back().as<uint32_t>();
+  hipLaunchKernelGGL(k_scan_block, dim3(blocks), dim3(kScanThreads), 0, s, in, out, sums, n);
+  HIP_OK(hipGetLastError());
+  exclusiveScan(sums, offsets, blocks, pool, s);
+  hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, out, offsets, n);
+  HIP_OK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of the 64-bit keys by their Morton code (bits 32-61), four
+// stable 8-bit passes. The keys arrive in primitive order with the index in
+// the low word, so the stable sort gives exactly the order of the full
+// 64-bit keys (code, then index). A pass: per-block digit counts
+// (digit-major, counts[d * blocks + b]), their exclusive scan, then a stable
+// scatter in which each block walks its 4,096 keys in tiles of 256 (one per
+// thread, in order), ranks each key among the equal digits of its wave with
+// eight ballots, and places it after the earlier waves and tiles.
+// ---------------------------------------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortTiles = 16;
+constexpr int kSortChunk = kSortThreads * kSortTiles;
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint64_t* __restrict__ keys, int n, int shift,
+                                                             uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[256];
+  hist[threadIdx.x] = 0u;
+  __syncthreads();
+  const int begin = blockIdx.x * kSortChunk;
+  const int end = min(n, begin + kSortChunk);
+  for (int i = begin + (int)threadIdx.x; i < end; i += kSortThreads)
+    atomicAdd(&hist[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+  __syncthreads();
+  counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ in,
+                                                               uint64_t* __restrict__ out, int n, int shift,
+                                                               const uint32_t* __restrict__ offsets) {
+  constexpr int kWaves = kSortThreads / 64;
+  __shared__ uint32_t running[256];         // next output position of each digit
+  __shared__ uint32_t waveHist[kWaves][256];  // this tile's digit counts per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  running[threadIdx.x] = offsets[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+  for (int w = 0; w < kWaves; w++) waveHist[w][threadIdx.x] = 0u;
+  __syncthreads();
+  const int begin = blockIdx.x * kSortChunk;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));  // the lanes before this one
+  for (int t = 0; t < kSortTiles && begin + t * kSortThreads < n; t++) {
+    const int i = begin + t * kSortThreads + (int)threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t key = valid ? in[i] : 0ull;
+    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+    // the lanes of this wave holding a valid key with the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & below);
+    if (valid && rank == 0u) waveHist[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+      for (int w = 0; w < wave; w++) pos += waveHist[w][d];
+      out[pos] = key;
+    }
+    __syncthreads();
+    uint32_t add = 0u;
+    for (int w = 0; w < kWaves; w++) {
+      add += waveHist[w][threadIdx.x];
+      waveHist[w][threadIdx.x] = 0u;
+    }
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Sorts n keys from `in` into `out` (both device buffers of n keys; `in` is
+// overwritten).
+void sortKeysByCode(uint64_t* in, uint64_t* out, int n, std::vector<DevBuf>& pool, hipStream_t s) {
+  const int blocks = (n + kSortChunk - 1) / kSortChunk;
+  pool.emplace_back();
+  pool.back().alloc((size_t)n * 8);
+  uint64_t* tmp = pool.back().as<uint64_t>();
+  pool.emplace_back();
+  pool.back().alloc((size_t)blocks * 256 * 4);
+  uint32_t* counts = pool.back().as<uint32_t>();
+  pool.emplace_back();
+  pool.back().alloc((size_t)blocks * 256 * 4);
+  uint32_t* offsets = pool.back().as<uint32_t>();
+  // passes at bits 32, 40, 48, 56: in -> tmp -> in -> tmp -> out
+  uint64_t* src[4] = {in, tmp, in, tmp};
+  uint64_t* dst[4] = {tmp, in, tmp, out};
+  for (int p = 0; p < 4; p++) {
+    const int shift = 32 + 8 * p;
+    hipLaunchKernelGGL(k_radix_count, dim3(blocks), dim3(kSortThreads), 0, s, src[p], n, shift, counts);
+    HIP_OK(hipGetLastError());
+    exclusiveScan(counts, offsets, blocks * 256, pool, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(kSortThreads), 0, s, src[p], dst[p], n, shift,
+                       offsets);
+    HIP_OK(hipGetLastError());
+  }
+}
+
 }  // namespace
 
 // Builds one tree into `out` (4-wide nodes at [0, numNodes)) from 2n float4
@@ -417,7 +579,8 @@ int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t 
               DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s) {
   if (n <= 0) throw std::runtime_error("lbvh: no primitives");
   if (maxLeaf < 1 || maxLeaf > 16) throw std::runtime_error("lbvh: maxLeaf must be 1..16");
-  DevBuf bounds, keysIn, work, nodes, leafParent, visits, isNode4, index4, fanout, bound;
+  DevBuf bounds, keysIn, nodes, leafParent, visits, isNode4, index4, fanout, bound;
+  std::vector<DevBuf> pool;  // sort and scan temporaries, alive until the final synchronisation
   bounds.alloc(6 * 4);
   keysIn.alloc((size_t)n * 8);
   sortedKeys.alloc((size_t)n * 8);
@@ -425,16 +588,14 @@ int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t 
   hipLaunchKernelGGL(k_centroid_bounds, dim3(grid(n)), dim3(kLbvhBlock), 0, s, boxes, n, bounds.as<uint32_t>());
   hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kLbvhBlock), 0, s, boxes, n, bounds.as<uint32_t>(),
                      keysIn.as<uint64_t>());
-  size_t tmp = 0;
-  HIP_OK(rocprim::radix_sort_keys(nullptr, tmp, keysIn.as<uint64_t>(), sortedKeys.as<uint64_t>(), (size_t)n, 0, 64, s));
-  work.alloc(tmp);
-  HIP_OK(rocprim::radix_sort_keys(work.p, tmp, keysIn.as<uint64_t>(), sortedKeys.as<uint64_t>(), (size_t)n, 0, 64, s));
+  sortKeysByCode(keysIn.as<uint64_t>(), sortedKeys.as<uint64_t>(), n, pool, s);
   const uint64_t* keys = sortedKeys.as<uint64_t>();
   if (n <= maxLeaf || n == 1) {
     out.alloc(sizeof(BvhNode4));
     hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, s, boxes, keys, n, refIsPrim ? 1 : 0, primBase,
                        out.as<BvhNode4>());
     HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
     stackBound = 0;
     return 1;
   }
@@ -451,12 +612,7 @@ int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t 
                      leafParent.as<uint32_t>(), visits.as<uint32_t>());
   hipLaunchKernelGGL(k_classify, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), ni, maxLeaf,
                      isNode4.as<uint32_t>());
-  tmp = 0;
-  HIP_OK(rocprim::exclusive_scan(nullptr, tmp, isNode4.as<uint32_t>(), index4.as<uint32_t>(), 0u, (size_t)ni,
-                                 rocprim::plus<uint32_t>(), s));
-  work.alloc(tmp);
-  HIP_OK(rocprim::exclusive_scan(work.p, tmp, isNode4.as<uint32_t>(), index4.as<uint32_t>(), 0u, (size_t)ni,
-                                 rocprim::plus<uint32_t>(), s));
+  exclusiveScan(isNode4.as<uint32_t>(), index4.as<uint32_t>(), ni, pool, s);
   uint32_t lastIdx = 0, lastFlag = 0;
   HIP_OK(hipMemcpyAsync(&lastIdx, index4.as<uint32_t>() + ni - 1, 4, hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(&lastFlag, isNode4.as<uint32_t>() + ni - 1, 4, hipMemcpyDeviceToHost, s));

@@ -203,6 +203,7 @@ def main():
              else "wavefront pass (k_wf_shade + k_wf_closest + k_wf_shadow)")
     beauty, _, _ = sess.download()
     finite = bool(np.isfinite(beauty).all())
+    traffic = measured_traffic(kname) if args.workload == "instanced" else None
 
     out = {
         "metric": METRIC,
@@ -236,8 +237,11 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": measured_traffic(kname) if args.workload == "instanced" else None,
-            "traffic_source": TRAFFIC_PROFILE_NAME if args.workload == "instanced" else None,
+            "traffic": traffic,
+            "traffic_source": TRAFFIC_PROFILE_NAME if traffic is not None else None,
+            # the profiled traffic over this run's kernel time, against 8 TB/s
+            "hbm_gbs": traffic / (kernel_ms_avg * 1e-3) / 1e9 if traffic is not None else None,
+            "hbm_frac": traffic / (kernel_ms_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic is not None else None,
             "peak_nonpacked_fp32": FP32_PEAK_NONPACKED_TFLOPS,
             "frac_nonpacked_fp32": achieved_tflops / FP32_PEAK_NONPACKED_TFLOPS,
             "kernel": kname,

@@ -98,7 +98,11 @@ def main():
                     help="oracle threads (0: the job's CPU share, OMP_NUM_THREADS capped by the affinity mask)")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests of the serving run reported under 'serving' (0: skip it)")
-    ap.add_argument("--serving-qps", type=float, default=4.0)
+    ap.add_argument("--serving-qps", type=float, default=4.0,
+                    help="arrival rate of the saturating serving run (throughput and served/serial ratio)")
+    ap.add_argument("--serving-qps-sustained", type=float, default=1.0,
+                    help="arrival rate of a second run of the same requests below the served capacity, "
+                         "whose p50 / p99 measure service rather than queue growth (0: skip it)")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
     ap.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
@@ -336,13 +340,25 @@ def serving_run(args):
         ser_ms += ms
     served = total / makespan
     serial = total / (ser_ms * 1e-3)
+    out = {"requests": len(reqs), "completed": len(lat), "qps": args.serving_qps, "paths": total,
+           "makespan_s": makespan, "served_paths_per_s": served, "serial_paths_per_s": serial,
+           "speedup_vs_serial": served / serial, "p50_ms": serving.percentile(lat, 50),
+           "p99_ms": serving.percentile(lat, 99),
+           "launch_shape": {k: tuned[k] for k in ("tile", "waves", "stack", "blocks_per_cu", "graph_passes")
+                            if k in tuned}}
+    if args.serving_qps_sustained > 0:
+        # the same requests (same seed: same sizes, scenes and seeds, arrival
+        # times scaled) at a rate the server keeps up with
+        reqs2 = serving.make_requests(args.serving_requests, args.serving_qps_sustained, seed=1)
+        print("serving: sustained run at %.2f qps" % args.serving_qps_sustained, flush=True)
+        done2, makespan2 = serving.run(reqs2, srv, sids)
+        lat2 = [d["t_done_ms"] - d["t_submit_ms"] for d in done2 if d["status"] == 2]
+        out["sustained"] = {"qps": args.serving_qps_sustained, "completed": len(lat2),
+                            "offered_paths_per_s": total / reqs2[-1]["arrival_s"],
+                            "makespan_s": makespan2, "p50_ms": serving.percentile(lat2, 50),
+                            "p99_ms": serving.percentile(lat2, 99)}
     srv.close()
-    return {"requests": len(reqs), "completed": len(lat), "qps": args.serving_qps, "paths": total,
-            "makespan_s": makespan, "served_paths_per_s": served, "serial_paths_per_s": serial,
-            "speedup_vs_serial": served / serial, "p50_ms": serving.percentile(lat, 50),
-            "p99_ms": serving.percentile(lat, 99),
-            "launch_shape": {k: tuned[k] for k in ("tile", "waves", "stack", "blocks_per_cu", "graph_passes")
-                             if k in tuned}}
+    return out
 
 
 if __name__ == "__main__":

@@ -298,6 +298,13 @@ SUNDOG_API int sundog_session_shadow(sundog_session*, const float* rays, int n,
  * marginal row and conditional column picked by the binary searches. */
 SUNDOG_API int sundog_session_env_bins(sundog_session*, const float* u, int n,
                                        int32_t* row_out, int32_t* col_out);
+/* The same bins by a chosen search, with the search kernel's time (ms, HIP
+ * events): mode 0 = the renderer's per-lane 16-ary table, mode 1 = a
+ * wave-cooperative search (the wave serves its lanes' lookups in turn, all
+ * 64 lanes probing keys of one CDF per step). Both return the bisection
+ * bins. No counterpart in the reference (its search is per thread). */
+SUNDOG_API int sundog_session_env_bins_mode(sundog_session*, const float* u, int n, int mode,
+                                            int32_t* row_out, int32_t* col_out, float* kernel_ms);
 /* Light sampling on device: n samples of light `light` (an index into the
  * scene's light list) from the points xs (n x 3); sample i draws from the
  * PCG32 stream init(seed0 + i, seq). out: 12 floats per sample — wi (3),

@@ -153,6 +153,7 @@ SIGNATURES = {
     "sundog_session_shadow": (c_i, [c_p, P_f, c_i, P_f]),
     "sundog_session_flame_field": (c_i, [c_p, c_i, P_f, c_i, P_f, P_f]),
     "sundog_session_env_bins": (c_i, [c_p, P_f, c_i, P_i32, P_i32]),
+    "sundog_session_env_bins_mode": (c_i, [c_p, P_f, c_i, c_i, P_i32, P_i32, P_f]),
     "sundog_session_light_samples": (c_i, [c_p, c_i, P_f, c_i, ctypes.c_uint64, ctypes.c_uint64, P_f]),
     "sundog_session_trace_noalloc": (c_i, [c_p, c_p, c_i, c_p, c_p, c_p]),
     "sundog_device_pcg32": (c_i, [c_u64, c_u64, c_i, P_u32]),

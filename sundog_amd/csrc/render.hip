@@ -1529,6 +1529,57 @@ __global__ void k_env_bins(LaunchParams P, const float* u, int n, int* rowOut, i
   colOut[i] = col;
 }
 
+// Wave-cooperative CDF search for comparison with the per-lane table the
+// renderer uses: the wave serves its lanes' lookups one lane after another;
+// for each, the 64 lanes load 64 evenly spaced keys strictly inside the
+// current interval (lo, hi) and one ballot counts those <= u, which narrows
+// the interval to the gap between two probes (a 65-ary step), keeping
+// cdf[lo] <= u < cdf[hi] until hi = lo + 1. The answer, lo, is the largest
+// i with cdf[i] <= u, bisection's bin. u is wave-uniform; every lane of the
+// wave must call it.
+DEV int cdfBinWave(const float* cdf, int nEntries, float u) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = nEntries - 1;
+  while (hi - lo > 1) {
+    const int m = hi - lo - 1;  // keys strictly inside (lo, hi)
+    const int cnt = m < 64 ? m : 64;
+    auto probe = [&](int j) { return m <= 64 ? lo + 1 + j : lo + 1 + (int)(((long long)j * m) / 64); };
+    const bool valid = lane < cnt;
+    const float key = valid ? cdf[probe(lane)] : 0.0f;
+    const int c = __popcll(__ballot(valid && key <= u));
+    const int nlo = c > 0 ? probe(c - 1) : lo;
+    const int nhi = c < cnt ? probe(c) : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_env_bins_wave(LaunchParams P, const float* u, int n, int* rowOut,
+                                                       int* colOut) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool mine = i < n;
+  const float u0 = mine ? u[2 * i] : 0.0f, u1 = mine ? u[2 * i + 1] : 0.0f;
+  const uint64_t active = __ballot(mine);
+  int row = 0, col = 0;
+  for (int l = 0; l < 64; l++) {
+    if (!((active >> l) & 1ull)) continue;  // wave-uniform
+    const float ur = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u0), l));
+    const float uc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u1), l));
+    const int r = cdfBinWave(P.margCdf, P.envH + 1, ur);
+    const int c = cdfBinWave(P.condCdf + (size_t)r * (P.envW + 1), P.envW + 1, uc);
+    if (lane == l) {
+      row = r;
+      col = c;
+    }
+  }
+  if (mine) {
+    rowOut[i] = row;
+    colOut[i] = col;
+  }
+}
+
 // Light samples from n points: sample i draws from init(seed0 + i, seq).
 // out: 12 floats per sample (wi, dist, Li, pdf, flags, draws, MIS-side pdf
 // at the sampled point or NaN for mesh / delta lights, pad).
@@ -1633,8 +1684,12 @@ hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* 
   return hipGetLastError();
 }
 
-hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s) {
-  hipLaunchKernelGGL(k_env_bins, dim3((n + 255) / 256), dim3(256), 0, s, P, u, n, row, col);
+hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s,
+                         int mode) {
+  if (mode == 1)
+    hipLaunchKernelGGL(k_env_bins_wave, dim3((n + 255) / 256), dim3(256), 0, s, P, u, n, row, col);
+  else
+    hipLaunchKernelGGL(k_env_bins, dim3((n + 255) / 256), dim3(256), 0, s, P, u, n, row, col);
   return hipGetLastError();
 }
 

@@ -28,7 +28,8 @@ hipError_t launchRender(const LaunchParams& P, bool countWork, int waves, int gr
 hipError_t launchTrace(const LaunchParams& P, const float* rays, int n, bool brute, int numInst, float* t,
                        int* inst, int* prim, int* counters, hipStream_t s);
 hipError_t launchShadow(const LaunchParams& P, const float* rays, int n, float* tr, hipStream_t s);
-hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s);
+hipError_t launchEnvBins(const LaunchParams& P, const float* u, int n, int* row, int* col, hipStream_t s,
+                         int mode = 0);
 hipError_t launchLightSample(const LaunchParams& P, int light, const float* xs, int n, uint64_t seed0, uint64_t seq,
                             float* out, hipStream_t s);
 hipError_t launchFlameField(const LaunchParams& P, int flame, const float* pts, int n, float* sigma, float* em,

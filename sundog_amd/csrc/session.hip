@@ -364,11 +364,13 @@ extern "C" int sundog_session_shadow(sundog_session* s, const float* rays, int n
   });
 }
 
-extern "C" int sundog_session_env_bins(sundog_session* s, const float* u, int n, int32_t* row_out,
-                                       int32_t* col_out) {
+extern "C" int sundog_session_env_bins_mode(sundog_session* s, const float* u, int n, int mode, int32_t* row_out,
+                                            int32_t* col_out, float* kernel_ms) {
   return guarded([&] {
     if (!s || !u || !row_out || !col_out) throw std::runtime_error("env bins: null argument");
     if (!s->P.hasEnv) throw std::runtime_error("env bins: scene has no environment map");
+    if (mode != 0 && mode != 1) throw std::runtime_error("env bins: mode must be 0 (per-lane table) or 1 (wave-cooperative)");
+    if (kernel_ms) *kernel_ms = 0.0f;
     if (n <= 0) return SUNDOG_OK;
     for (int i = 0; i < 2 * n; i++)
       if (!(u[i] >= 0.0f && u[i] < 1.0f)) throw std::runtime_error("env bins: u must lie in [0, 1)");
@@ -377,12 +379,22 @@ extern "C" int sundog_session_env_bins(sundog_session* s, const float* u, int n,
     dr.alloc((size_t)n * 4);
     dc.alloc((size_t)n * 4);
     HIP_OK(hipMemcpyAsync(du.p, u, (size_t)n * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_OK(launchEnvBins(s->P, du.as<float>(), n, dr.as<int>(), dc.as<int>(), s->stream));
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    HIP_OK(launchEnvBins(s->P, du.as<float>(), n, dr.as<int>(), dc.as<int>(), s->stream, mode));
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
     HIP_OK(hipMemcpyAsync(row_out, dr.p, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipMemcpyAsync(col_out, dc.p, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    if (kernel_ms) *kernel_ms = ms;
     return SUNDOG_OK;
   });
+}
+
+extern "C" int sundog_session_env_bins(sundog_session* s, const float* u, int n, int32_t* row_out,
+                                       int32_t* col_out) {
+  return sundog_session_env_bins_mode(s, u, n, 0, row_out, col_out, nullptr);
 }
 
 extern "C" int sundog_session_light_samples(sundog_session* s, int light, const float* xs, int n, uint64_t seed0,

@@ -668,6 +668,19 @@ class Session(object):
                    self.lib)
         return row, col
 
+    def env_bins_mode(self, u, mode):
+        """(row, col, kernel ms) of the environment bins by search `mode`:
+        0 the renderer's per-lane 16-ary table, 1 the wave-cooperative
+        search."""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        n = u.shape[0]
+        row = np.zeros(n, np.int32)
+        col = np.zeros(n, np.int32)
+        ms = ctypes.c_float(0.0)
+        _lib.check(self.lib.sundog_session_env_bins_mode(self.ptr, _fptr(u), n, int(mode), _iptr(row), _iptr(col),
+                                                         ctypes.byref(ms)), self.lib)
+        return row, col, ms.value
+
     def light_samples(self, light, xs, seed0=1, seq=7):
         """Device light samples of light `light` from (n, 3) points, sample i
         from the stream init(seed0 + i, seq): the oracle's light_sample keys."""

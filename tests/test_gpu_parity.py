@@ -156,6 +156,36 @@ def test_env_bins_bit_exact(orc, asset_dir):
     assert np.array_equal(c_g, c_c)
 
 
+def test_env_bins_search_modes(orc, asset_dir):
+    """The full 2048x1024 sky (its CDFs have 1,025 and 2,049 entries, and a
+    0.5 degree sun makes them nearly flat around one steep step): the
+    renderer's per-lane 16-ary table and the wave-cooperative search give
+    the oracle's bisection bins for 2^20 samples, including u on CDF entries
+    exactly and u in the flat stretches; kernel times of both are printed."""
+    sc = configs.volumes(asset_dir, width=32, height=18, spp=1).build()
+    sess = sc.session()
+    flat = sc.flatten()
+    rng = np.random.default_rng(9)
+    u = rng.random((1 << 20, 2)).astype(np.float32)
+    # samples on the CDFs' own entries (ties: the largest index wins)
+    marg, cond = orc.env_cdf(flat, 2048, 1024)
+    below1 = np.nextafter(np.float32(1.0), np.float32(0.0))
+    u[:1024, 0] = np.minimum(marg[:1024], below1)
+    u[1024:3072, 1] = np.minimum(cond[512, :2048], below1)
+    u[1024:3072, 0] = marg[512]      # rows near the horizon
+    u[-1] = [0.0, 0.0]
+    u[-2] = [np.nextafter(np.float32(1.0), np.float32(0.0))] * 2
+    r_c, c_c = orc.env_bins(flat, u)
+    times = {}
+    for mode in (0, 1):
+        r_g, c_g, ms = sess.env_bins_mode(u, mode)
+        assert np.array_equal(r_g, r_c), mode
+        assert np.array_equal(c_g, c_c), mode
+        times[mode] = ms
+    print("env bins for %d samples: per-lane table %.3f ms, wave-cooperative %.3f ms"
+          % (len(u), times[0], times[1]), flush=True)
+
+
 def _render_pair(scene, w, h, spp, orc, tile_stride=1):
     sess = scene.session()
     sess.render_all()

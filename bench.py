@@ -90,9 +90,12 @@ def main():
     ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
     ap.add_argument("--wf-waves", type=int, default=4, choices=[4, 5],
                     help="minimum waves per SIMD of the wavefront query kernels")
-    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh", "lbvh-sah-top"],
-                    help="host binned-SAH build (default), the GPU LBVH rebuild of both levels, or the GPU "
-                         "LBVH for the meshes with a host SAH top level over GPU instance boxes")
+    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh", "lbvh-sah-top", "ploc", "ploc-sah-top"],
+                    help="host binned-SAH build (default), the GPU LBVH rebuild of both levels, the GPU "
+                         "LBVH for the meshes with a host SAH top level over GPU instance boxes, the GPU "
+                         "PLOC rebuild of both levels, or GPU PLOC meshes under a host SAH top level")
+    ap.add_argument("--ploc-radius", type=int, default=16,
+                    help="neighbours searched on either side by the PLOC build (1..32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (0: the job's CPU share, OMP_NUM_THREADS capped by the affinity mask)")
@@ -152,6 +155,9 @@ def main():
         sess.tune("bvh", 1)
     elif args.bvh == "lbvh-sah-top":
         sess.tune("bvh", 2)
+    elif args.bvh in ("ploc", "ploc-sah-top"):
+        sess.tune("ploc_radius", args.ploc_radius)
+        sess.tune("bvh", 3 if args.bvh == "ploc" else 4)
     setup_s = time.time() - t0
     k = args.spp_per_step
 
@@ -262,7 +268,7 @@ def main():
         },
         "setup_s": setup_s,
         "bvh_build_ms": {"host_sah_total_build": warm_build_ms, "first_session_with_runtime_start": st0["build_ms"],
-                         "gpu_lbvh": st0["bvh_gpu_ms"]},
+                         "gpu_rebuild": st0["bvh_gpu_ms"]},
         "finite": finite,
     }
 

@@ -215,7 +215,8 @@ typedef struct {
    * 4-wide node visits (4 box tests each), triangle tests, quadric tests,
    * world->object instance transforms */
   uint64_t node_visits, tri_tests, quad_tests, inst_transforms;
-  int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH, 2: GPU LBVH meshes + host SAH top */
+  int32_t bvh_builder;      /* 0: host binned SAH (default), 1: GPU LBVH, 2: GPU LBVH meshes + host SAH
+                               top, 3: GPU PLOC, 4: GPU PLOC meshes + host SAH top */
   int32_t pad2;
   double bvh_gpu_ms;        /* wall time of the last GPU LBVH rebuild */
   int32_t waves;            /* megakernel occupancy variant in use (0: auto mode still timing) */
@@ -244,7 +245,10 @@ SUNDOG_API int sundog_session_set_count_rays(sundog_session*, int on);
  * films are bit-identical for every choice), and
  * "bvh" (1: rebuild both BVH levels on the GPU with the LBVH builder; 2:
  * the mesh levels with the GPU LBVH and the top level with the host SAH
- * builder over instance boxes computed on the GPU). */
+ * builder over instance boxes computed on the GPU; 3: both levels on the GPU
+ * with PLOC clustering; 4: the mesh levels with GPU PLOC and the top level
+ * with the host SAH builder), and "ploc_radius" (1..32, default 16:
+ * neighbours searched on either side by the PLOC build; set before "bvh"). */
 SUNDOG_API int sundog_session_set_tuning(sundog_session*, const char* key, int value);
 /* Times the megakernel's occupancy variants on passes of `count` samples
  * from sample 0 (one untimed warm-up pass, then 5, 4, 3 and 2 waves per

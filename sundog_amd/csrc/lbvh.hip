@@ -1,15 +1,12 @@
-// GPU build of the two-level BVH: a linear BVH (Karras, "Maximizing
-// Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG
-// 2012) over 30-bit Morton codes of primitive centroids, refit bottom-up,
-// then collapsed on the device into the same 4-wide 128-byte nodes the host
-// SAH build emits (bvh.h). It replaces the driver-side builds of the
-// reference (src/accel.cpp:8-121) where build time matters more than tree
-// quality; the host SAH build stays the default.
-//
-// Collapse: binary internal nodes at even depth become 4-wide nodes whose
-// slots are their grandchildren (or a child directly when that child is a
-// leaf); a subtree of at most maxLeaf primitives (a contiguous range of the
-// sorted order) becomes one leaf.
+// GPU builds of the two-level BVH over 30-bit Morton codes of primitive
+// centroids: a linear BVH (Karras, "Maximizing Parallelism in the
+// Construction of BVHs, Octrees, and k-d Trees", HPG 2012) refit bottom-up,
+// or PLOC clustering (Meister and Bittner, IEEE TVCG 2018) whose merges
+// follow the surface-area cost. Either binary tree is collapsed on the
+// device into the same 4-wide 128-byte nodes the host SAH build emits
+// (bvh.h), opening the largest child first; a subtree of at most maxLeaf
+// primitives (a contiguous range of the leaf order) becomes one leaf. They
+// replace the driver-side builds of the reference (src/accel.cpp:8-121).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -191,101 +188,6 @@ __global__ void k_refit(BinNode* nodes, const float4* __restrict__ boxes, const 
 }
 
 __device__ inline bool collapsed(const BinNode& nd, int maxLeaf) { return (int)(nd.hi - nd.lo + 1) <= maxLeaf; }
-
-// An internal node becomes a 4-wide node when it sits at even depth and is
-// not inside (or itself) a collapsed leaf range.
-__global__ void k_classify(const BinNode* __restrict__ nodes, int numInternal, int maxLeaf,
-                           uint32_t* __restrict__ isNode4) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numInternal) return;
-  int depth = 0;
-  bool inLeaf = collapsed(nodes[i], maxLeaf) && i != 0;
-  for (uint32_t p = nodes[i].parent; p != 0xFFFFFFFFu; p = nodes[p].parent) {
-    depth++;
-    if (collapsed(nodes[p], maxLeaf)) inLeaf = true;
-  }
-  isNode4[i] = (!inLeaf && (depth & 1) == 0) ? 1u : 0u;
-}
-
-// Slot of a 4-wide node for binary child c of a node at even depth.
-struct Slot {
-  uint32_t ref;
-  int32_t count;
-  float4 lo, hi;
-};
-
-__device__ inline Slot makeSlot(const BinNode* nodes, const float4* boxes, const uint64_t* keys,
-                                const uint32_t* node4Index, int maxLeaf, bool refIsPrim, uint32_t nodeBase,
-                                uint32_t primBase, uint32_t c) {
-  Slot s;
-  childBox(nodes, boxes, keys, c, s.lo, s.hi);
-  uint32_t first, cnt;
-  if (c & kLeafBit) {
-    first = c & ~kLeafBit;
-    cnt = 1;
-  } else if (collapsed(nodes[c], maxLeaf)) {
-    first = nodes[c].lo;
-    cnt = nodes[c].hi - nodes[c].lo + 1;
-  } else {
-    s.ref = nodeBase + node4Index[c];
-    s.count = 0;
-    return s;
-  }
-  uint32_t target = refIsPrim ? (uint32_t)keys[first] : primBase + first;
-  s.ref = kRefLeaf | ((cnt - 1) << 27) | target;
-  s.count = (int32_t)cnt;
-  return s;
-}
-
-__global__ void k_emit(const BinNode* __restrict__ nodes, int numInternal, const float4* __restrict__ boxes,
-                       const uint64_t* __restrict__ keys, const uint32_t* __restrict__ isNode4,
-                       const uint32_t* __restrict__ node4Index, int maxLeaf, int refIsPrim, uint32_t nodeBase,
-                       uint32_t primBase, BvhNode4* __restrict__ out, int32_t* __restrict__ fanout) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numInternal || !isNode4[i]) return;
-  uint32_t kids[4];
-  int k = 0;
-  const BinNode& nd = nodes[i];
-  uint32_t ch[2] = {nd.left, nd.right};
-  for (int a = 0; a < 2; a++) {
-    uint32_t c = ch[a];
-    if ((c & kLeafBit) || collapsed(nodes[c], maxLeaf)) {
-      kids[k++] = c;
-    } else {
-      kids[k++] = nodes[c].left;
-      kids[k++] = nodes[c].right;
-    }
-  }
-  BvhNode4 o;
-  for (int s = 0; s < 4; s++) {
-    if (s < k) {
-      Slot sl = makeSlot(nodes, boxes, keys, node4Index, maxLeaf, refIsPrim != 0, nodeBase, primBase, kids[s]);
-      o.lox[s] = sl.lo.x; o.loy[s] = sl.lo.y; o.loz[s] = sl.lo.z;
-      o.hix[s] = sl.hi.x; o.hiy[s] = sl.hi.y; o.hiz[s] = sl.hi.z;
-      o.child[s] = (int32_t)sl.ref;
-      o.count[s] = sl.count;
-    } else {
-      o.lox[s] = o.loy[s] = o.loz[s] = o.hix[s] = o.hiy[s] = o.hiz[s] = 3e38f;
-      o.child[s] = (int32_t)kRefEmpty;
-      o.count[s] = -1;
-    }
-  }
-  out[node4Index[i]] = o;
-  fanout[node4Index[i]] = k;
-}
-
-// Worst-case traversal stack: max over 4-wide root-to-node paths of the sum
-// of (fan-out - 1), the entries a depth-first traversal can hold.
-__global__ void k_stack_bound(const BinNode* __restrict__ nodes, int numInternal, const uint32_t* __restrict__ isNode4,
-                              const uint32_t* __restrict__ node4Index, const int32_t* __restrict__ fanout,
-                              int32_t* __restrict__ bound) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numInternal || !isNode4[i]) return;
-  int sum = 0;
-  for (uint32_t p = (uint32_t)i; p != 0xFFFFFFFFu; p = nodes[p].parent)
-    if (isNode4[p]) sum += fanout[node4Index[p]] - 1;
-  atomicMax(bound, sum);
-}
 
 // Triangle pool in leaf order: 3 float4 per triangle, v0.w = triangle id.
 __global__ void k_emit_tris(const uint64_t* __restrict__ keys, int n, const float* __restrict__ pos,
@@ -568,18 +470,437 @@ void sortKeysByCode(uint64_t* in, uint64_t* out, int n, std::vector<DevBuf>& poo
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// PLOC: parallel locally-ordered clustering (Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 2018). The clusters start as the Morton-sorted primitives. In
+// each round every cluster finds the cluster within `radius` positions of
+// it whose merged box has the smallest surface area (ties: the lower
+// position), mutual nearest neighbours merge into a new binary node, and the
+// surviving clusters are compacted in order. Some pair is always mutual (the
+// lowest-positioned cluster of a closest pair picks its partner, which picks
+// it back), so every round merges at least one pair. Internal nodes are
+// numbered from the root down (the last merge is node 0, the root the
+// collapse expects), and a final pass orders the leaves depth-first so that
+// every subtree covers a contiguous range of positions, the form the 4-wide
+// collapse reads.
+// ---------------------------------------------------------------------------
+
+constexpr int kPlocRadiusMax = 32;  // neighbours searched on either side (1..32)
+
+// Half the surface area of a box; NaN (a box with NaN bounds) counts as +inf
+// so that the neighbour search stays a total order.
+__device__ inline float halfArea(float4 lo, float4 hi) {
+  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  const float a = dx * dy + dy * dz + dz * dx;
+  return a < __builtin_inff() ? a : __builtin_inff();
+}
+__device__ inline float4 min4(float4 a, float4 b) {
+  return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+}
+__device__ inline float4 max4(float4 a, float4 b) {
+  return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.0f);
+}
+
+// Initial clusters: one leaf per sorted position, with its primitive's box.
+__global__ void k_ploc_init(const float4* __restrict__ boxes, const uint64_t* __restrict__ keys, int n,
+                            uint32_t* __restrict__ clusters, float4* __restrict__ cbox) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t prim = (uint32_t)keys[p];
+  clusters[p] = kLeafBit | (uint32_t)p;
+  cbox[2 * p] = boxes[2 * prim];
+  cbox[2 * p + 1] = boxes[2 * prim + 1];
+}
+
+// Nearest neighbour of every cluster within the radius; the block's clusters
+// and the radius on either side are staged in LDS.
+__global__ __launch_bounds__(kLbvhBlock) void k_ploc_nn(const float4* __restrict__ cbox, int m, int radius,
+                                                        uint32_t* __restrict__ nn) {
+  __shared__ float4 s[2 * (kLbvhBlock + 2 * kPlocRadiusMax)];
+  const int span = kLbvhBlock + 2 * radius;
+  const int base = (int)blockIdx.x * kLbvhBlock - radius;
+  for (int k = threadIdx.x; k < span; k += blockDim.x) {
+    const int j = base + k;
+    if (j >= 0 && j < m) {
+      s[2 * k] = cbox[2 * j];
+      s[2 * k + 1] = cbox[2 * j + 1];
+    }
+  }
+  __syncthreads();
+  const int i = (int)blockIdx.x * kLbvhBlock + threadIdx.x;
+  if (i >= m) return;
+  const float4 lo = s[2 * (i - base)], hi = s[2 * (i - base) + 1];
+  float best = __builtin_inff();
+  int bestJ = -1;
+  const int j0 = max(0, i - radius), j1 = min(m - 1, i + radius);
+  for (int j = j0; j <= j1; j++) {
+    if (j == i) continue;
+    const int k = j - base;
+    const float a = halfArea(min4(lo, s[2 * k]), max4(hi, s[2 * k + 1]));
+    if (bestJ < 0 || a < best) {
+      best = a;
+      bestJ = j;
+    }
+  }
+  nn[i] = (uint32_t)bestJ;
+}
+
+// merge[i]: cluster i is the lower of a mutual pair (it becomes the new
+// node); keep[i]: cluster i survives the round (it is not the upper one).
+__global__ void k_ploc_flags(const uint32_t* __restrict__ nn, int m, uint32_t* __restrict__ merge,
+                             uint32_t* __restrict__ keep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t j = nn[i];
+  const bool mutual = nn[j] == (uint32_t)i;
+  merge[i] = (mutual && (uint32_t)i < j) ? 1u : 0u;
+  keep[i] = (mutual && (uint32_t)i > j) ? 0u : 1u;
+}
+
+// The round's totals: merges and surviving clusters.
+__global__ void k_ploc_totals(const uint32_t* __restrict__ merge, const uint32_t* __restrict__ mergePos,
+                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ keepPos, int m,
+                              uint32_t* __restrict__ totals) {
+  if (threadIdx.x != 0) return;
+  totals[0] = mergePos[m - 1] + merge[m - 1];
+  totals[1] = keepPos[m - 1] + keep[m - 1];
+}
+
+// New binary nodes for the merged pairs and the compacted cluster list.
+// `created` merges happened in earlier rounds; node ids count down from
+// numInternal - 1, so the final merge is node 0.
+__global__ void k_ploc_merge(const uint32_t* __restrict__ clusters, const float4* __restrict__ cbox,
+                             const uint32_t* __restrict__ nn, const uint32_t* __restrict__ merge,
+                             const uint32_t* __restrict__ mergePos, const uint32_t* __restrict__ keep,
+                             const uint32_t* __restrict__ keepPos, int m, uint32_t created, int numInternal,
+                             BinNode* __restrict__ nodes, uint32_t* __restrict__ leafParent,
+                             uint32_t* __restrict__ count, uint32_t* __restrict__ outClusters,
+                             float4* __restrict__ outBox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !keep[i]) return;
+  const uint32_t dst = keepPos[i];
+  if (!merge[i]) {
+    outClusters[dst] = clusters[i];
+    outBox[2 * dst] = cbox[2 * i];
+    outBox[2 * dst + 1] = cbox[2 * i + 1];
+    return;
+  }
+  const uint32_t j = nn[i];
+  if (created + mergePos[i] >= (uint32_t)numInternal) return;  // cannot happen: n - 1 merges in all
+  const uint32_t id = (uint32_t)(numInternal - 1) - (created + mergePos[i]);
+  const uint32_t a = clusters[i], b = clusters[j];
+  const float4 lo = min4(cbox[2 * i], cbox[2 * j]), hi = max4(cbox[2 * i + 1], cbox[2 * j + 1]);
+  BinNode& nd = nodes[id];
+  nd.left = a;
+  nd.right = b;
+  nd.parent = 0xFFFFFFFFu;  // set when this node is merged in a later round
+  nd.boxLo = lo;
+  nd.boxHi = hi;
+  count[id] = ((a & kLeafBit) ? 1u : count[a]) + ((b & kLeafBit) ? 1u : count[b]);
+  if (a & kLeafBit) leafParent[a & ~kLeafBit] = id;
+  else nodes[a].parent = id;
+  if (b & kLeafBit) leafParent[b & ~kLeafBit] = id;
+  else nodes[b].parent = id;
+  outClusters[dst] = id;
+  outBox[2 * dst] = lo;
+  outBox[2 * dst + 1] = hi;
+}
+
+__device__ inline uint32_t subtreeCount(const uint32_t* count, uint32_t c) {
+  return (c & kLeafBit) ? 1u : count[c];
+}
+
+// Depth-first position of a subtree's first leaf: the sizes of the left
+// siblings along its path to the root (at most numInternal steps).
+__device__ inline uint32_t dfsOffset(const BinNode* nodes, const uint32_t* count, uint32_t child, uint32_t parent,
+                                     int numInternal) {
+  uint32_t off = 0;
+  for (int step = 0; parent != 0xFFFFFFFFu && step < numInternal; step++) {
+    const BinNode& nd = nodes[parent];
+    if (nd.right == child) off += subtreeCount(count, nd.left);
+    child = parent;
+    parent = nd.parent;
+  }
+  return off;
+}
+
+__global__ void k_ploc_leaf_order(const BinNode* __restrict__ nodes, const uint32_t* __restrict__ count,
+                                  const uint32_t* __restrict__ leafParent, int n, uint32_t* __restrict__ dfsPos) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  dfsPos[p] = dfsOffset(nodes, count, kLeafBit | (uint32_t)p, leafParent[p], n - 1);
+}
+
+// Each internal node's range of depth-first positions, and its leaf
+// references renamed to depth-first positions (only leaf references change,
+// and the walk compares internal ids only).
+__global__ void k_ploc_ranges(BinNode* __restrict__ nodes, const uint32_t* __restrict__ count,
+                              const uint32_t* __restrict__ dfsPos, int numInternal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numInternal) return;
+  const uint32_t lo = dfsOffset(nodes, count, (uint32_t)i, nodes[i].parent, numInternal);
+  nodes[i].lo = lo;
+  nodes[i].hi = lo + count[i] - 1;
+  const uint32_t l = nodes[i].left, r = nodes[i].right;
+  if (l & kLeafBit) nodes[i].left = kLeafBit | dfsPos[l & ~kLeafBit];
+  if (r & kLeafBit) nodes[i].right = kLeafBit | dfsPos[r & ~kLeafBit];
+}
+
+__global__ void k_ploc_keys(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ dfsPos, int n,
+                            uint64_t* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t q = dfsPos[p];
+  if (q < (uint32_t)n) out[q] = keys[p];  // a permutation of [0, n) for a well-formed tree
+}
+
+// The PLOC tree over n >= 2 Morton-sorted primitives: n - 1 internal nodes
+// with depth-first ranges, and the primitives' keys in depth-first order
+// (replacing `keys`).
+void plocTree(const float4* boxes, DevBuf& keys, int n, int radius, DevBuf& nodes, hipStream_t s) {
+  if (radius < 1 || radius > kPlocRadiusMax) throw std::runtime_error("ploc: radius must be 1..32");
+  const int ni = n - 1;
+  DevBuf clA, clB, bxA, bxB, nn, merge, keep, mergePos, keepPos, count, leafParent, totals, dfsPos, keysDfs;
+  clA.alloc((size_t)n * 4);
+  clB.alloc((size_t)n * 4);
+  bxA.alloc((size_t)n * 2 * sizeof(float4));
+  bxB.alloc((size_t)n * 2 * sizeof(float4));
+  nn.alloc((size_t)n * 4);
+  merge.alloc((size_t)n * 4);
+  keep.alloc((size_t)n * 4);
+  mergePos.alloc((size_t)n * 4);
+  keepPos.alloc((size_t)n * 4);
+  count.alloc((size_t)ni * 4);
+  leafParent.alloc((size_t)n * 4);
+  totals.alloc(8);
+  nodes.alloc((size_t)ni * sizeof(BinNode));
+  hipLaunchKernelGGL(k_ploc_init, dim3(grid(n)), dim3(kLbvhBlock), 0, s, boxes, keys.as<uint64_t>(), n,
+                     clA.as<uint32_t>(), bxA.as<float4>());
+  HIP_OK(hipGetLastError());
+  int m = n;
+  uint32_t created = 0;
+  DevBuf *cl = &clA, *clOut = &clB, *bx = &bxA, *bxOut = &bxB;
+  for (int round = 0; m > 1; round++) {
+    if (round > 2 * n + 64) throw std::runtime_error("ploc: clustering did not converge");
+    std::vector<DevBuf> pool;  // scan temporaries of this round, freed after its synchronisation
+    hipLaunchKernelGGL(k_ploc_nn, dim3(grid(m)), dim3(kLbvhBlock), 0, s, bx->as<float4>(), m, radius,
+                       nn.as<uint32_t>());
+    hipLaunchKernelGGL(k_ploc_flags, dim3(grid(m)), dim3(kLbvhBlock), 0, s, nn.as<uint32_t>(), m,
+                       merge.as<uint32_t>(), keep.as<uint32_t>());
+    HIP_OK(hipGetLastError());
+    exclusiveScan(merge.as<uint32_t>(), mergePos.as<uint32_t>(), m, pool, s);
+    exclusiveScan(keep.as<uint32_t>(), keepPos.as<uint32_t>(), m, pool, s);
+    hipLaunchKernelGGL(k_ploc_merge, dim3(grid(m)), dim3(kLbvhBlock), 0, s, cl->as<uint32_t>(), bx->as<float4>(),
+                       nn.as<uint32_t>(), merge.as<uint32_t>(), mergePos.as<uint32_t>(), keep.as<uint32_t>(),
+                       keepPos.as<uint32_t>(), m, created, ni, nodes.as<BinNode>(), leafParent.as<uint32_t>(),
+                       count.as<uint32_t>(), clOut->as<uint32_t>(), bxOut->as<float4>());
+    hipLaunchKernelGGL(k_ploc_totals, dim3(1), dim3(64), 0, s, merge.as<uint32_t>(), mergePos.as<uint32_t>(),
+                       keep.as<uint32_t>(), keepPos.as<uint32_t>(), m, totals.as<uint32_t>());
+    HIP_OK(hipGetLastError());
+    uint32_t t[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(t, totals.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (t[0] == 0 || created + t[0] > (uint32_t)ni || t[1] != (uint32_t)m - t[0])
+      throw std::runtime_error("ploc: inconsistent round (merges " + std::to_string(t[0]) + ", clusters " +
+                               std::to_string(t[1]) + " of " + std::to_string(m) + ")");
+    created += t[0];
+    m = (int)t[1];
+    std::swap(cl, clOut);
+    std::swap(bx, bxOut);
+  }
+  if (created != (uint32_t)ni) throw std::runtime_error("ploc: incomplete tree");
+  dfsPos.alloc((size_t)n * 4);
+  keysDfs.alloc((size_t)n * 8);
+  hipLaunchKernelGGL(k_ploc_leaf_order, dim3(grid(n)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(),
+                     count.as<uint32_t>(), leafParent.as<uint32_t>(), n, dfsPos.as<uint32_t>());
+  hipLaunchKernelGGL(k_ploc_ranges, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(),
+                     count.as<uint32_t>(), dfsPos.as<uint32_t>(), ni);
+  hipLaunchKernelGGL(k_ploc_keys, dim3(grid(n)), dim3(kLbvhBlock), 0, s, keys.as<uint64_t>(), dfsPos.as<uint32_t>(),
+                     n, keysDfs.as<uint64_t>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));
+  keys = std::move(keysDfs);
+}
+
+// ---------------------------------------------------------------------------
+// Collapse of a binary tree (LBVH or PLOC) into 4-wide nodes, top down, one
+// level of 4-wide nodes per round: a node's slots start as its binary
+// root's two children, and while there are fewer than four, the slot with
+// the largest surface area that is an inner binary node (not a collapsed
+// leaf range) is replaced by its two children (the greedy opening of wide
+// BVH builders). Inner slots become the next level's 4-wide nodes, numbered
+// level by level (the root is node 0).
+// ---------------------------------------------------------------------------
+
+struct Slots {
+  uint32_t c[4];
+  int k;
+};
+
+__device__ inline bool openable(const BinNode* nodes, uint32_t c, int maxLeaf) {
+  return !(c & kLeafBit) && !collapsed(nodes[c], maxLeaf);
+}
+
+__device__ inline Slots expandSlots(const BinNode* nodes, uint32_t r, int maxLeaf) {
+  Slots s;
+  s.k = 2;
+  s.c[0] = nodes[r].left;
+  s.c[1] = nodes[r].right;
+  s.c[2] = s.c[3] = 0;
+  while (s.k < 4) {
+    int best = -1;
+    float bestA = -1.0f;
+    for (int i = 0; i < s.k; i++) {
+      if (!openable(nodes, s.c[i], maxLeaf)) continue;
+      const float a = halfArea(nodes[s.c[i]].boxLo, nodes[s.c[i]].boxHi);
+      if (a > bestA) {
+        bestA = a;
+        best = i;
+      }
+    }
+    if (best < 0) break;
+    const uint32_t c = s.c[best];
+    s.c[best] = nodes[c].left;
+    s.c[s.k++] = nodes[c].right;
+  }
+  return s;
+}
+
+// Pass 1 of a level: how many of each 4-wide node's slots are inner nodes.
+__global__ void k_wide_count(const BinNode* __restrict__ nodes, const uint32_t* __restrict__ frontier, int nF,
+                             int maxLeaf, uint32_t* __restrict__ innerCount) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nF) return;
+  const Slots s = expandSlots(nodes, frontier[f], maxLeaf);
+  uint32_t c = 0;
+  for (int i = 0; i < s.k; i++) c += openable(nodes, s.c[i], maxLeaf) ? 1u : 0u;
+  innerCount[f] = c;
+}
+
+__global__ void k_wide_total(const uint32_t* __restrict__ innerCount, const uint32_t* __restrict__ innerPos, int nF,
+                             uint32_t* __restrict__ total) {
+  if (threadIdx.x == 0) total[0] = innerPos[nF - 1] + innerCount[nF - 1];
+}
+
+// Pass 2 of a level: write the level's 4-wide nodes (indices levelBase + f)
+// and the next level's frontier (indices levelBase + nF + position). acc[f]
+// is the sum of (fan-out - 1) over the node's 4-wide ancestors; the largest
+// acc + fan-out - 1 of any node bounds the traversal stack.
+__global__ void k_wide_emit(const BinNode* __restrict__ nodes, const float4* __restrict__ boxes,
+                            const uint64_t* __restrict__ keys, const uint32_t* __restrict__ frontier,
+                            const uint32_t* __restrict__ acc, const uint32_t* __restrict__ innerPos, int nF,
+                            uint32_t levelBase, int maxLeaf, int refIsPrim, uint32_t nodeBase, uint32_t primBase,
+                            BvhNode4* __restrict__ out, uint32_t* __restrict__ nextFrontier,
+                            uint32_t* __restrict__ nextAcc, int32_t* __restrict__ bound) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nF) return;
+  const Slots s = expandSlots(nodes, frontier[f], maxLeaf);
+  const uint32_t myAcc = acc[f] + (uint32_t)(s.k - 1);
+  uint32_t next = innerPos[f];
+  BvhNode4 o;
+  for (int i = 0; i < 4; i++) {
+    if (i >= s.k) {
+      o.lox[i] = o.loy[i] = o.loz[i] = o.hix[i] = o.hiy[i] = o.hiz[i] = 3e38f;
+      o.child[i] = (int32_t)kRefEmpty;
+      o.count[i] = -1;
+      continue;
+    }
+    const uint32_t c = s.c[i];
+    float4 lo, hi;
+    childBox(nodes, boxes, keys, c, lo, hi);
+    o.lox[i] = lo.x; o.loy[i] = lo.y; o.loz[i] = lo.z;
+    o.hix[i] = hi.x; o.hiy[i] = hi.y; o.hiz[i] = hi.z;
+    if (openable(nodes, c, maxLeaf)) {
+      o.child[i] = (int32_t)(nodeBase + levelBase + (uint32_t)nF + next);
+      o.count[i] = 0;
+      nextFrontier[next] = c;
+      nextAcc[next] = myAcc;
+      next++;
+    } else {
+      uint32_t first, cnt;
+      if (c & kLeafBit) {
+        first = c & ~kLeafBit;
+        cnt = 1;
+      } else {
+        first = nodes[c].lo;
+        cnt = nodes[c].hi - nodes[c].lo + 1;
+      }
+      const uint32_t target = refIsPrim ? (uint32_t)keys[first] : primBase + first;
+      o.child[i] = (int32_t)(kRefLeaf | ((cnt - 1) << 27) | target);
+      o.count[i] = (int32_t)cnt;
+    }
+  }
+  out[levelBase + f] = o;
+  atomicMax(bound, (int32_t)myAcc);
+}
+
+// 4-wide nodes of the binary tree over n >= 2 primitives (root: binary node
+// 0, not a collapsed range) into `out`; returns their number.
+int collapseWide(const BinNode* nodes, const float4* boxes, const uint64_t* keys, int n, int maxLeaf, bool refIsPrim,
+                 uint32_t nodeBase, uint32_t primBase, DevBuf& out, int& stackBound, hipStream_t s) {
+  const int ni = n - 1;
+  DevBuf fA, fB, aA, aB, innerCount, innerPos, total, bound;
+  fA.alloc((size_t)ni * 4);
+  fB.alloc((size_t)ni * 4);
+  aA.alloc((size_t)ni * 4);
+  aB.alloc((size_t)ni * 4);
+  innerCount.alloc((size_t)ni * 4);
+  innerPos.alloc((size_t)ni * 4);
+  total.alloc(4);
+  bound.alloc(4);
+  out.alloc((size_t)ni * sizeof(BvhNode4));  // every 4-wide node opens at least one binary node
+  HIP_OK(hipMemsetAsync(fA.p, 0, 4, s));     // the root
+  HIP_OK(hipMemsetAsync(aA.p, 0, 4, s));
+  HIP_OK(hipMemsetAsync(bound.p, 0, 4, s));
+  DevBuf *fr = &fA, *frNext = &fB, *ac = &aA, *acNext = &aB;
+  uint32_t levelBase = 0;
+  int nF = 1;
+  while (nF > 0) {
+    if (levelBase + (uint32_t)nF > (uint32_t)ni)
+      throw std::runtime_error("bvh collapse: more 4-wide nodes than binary nodes");
+    std::vector<DevBuf> pool;
+    hipLaunchKernelGGL(k_wide_count, dim3(grid(nF)), dim3(kLbvhBlock), 0, s, nodes, fr->as<uint32_t>(), nF, maxLeaf,
+                       innerCount.as<uint32_t>());
+    HIP_OK(hipGetLastError());
+    exclusiveScan(innerCount.as<uint32_t>(), innerPos.as<uint32_t>(), nF, pool, s);
+    hipLaunchKernelGGL(k_wide_total, dim3(1), dim3(64), 0, s, innerCount.as<uint32_t>(), innerPos.as<uint32_t>(), nF,
+                       total.as<uint32_t>());
+    hipLaunchKernelGGL(k_wide_emit, dim3(grid(nF)), dim3(kLbvhBlock), 0, s, nodes, boxes, keys, fr->as<uint32_t>(),
+                       ac->as<uint32_t>(), innerPos.as<uint32_t>(), nF, levelBase, maxLeaf, refIsPrim ? 1 : 0,
+                       nodeBase, primBase, out.as<BvhNode4>(), frNext->as<uint32_t>(), acNext->as<uint32_t>(),
+                       bound.as<int32_t>());
+    HIP_OK(hipGetLastError());
+    uint32_t nNext = 0;
+    HIP_OK(hipMemcpyAsync(&nNext, total.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    levelBase += (uint32_t)nF;
+    nF = (int)nNext;
+    std::swap(fr, frNext);
+    std::swap(ac, acNext);
+  }
+  int32_t sb = 0;
+  HIP_OK(hipMemcpyAsync(&sb, bound.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  stackBound = sb;
+  return (int)levelBase;
+}
+
 // Builds one tree into `out` (4-wide nodes at [0, numNodes)) from 2n float4
 // primitive boxes (lo, hi per primitive). Leaf references are final: with
 // refIsPrim they name the input primitive (the top level: one instance per
 // leaf), otherwise primBase + a position in the sorted order (a BLAS whose
 // triangles are emitted in that order from `sortedKeys`). Inner references
 // are offset by nodeBase. Returns the node count; `stackBound` receives the
-// worst-case traversal stack of the tree.
+// worst-case traversal stack of the tree. method 0 builds Karras' radix
+// tree over the Morton order, 1 the PLOC tree (above) with the given search
+// radius.
 int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t nodeBase, uint32_t primBase,
-              DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s) {
+              DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s, int method, int plocRadius) {
+  if (method != 0 && method != 1) throw std::runtime_error("lbvh: method must be 0 (LBVH) or 1 (PLOC)");
   if (n <= 0) throw std::runtime_error("lbvh: no primitives");
   if (maxLeaf < 1 || maxLeaf > 16) throw std::runtime_error("lbvh: maxLeaf must be 1..16");
-  DevBuf bounds, keysIn, nodes, leafParent, visits, isNode4, index4, fanout, bound;
+  DevBuf bounds, keysIn, nodes, leafParent, visits;
   std::vector<DevBuf> pool;  // sort and scan temporaries, alive until the final synchronisation
   bounds.alloc(6 * 4);
   keysIn.alloc((size_t)n * 8);
@@ -600,38 +921,23 @@ int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t 
     return 1;
   }
   const int ni = n - 1;
-  nodes.alloc((size_t)ni * sizeof(BinNode));
-  leafParent.alloc((size_t)n * 4);
-  visits.alloc((size_t)ni * 4);
-  isNode4.alloc((size_t)ni * 4);
-  index4.alloc((size_t)ni * 4);
-  HIP_OK(hipMemsetAsync(visits.p, 0, (size_t)ni * 4, s));
-  hipLaunchKernelGGL(k_karras, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, keys, n, nodes.as<BinNode>(),
-                     leafParent.as<uint32_t>());
-  hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), boxes, keys, n,
-                     leafParent.as<uint32_t>(), visits.as<uint32_t>());
-  hipLaunchKernelGGL(k_classify, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), ni, maxLeaf,
-                     isNode4.as<uint32_t>());
-  exclusiveScan(isNode4.as<uint32_t>(), index4.as<uint32_t>(), ni, pool, s);
-  uint32_t lastIdx = 0, lastFlag = 0;
-  HIP_OK(hipMemcpyAsync(&lastIdx, index4.as<uint32_t>() + ni - 1, 4, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipMemcpyAsync(&lastFlag, isNode4.as<uint32_t>() + ni - 1, 4, hipMemcpyDeviceToHost, s));
+  if (method == 1) {
+    plocTree(boxes, sortedKeys, n, plocRadius, nodes, s);
+    keys = sortedKeys.as<uint64_t>();
+  } else {
+    nodes.alloc((size_t)ni * sizeof(BinNode));
+    leafParent.alloc((size_t)n * 4);
+    visits.alloc((size_t)ni * 4);
+    HIP_OK(hipMemsetAsync(visits.p, 0, (size_t)ni * 4, s));
+    hipLaunchKernelGGL(k_karras, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, keys, n, nodes.as<BinNode>(),
+                       leafParent.as<uint32_t>());
+    hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), boxes, keys, n,
+                       leafParent.as<uint32_t>(), visits.as<uint32_t>());
+    HIP_OK(hipGetLastError());
+  }
+  const int num4 = collapseWide(nodes.as<BinNode>(), boxes, keys, n, maxLeaf, refIsPrim, nodeBase, primBase, out,
+                                stackBound, s);
   HIP_OK(hipStreamSynchronize(s));
-  const int num4 = (int)(lastIdx + lastFlag);
-  out.alloc((size_t)num4 * sizeof(BvhNode4));
-  fanout.alloc((size_t)num4 * 4);
-  bound.alloc(4);
-  HIP_OK(hipMemsetAsync(bound.p, 0, 4, s));
-  hipLaunchKernelGGL(k_emit, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), ni, boxes, keys,
-                     isNode4.as<uint32_t>(), index4.as<uint32_t>(), maxLeaf, refIsPrim ? 1 : 0, nodeBase, primBase,
-                     out.as<BvhNode4>(), fanout.as<int32_t>());
-  hipLaunchKernelGGL(k_stack_bound, dim3(grid(ni)), dim3(kLbvhBlock), 0, s, nodes.as<BinNode>(), ni,
-                     isNode4.as<uint32_t>(), index4.as<uint32_t>(), fanout.as<int32_t>(), bound.as<int32_t>());
-  HIP_OK(hipGetLastError());
-  int32_t sb = 0;
-  HIP_OK(hipMemcpyAsync(&sb, bound.p, 4, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  stackBound = sb;
   return num4;
 }
 

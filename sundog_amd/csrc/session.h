@@ -53,7 +53,8 @@ int stackLdsDefault();
 int stackLdsMax();
 struct DevBuf;
 int lbvhBuild(const float4* boxes, int n, int maxLeaf, bool refIsPrim, uint32_t nodeBase, uint32_t primBase,
-              DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s);
+              DevBuf& out, DevBuf& sortedKeys, int& stackBound, hipStream_t s, int method = 0,
+              int plocRadius = 16);
 // One world-box job of the GPU rebuild: the object-to-world transform and the
 // object's vertices in the position pool (numVerts 0: keep the host's box).
 struct InstanceBoxJob {
@@ -425,7 +426,10 @@ struct sundog_session {
   GroupPicker groupPick;
   uint64_t paths = 0;
   int tlasNodes = 0, blasNodes = 0;
-  int bvhBuilder = 0;        // 0: host binned SAH, 1: GPU LBVH
+  // 0: host binned SAH, 1: GPU LBVH, 2: GPU LBVH meshes + host SAH top,
+  // 3: GPU PLOC, 4: GPU PLOC meshes + host SAH top
+  int bvhBuilder = 0;
+  int plocRadius = 16;       // neighbours searched on either side by the PLOC build
   double bvhGpuMs = 0.0;     // wall time of the last GPU build
   std::vector<DevInstance> instHost;
   std::vector<int> meshPosOff, meshIdxOff;
@@ -868,8 +872,10 @@ struct sundog_session {
   // on the tree, so films stay bit-identical wherever shadow rays are opaque.
   // GPU rebuild of the BVH. sahTop: the top level is built with the host's
   // binned SAH over the instance boxes the GPU computed (4,096 boxes take
-  // about a millisecond), the mesh levels with the GPU LBVH.
-  void buildBvhGpu(bool sahTop = false) {
+  // about a millisecond), the mesh levels on the GPU. method: 0 Karras'
+  // LBVH, 1 PLOC clustering with plocRadius (both collapsed to the same
+  // 4-wide nodes).
+  void buildBvhGpu(bool sahTop = false, int method = 0) {
     HIP_OK(hipStreamSynchronize(stream));
     auto t0 = std::chrono::steady_clock::now();
     const Scene& S = prep.scene;
@@ -916,7 +922,8 @@ struct sundog_session {
       nT = (int)tn.size();
       tlasStack = top.stackBound;
     } else {
-      nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack, stream);
+      nT = lbvhBuild(instBoxes.as<float4>(), (int)S.objects.size(), 1, true, 0, 0, tlas, tlasKeys, tlasStack, stream,
+                     method, plocRadius);
     }
     struct Blas {
       DevBuf nodes, keys;
@@ -944,7 +951,8 @@ struct sundog_session {
       b.tris = n;
       b.nodeBase = nodeBase;
       b.primBase = primBase;
-      b.num = lbvhBuild(boxes.as<float4>(), n, 4, false, nodeBase, primBase, b.nodes, b.keys, b.stack, stream);
+      b.num = lbvhBuild(boxes.as<float4>(), n, 4, false, nodeBase, primBase, b.nodes, b.keys, b.stack, stream, method,
+                        plocRadius);
       root[mi] = nodeBase;
       nodeBase += (uint32_t)b.num;
       primBase += (uint32_t)n;
@@ -975,7 +983,7 @@ struct sundog_session {
     blasNodes = (int)nodeBase - nT;
     stackNeed = tlasStack + 1 + blasStack;
     spillRows = spillRowsFor(P.stackLds);
-    bvhBuilder = sahTop ? 2 : 1;
+    bvhBuilder = method == 1 ? (sahTop ? 4 : 3) : (sahTop ? 2 : 1);
     bvhGpuMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 

@@ -127,10 +127,14 @@ extern "C" int sundog_session_set_tuning(sundog_session* s, const char* key, int
       HIP_OK(hipStreamSynchronize(s->stream));
       s->P.stackLds = value;
     } else if (k == "bvh") {
-      if (value != 1 && value != 2)
-        throw std::runtime_error("tuning: bvh must be 1 (rebuild both levels with the GPU LBVH) or 2 (GPU LBVH "
-                                 "mesh levels, host SAH top level over GPU instance boxes)");
-      s->buildBvhGpu(value == 2);
+      if (value < 1 || value > 4)
+        throw std::runtime_error("tuning: bvh must be 1 (rebuild both levels with the GPU LBVH), 2 (GPU LBVH "
+                                 "mesh levels, host SAH top level over GPU instance boxes), 3 (both levels "
+                                 "with GPU PLOC clustering) or 4 (GPU PLOC mesh levels, host SAH top level)");
+      s->buildBvhGpu(value == 2 || value == 4, value >= 3 ? 1 : 0);
+    } else if (k == "ploc_radius") {
+      if (value < 1 || value > 32) throw std::runtime_error("tuning: ploc_radius must be 1..32");
+      s->plocRadius = value;
     } else if (k == "wf_waves") {
       if (value < 4 || value > 5) throw std::runtime_error("tuning: wf_waves must be 4 or 5");
       s->wfWaves = value;

@@ -1,5 +1,6 @@
-"""GPU LBVH build (both levels, or the mesh levels under a host-SAH top
-level over GPU instance boxes): closest hits through the rebuilt two-level
+"""GPU BVH builds (the LBVH over both levels, the LBVH mesh levels under a
+host-SAH top level over GPU instance boxes, and PLOC clustering over both
+levels or under a host-SAH top level): closest hits through the rebuilt two-level
 BVH equal brute force bit-exactly (t bits, instance, primitive), and films
 equal the host-SAH BVH's films bit-exactly where shadow rays are opaque
 (closest hits and opaque occlusion do not depend on the tree)."""
@@ -32,7 +33,7 @@ def _scene(name, asset_dir, **kw):
     return configs.instanced(asset_dir, instances=2048, subdiv=4, **kw)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini", "instanced_2k"])
 def test_lbvh_closest_hit_equals_brute_force(name, mode, asset_dir):
     sc = _scene(name, asset_dir, width=32, height=32, spp=1).build()
@@ -51,7 +52,7 @@ def test_lbvh_closest_hit_equals_brute_force(name, mode, asset_dir):
     assert np.array_equal(t.view(np.uint32), tb.view(np.uint32))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["cornell", "instanced_mini"])
 def test_lbvh_film_equals_sah_film(name, mode, asset_dir):
     sc = _scene(name, asset_dir, width=48, height=32, spp=8).build()
@@ -65,7 +66,7 @@ def test_lbvh_film_equals_sah_film(name, mode, asset_dir):
     assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_lbvh_full_instanced_scene_equals_sah(mode, asset_dir):
     """The full 4,096-instance scene (20,480-triangle mesh): the GPU sort and
     scan run over several blocks (the mesh's keys span five sort blocks and

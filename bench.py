@@ -160,6 +160,9 @@ def main():
         sess.tune("bvh", 3 if args.bvh == "ploc" else 4)
     setup_s = time.time() - t0
     k = args.spp_per_step
+    from sundog_amd import bvhinfo
+    summ = bvhinfo.summary(sess.bvh())
+    bvh_sah = {"top": summ["tlas_sah"], "meshes": list(summ["blas_sah"].values())}
 
     st0 = sess.stats()
     # a second session over the same scene times the build with the HIP
@@ -269,6 +272,9 @@ def main():
         "setup_s": setup_s,
         "bvh_build_ms": {"host_sah_total_build": warm_build_ms, "first_session_with_runtime_start": st0["build_ms"],
                          "gpu_rebuild": st0["bvh_gpu_ms"]},
+        # SAH cost of the tree the timed passes traversed (node visits + primitive
+        # tests for a ray that hits the root box; sundog_amd/bvhinfo.py)
+        "bvh_sah": bvh_sah,
         "finite": finite,
     }
 

@@ -267,6 +267,27 @@ SUNDOG_API int sundog_session_sync(sundog_session*);
 SUNDOG_API int sundog_session_download(sundog_session*, float* beauty,
                                        float* albedo, float* normal);
 SUNDOG_API int sundog_session_get_stats(sundog_session*, sundog_session_stats*);
+/* The resident two-level BVH as the traversal kernels read it (for tree
+ * quality and build-determinism checks). */
+typedef struct {
+  int32_t num_nodes;      /* 128-byte 4-wide nodes: the top level at [0, tlas_nodes), the meshes' after it */
+  int32_t tlas_nodes;
+  int32_t num_tris;       /* leaf-ordered triangles: 3 float4 each, v0.w = the triangle's index bits */
+  int32_t num_instances;
+  uint32_t tlas_root;     /* packed reference of the top-level root */
+  int32_t pad;
+} sundog_bvh_layout;
+/* Fills `out`; copies the node array (num_nodes x 128 bytes: six float4 rows
+ * of child bounds lo.x, hi.x, lo.y, hi.y, lo.z, hi.z, then the int4 of packed
+ * child references and the int4 of child primitive counts) into `nodes` and
+ * the triangle pool (num_tris x 12 floats) into `tris` when non-NULL and
+ * large enough (an error otherwise), and each instance's mesh root node (-1
+ * for quadrics) into the first blas_roots_cap entries of blas_roots when
+ * non-NULL. Packed references: bit 31 leaf, bits 27-30 count - 1, bits 0-26
+ * the first primitive (a triangle position, or an instance in the top
+ * level); inner references are node indices; 0xFFFFFFFF is an empty slot. */
+SUNDOG_API int sundog_session_bvh(sundog_session*, sundog_bvh_layout* out, void* nodes, size_t nodes_bytes,
+                                  float* tris, size_t tris_bytes, int32_t* blas_roots, int blas_roots_cap);
 SUNDOG_API int sundog_session_size(sundog_session*, int* width, int* height, int* spp);
 /* The HIP stream (hipStream_t) the session launches on, for event timing. */
 SUNDOG_API void* sundog_session_stream(sundog_session*);

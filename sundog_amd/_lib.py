@@ -60,6 +60,11 @@ class SessionStats(ctypes.Structure):
                 ("waves", c_i32), ("stack_lds", c_i32), ("sample_groups", c_i32), ("pad3", c_i32)]
 
 
+class BvhLayout(ctypes.Structure):
+    _fields_ = [("num_nodes", c_i32), ("tlas_nodes", c_i32), ("num_tris", c_i32), ("num_instances", c_i32),
+                ("tlas_root", ctypes.c_uint32), ("pad", c_i32)]
+
+
 class RequestInfo(ctypes.Structure):
     _fields_ = [("id", c_i64), ("status", c_i32), ("scene", c_i32), ("width", c_i32), ("height", c_i32),
                 ("spp", c_i32), ("pad", c_i32), ("t_submit_ms", c_d), ("t_admit_ms", c_d),
@@ -147,6 +152,8 @@ SIGNATURES = {
     "sundog_session_sync": (c_i, [c_p]),
     "sundog_session_download": (c_i, [c_p, P_f, P_f, P_f]),
     "sundog_session_get_stats": (c_i, [c_p, ctypes.POINTER(SessionStats)]),
+    "sundog_session_bvh": (c_i, [c_p, ctypes.POINTER(BvhLayout), c_p, ctypes.c_size_t, P_f, ctypes.c_size_t,
+                                 P_i32, c_i]),
     "sundog_session_size": (c_i, [c_p, P_i32, P_i32, P_i32]),
     "sundog_session_stream": (c_p, [c_p]),
     "sundog_session_trace": (c_i, [c_p, P_f, c_i, c_i, P_f, P_i32, P_i32]),

@@ -218,6 +218,35 @@ extern "C" int sundog_session_download(sundog_session* s, float* beauty, float* 
   });
 }
 
+extern "C" int sundog_session_bvh(sundog_session* s, sundog_bvh_layout* out, void* nodes, size_t nodesBytes,
+                                  float* tris, size_t trisBytes, int32_t* blasRoots, int blasRootsCap) {
+  return guarded([&] {
+    if (!s || !out) throw std::runtime_error("session: null argument");
+    HIP_OK(hipStreamSynchronize(s->stream));
+    *out = sundog_bvh_layout{};
+    out->num_nodes = s->tlasNodes + s->blasNodes;
+    out->tlas_nodes = s->tlasNodes;
+    out->num_tris = (int32_t)(s->tris.bytes / (3 * sizeof(float4)));
+    out->num_instances = s->numInstances;
+    out->tlas_root = s->P.tlasRoot;
+    if (nodes) {
+      const size_t need = (size_t)out->num_nodes * sizeof(BvhNode4);
+      if (nodesBytes < need) throw std::runtime_error("bvh: node buffer needs " + std::to_string(need) + " bytes");
+      if (need) HIP_OK(hipMemcpy(nodes, s->nodes.p, need, hipMemcpyDeviceToHost));
+    }
+    if (tris) {
+      const size_t need = (size_t)out->num_tris * 3 * sizeof(float4);
+      if (trisBytes < need) throw std::runtime_error("bvh: triangle buffer needs " + std::to_string(need) + " bytes");
+      if (need) HIP_OK(hipMemcpy(tris, s->tris.p, need, hipMemcpyDeviceToHost));
+    }
+    if (blasRoots) {
+      for (int i = 0; i < blasRootsCap && i < (int)s->instHost.size(); i++)
+        blasRoots[i] = s->instHost[i].geomKind == GK_MESH ? (int32_t)s->instHost[i].blasRoot : -1;
+    }
+    return SUNDOG_OK;
+  });
+}
+
 extern "C" int sundog_session_get_stats(sundog_session* s, sundog_session_stats* out) {
   return guarded([&] {
     if (!s || !out) throw std::runtime_error("session: null argument");

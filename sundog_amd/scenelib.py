@@ -628,6 +628,22 @@ class Session(object):
         _lib.check(self.lib.sundog_session_get_stats(self.ptr, ctypes.byref(st)), self.lib)
         return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
 
+    def bvh(self):
+        """The resident BVH: {"nodes": raw node bytes, "tris": (num_tris, 12)
+        float32, "blas_roots": per instance (-1 for quadrics), "num_nodes",
+        "tlas_nodes", "num_tris", "num_instances", "tlas_root"}."""
+        lay = _lib.BvhLayout()
+        _lib.check(self.lib.sundog_session_bvh(self.ptr, ctypes.byref(lay), None, 0, None, 0, None, 0), self.lib)
+        nodes = np.zeros(lay.num_nodes * 32, np.float32)
+        tris = np.zeros((lay.num_tris, 12), np.float32)
+        roots = np.zeros(max(lay.num_instances, 1), np.int32)
+        _lib.check(self.lib.sundog_session_bvh(self.ptr, ctypes.byref(lay), nodes.ctypes.data_as(ctypes.c_void_p),
+                                               nodes.nbytes, _fptr(tris), tris.nbytes, _iptr(roots),
+                                               lay.num_instances), self.lib)
+        out = {k: getattr(lay, k) for k, _ in lay._fields_ if k != "pad"}
+        out.update(nodes=nodes.tobytes(), tris=tris, blas_roots=roots[:lay.num_instances])
+        return out
+
     def stream(self):
         return self.lib.sundog_session_stream(self.ptr)
 

@@ -8,7 +8,7 @@ equal the host-SAH BVH's films bit-exactly where shadow rays are opaque
 import numpy as np
 import pytest
 
-from sundog_amd import configs
+from sundog_amd import bvhinfo, configs
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,28 @@ def test_lbvh_full_instanced_scene_equals_sah(mode, asset_dir):
     assert int((ia >= 0).sum()) > len(rays) // 2
     assert np.array_equal(ia, ib) and np.array_equal(pa, pb)
     assert np.array_equal(ta.view(np.uint32), tb.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3, 4])
+def test_builds_are_deterministic_and_costed(mode, asset_dir):
+    """Two builds of the same scene (the host SAH build, or a GPU rebuild)
+    give the same node and triangle bytes, and the tree's SAH cost is
+    reported from those bytes."""
+    sc = configs.instanced(asset_dir, instances=2048, subdiv=4, width=32, height=32, spp=1).build()
+    recs = []
+    for _ in range(2):
+        s = sc.session()
+        if mode:
+            s.tune("bvh", mode)
+        assert s.stats()["bvh_builder"] == mode
+        recs.append(s.bvh())
+    a, b = recs
+    assert a["num_nodes"] == b["num_nodes"] and a["nodes"] == b["nodes"]
+    assert np.array_equal(a["tris"].view(np.uint32), b["tris"].view(np.uint32))
+    # every triangle of the mesh once, in leaf order
+    ids = np.sort(a["tris"][:, 3].view(np.int32))
+    assert np.array_equal(ids, np.arange(a["num_tris"]))
+    summ = bvhinfo.summary(a)
+    print("bvh mode %d: %s" % (mode, summ))
+    assert summ["tlas_sah"] > 1.0 and len(summ["blas_sah"]) == 1
+    assert all(v > 1.0 for v in summ["blas_sah"].values())

@@ -433,7 +433,8 @@ def test_ctypes_struct_layouts_match_the_header(tmp_path):
     pairs = [("sundog_xform_step", _lib.XformStep), ("sundog_physics_body", _lib.PhysicsBody),
              ("sundog_render_options", _lib.RenderOptions), ("sundog_device_info", _lib.DeviceInfo),
              ("sundog_session_stats", _lib.SessionStats), ("sundog_request_info", _lib.RequestInfo),
-             ("sundog_server_stats", _lib.ServerStats), ("sundog_tune_result", _lib.TuneResult)]
+             ("sundog_server_stats", _lib.ServerStats), ("sundog_tune_result", _lib.TuneResult),
+             ("sundog_bvh_layout", _lib.BvhLayout)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sundog_api.h"', "int main(void) {"]
     for cname, py in pairs:
         lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))

@@ -224,32 +224,39 @@ SDEV vec3 rippleNormal(vec3 x, vec3 ns, vec3 wo, float amp, float freq, int octa
 // partial sum bounds the shaped distance dd from below: once
 // a + (partial - 0.45) >= 1.01, dd >= 1 and the mask is 0 whatever the
 // remaining octaves (0.01 of margin for rounding), and they are skipped.
-SDEV void flameField(const DevFlame& fl, vec3 p, float& sigma, vec3& emission) {
+//
+// Returns whether the field is non-zero; when it returns false, sigma and
+// emission are 0 (a march step there changes nothing). (1 - h)^1.55 is
+// 2^(1.55 log2(1 - h)) on the hardware log2 / exp2 (1 - h >= 2^-24 is
+// normal here; within 1e-6 relative of powf, far inside the parity
+// tolerance, and the cut-offs below sit where the mask is already 0).
+SDEV bool flameField(const DevFlame& fl, vec3 p, float& sigma, vec3& emission) {
   sigma = 0.0f;
   emission = v3(0.0f);
   const float h = (p.y - fl.base[1]) / fl.height;
-  if (h <= 0.0f || h >= 1.0f) return;
+  if (h <= 0.0f || h >= 1.0f) return false;
   const float rx = p.x - fl.base[0], rz = p.z - fl.base[2];
   const float r = sqrtf(rx * rx + rz * rz) / fl.radius;
-  const float profile = 3.2f * sqrtf(h) * powf(1.0f - h, 1.55f);
+  const float profile = 3.2f * sqrtf(h) * __builtin_amdgcn_exp2f(1.55f * __builtin_amdgcn_logf(1.0f - h));
   const float a = r / fmaxf(profile, 1e-3f);
-  if (a >= 1.46f) return;
+  if (a >= 1.46f) return false;
   vec3 q = vec3{rx / fl.radius, (p.y - fl.base[1]) / fl.radius * 0.32f, rz / fl.radius} * fl.noiseScale;
   float amp = 0.5f, n = 0.0f;
   for (int i = 0; i < fl.octaves; i++) {
     n += amp * valueNoise(q, octaveSeed(fl.seed, i));
-    if (a + (n - 0.45f) >= 1.01f) return;
+    if (a + (n - 0.45f) >= 1.01f) return false;
     q = nextOctave(q);
     amp *= 0.5f;
   }
   const float dd = a + (n - 0.45f) * 1.0f;
   const float mask = smoothstepf(1.0f, 0.62f, dd) * smoothstepf(0.0f, 0.12f, h);
-  if (mask <= 0.0f) return;
+  if (mask <= 0.0f) return false;
   sigma = fl.sigma * mask;
   const float heat = clampf((0.88f - dd) * (1.25f - h), 0.0f, 1.0f);
   const float h2 = heat * heat;
   const vec3 ramp = vec3{fminf(heat * 2.6f, 1.0f), fminf(h2 * 2.1f, 1.0f), fminf(h2 * heat * 1.7f, 1.0f)};
   emission = ramp * (fl.intensity * mask);
+  return true;
 }
 
 // Narrows [t0, t1] to the ray's overlap with one axis slab [lo, hi]; false
@@ -320,7 +327,9 @@ SDEV float marchVolumes(const LaunchParams& P, vec3 o, vec3 d, float tEnd, Pcg& 
     for (int k = 0; k < steps; k++) {
       float sigma;
       vec3 em;
-      flameField(fl, o + (t0 + (k + jitter) * dt) * d, sigma, em);
+      // a zero field adds +0 to acc (which starts at +0 and only grows) and
+      // multiplies tl by exp(-0) = 1: skipping the step changes no bit
+      if (!flameField(fl, o + (t0 + (k + jitter) * dt) * d, sigma, em)) continue;
       if (kEmit) acc += (tl * dt) * em;
       tl *= expf(-sigma * dt);
       if (tl < 1e-4f) break;

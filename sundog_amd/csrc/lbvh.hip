@@ -370,6 +370,40 @@ void exclusiveScan(const uint32_t* in, uint32_t* out, int n, std::vector<DevBuf>
   HIP_OK(hipGetLastError());
 }
 
+// The same scan on buffers allocated once for a largest n, for builds that
+// scan every round (allocating per call would cost more than the scans).
+struct ScanScratch {
+  std::vector<DevBuf> sums, offsets;  // one pair per level of block totals
+  void reserve(int n) {
+    sums.clear();
+    offsets.clear();
+    for (int blocks = (n + kScanChunk - 1) / kScanChunk; blocks > 1; blocks = (blocks + kScanChunk - 1) / kScanChunk) {
+      sums.emplace_back();
+      sums.back().alloc((size_t)blocks * 4);
+      offsets.emplace_back();
+      offsets.back().alloc((size_t)blocks * 4);
+    }
+  }
+};
+
+void exclusiveScan(const uint32_t* in, uint32_t* out, int n, ScanScratch& ws, hipStream_t s, size_t level = 0) {
+  const int blocks = (n + kScanChunk - 1) / kScanChunk;
+  if (blocks <= 1) {
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kScanThreads), 0, s, in, out, (uint32_t*)nullptr, n);
+    HIP_OK(hipGetLastError());
+    return;
+  }
+  if (level >= ws.sums.size() || ws.sums[level].bytes < (size_t)blocks * 4)
+    throw std::runtime_error("scan: workspace reserved for fewer elements");
+  uint32_t* sums = ws.sums[level].as<uint32_t>();
+  uint32_t* offsets = ws.offsets[level].as<uint32_t>();
+  hipLaunchKernelGGL(k_scan_block, dim3(blocks), dim3(kScanThreads), 0, s, in, out, sums, n);
+  HIP_OK(hipGetLastError());
+  exclusiveScan(sums, offsets, blocks, ws, s, level + 1);
+  hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, out, offsets, n);
+  HIP_OK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // LSD radix sort of the 64-bit keys by their Morton code (bits 32-61), four
 // stable 8-bit passes. The keys arrive in primitive order with the index in
@@ -515,8 +549,21 @@ __global__ void k_ploc_init(const float4* __restrict__ boxes, const uint64_t* __
 
 // Nearest neighbour of every cluster within the radius; the block's clusters
 // and the radius on either side are staged in LDS.
-__global__ __launch_bounds__(kLbvhBlock) void k_ploc_nn(const float4* __restrict__ cbox, int m, int radius,
+// Clustering state kept on the device, so that several rounds run between
+// host synchronisations (a round sized for more clusters than are left only
+// skips the extra positions; once one cluster is left, rounds do nothing).
+struct PlocState {
+  uint32_t m;        // clusters left
+  uint32_t created;  // merges so far (internal nodes written)
+  uint32_t error;    // a round that merged nothing or lost count
+  uint32_t rounds;   // rounds that merged
+};
+
+__global__ __launch_bounds__(kLbvhBlock) void k_ploc_nn(const float4* __restrict__ cbox,
+                                                        const PlocState* __restrict__ st, int radius,
                                                         uint32_t* __restrict__ nn) {
+  const int m = (int)st->m;
+  if (m <= 1) return;  // uniform: every thread of the block returns
   __shared__ float4 s[2 * (kLbvhBlock + 2 * kPlocRadiusMax)];
   const int span = kLbvhBlock + 2 * radius;
   const int base = (int)blockIdx.x * kLbvhBlock - radius;
@@ -548,23 +595,40 @@ __global__ __launch_bounds__(kLbvhBlock) void k_ploc_nn(const float4* __restrict
 
 // merge[i]: cluster i is the lower of a mutual pair (it becomes the new
 // node); keep[i]: cluster i survives the round (it is not the upper one).
-__global__ void k_ploc_flags(const uint32_t* __restrict__ nn, int m, uint32_t* __restrict__ merge,
-                             uint32_t* __restrict__ keep) {
+__global__ void k_ploc_flags(const uint32_t* __restrict__ nn, int mUpper, const PlocState* __restrict__ st,
+                             uint32_t* __restrict__ merge, uint32_t* __restrict__ keep) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
+  if (i >= mUpper) return;
+  const int m = (int)st->m;
+  if (m <= 1 || i >= m) {  // positions past the clusters left scan as zeros
+    merge[i] = 0u;
+    keep[i] = 0u;
+    return;
+  }
   const uint32_t j = nn[i];
   const bool mutual = nn[j] == (uint32_t)i;
   merge[i] = (mutual && (uint32_t)i < j) ? 1u : 0u;
   keep[i] = (mutual && (uint32_t)i > j) ? 0u : 1u;
 }
 
-// The round's totals: merges and surviving clusters.
+// The round's totals (merges, surviving clusters) into the state, checked:
+// a round merges at least one pair and every merge removes one cluster.
 __global__ void k_ploc_totals(const uint32_t* __restrict__ merge, const uint32_t* __restrict__ mergePos,
-                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ keepPos, int m,
-                              uint32_t* __restrict__ totals) {
+                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ keepPos, int mUpper,
+                              int numInternal, PlocState* __restrict__ st) {
   if (threadIdx.x != 0) return;
-  totals[0] = mergePos[m - 1] + merge[m - 1];
-  totals[1] = keepPos[m - 1] + keep[m - 1];
+  const uint32_t m = st->m;
+  if (m <= 1) return;
+  const uint32_t merges = mergePos[mUpper - 1] + merge[mUpper - 1];
+  const uint32_t kept = keepPos[mUpper - 1] + keep[mUpper - 1];
+  if (merges == 0 || kept != m - merges || st->created + merges > (uint32_t)numInternal) {
+    st->error = 1u;
+    st->m = 1u;  // stop the remaining rounds
+    return;
+  }
+  st->created += merges;
+  st->m = kept;
+  st->rounds += 1u;
 }
 
 // New binary nodes for the merged pairs and the compacted cluster list.
@@ -573,12 +637,15 @@ __global__ void k_ploc_totals(const uint32_t* __restrict__ merge, const uint32_t
 __global__ void k_ploc_merge(const uint32_t* __restrict__ clusters, const float4* __restrict__ cbox,
                              const uint32_t* __restrict__ nn, const uint32_t* __restrict__ merge,
                              const uint32_t* __restrict__ mergePos, const uint32_t* __restrict__ keep,
-                             const uint32_t* __restrict__ keepPos, int m, uint32_t created, int numInternal,
+                             const uint32_t* __restrict__ keepPos, const PlocState* __restrict__ st,
+                             int numInternal,
                              BinNode* __restrict__ nodes, uint32_t* __restrict__ leafParent,
                              uint32_t* __restrict__ count, uint32_t* __restrict__ outClusters,
                              float4* __restrict__ outBox) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || !keep[i]) return;
+  const int m = (int)st->m;
+  if (m <= 1 || i >= m || !keep[i]) return;
+  const uint32_t created = st->created;  // updated by k_ploc_totals after this kernel
   const uint32_t dst = keepPos[i];
   if (!merge[i]) {
     outClusters[dst] = clusters[i];
@@ -660,8 +727,9 @@ __global__ void k_ploc_keys(const uint64_t* __restrict__ keys, const uint32_t* _
 // (replacing `keys`).
 void plocTree(const float4* boxes, DevBuf& keys, int n, int radius, DevBuf& nodes, hipStream_t s) {
   if (radius < 1 || radius > kPlocRadiusMax) throw std::runtime_error("ploc: radius must be 1..32");
+  constexpr int kRoundsPerSync = 4;
   const int ni = n - 1;
-  DevBuf clA, clB, bxA, bxB, nn, merge, keep, mergePos, keepPos, count, leafParent, totals, dfsPos, keysDfs;
+  DevBuf clA, clB, bxA, bxB, nn, merge, keep, mergePos, keepPos, count, leafParent, state, dfsPos, keysDfs;
   clA.alloc((size_t)n * 4);
   clB.alloc((size_t)n * 4);
   bxA.alloc((size_t)n * 2 * sizeof(float4));
@@ -673,42 +741,47 @@ void plocTree(const float4* boxes, DevBuf& keys, int n, int radius, DevBuf& node
   keepPos.alloc((size_t)n * 4);
   count.alloc((size_t)ni * 4);
   leafParent.alloc((size_t)n * 4);
-  totals.alloc(8);
+  state.alloc(sizeof(PlocState));
   nodes.alloc((size_t)ni * sizeof(BinNode));
+  ScanScratch ws;
+  ws.reserve(n);
+  const PlocState init{(uint32_t)n, 0u, 0u, 0u};
+  HIP_OK(hipMemcpyAsync(state.p, &init, sizeof init, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_ploc_init, dim3(grid(n)), dim3(kLbvhBlock), 0, s, boxes, keys.as<uint64_t>(), n,
                      clA.as<uint32_t>(), bxA.as<float4>());
   HIP_OK(hipGetLastError());
-  int m = n;
-  uint32_t created = 0;
+  PlocState* st = state.as<PlocState>();
+  int mUpper = n;  // clusters left as of the last synchronisation (an upper bound afterwards)
   DevBuf *cl = &clA, *clOut = &clB, *bx = &bxA, *bxOut = &bxB;
-  for (int round = 0; m > 1; round++) {
-    if (round > 2 * n + 64) throw std::runtime_error("ploc: clustering did not converge");
-    std::vector<DevBuf> pool;  // scan temporaries of this round, freed after its synchronisation
-    hipLaunchKernelGGL(k_ploc_nn, dim3(grid(m)), dim3(kLbvhBlock), 0, s, bx->as<float4>(), m, radius,
-                       nn.as<uint32_t>());
-    hipLaunchKernelGGL(k_ploc_flags, dim3(grid(m)), dim3(kLbvhBlock), 0, s, nn.as<uint32_t>(), m,
-                       merge.as<uint32_t>(), keep.as<uint32_t>());
-    HIP_OK(hipGetLastError());
-    exclusiveScan(merge.as<uint32_t>(), mergePos.as<uint32_t>(), m, pool, s);
-    exclusiveScan(keep.as<uint32_t>(), keepPos.as<uint32_t>(), m, pool, s);
-    hipLaunchKernelGGL(k_ploc_merge, dim3(grid(m)), dim3(kLbvhBlock), 0, s, cl->as<uint32_t>(), bx->as<float4>(),
-                       nn.as<uint32_t>(), merge.as<uint32_t>(), mergePos.as<uint32_t>(), keep.as<uint32_t>(),
-                       keepPos.as<uint32_t>(), m, created, ni, nodes.as<BinNode>(), leafParent.as<uint32_t>(),
-                       count.as<uint32_t>(), clOut->as<uint32_t>(), bxOut->as<float4>());
-    hipLaunchKernelGGL(k_ploc_totals, dim3(1), dim3(64), 0, s, merge.as<uint32_t>(), mergePos.as<uint32_t>(),
-                       keep.as<uint32_t>(), keepPos.as<uint32_t>(), m, totals.as<uint32_t>());
-    HIP_OK(hipGetLastError());
-    uint32_t t[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(t, totals.p, 8, hipMemcpyDeviceToHost, s));
+  PlocState h{};
+  for (int batch = 0;; batch++) {
+    if (batch * kRoundsPerSync > n + 64) throw std::runtime_error("ploc: clustering did not converge");
+    for (int r = 0; r < kRoundsPerSync; r++) {
+      hipLaunchKernelGGL(k_ploc_nn, dim3(grid(mUpper)), dim3(kLbvhBlock), 0, s, bx->as<float4>(), st, radius,
+                         nn.as<uint32_t>());
+      hipLaunchKernelGGL(k_ploc_flags, dim3(grid(mUpper)), dim3(kLbvhBlock), 0, s, nn.as<uint32_t>(), mUpper, st,
+                         merge.as<uint32_t>(), keep.as<uint32_t>());
+      HIP_OK(hipGetLastError());
+      exclusiveScan(merge.as<uint32_t>(), mergePos.as<uint32_t>(), mUpper, ws, s);
+      exclusiveScan(keep.as<uint32_t>(), keepPos.as<uint32_t>(), mUpper, ws, s);
+      hipLaunchKernelGGL(k_ploc_merge, dim3(grid(mUpper)), dim3(kLbvhBlock), 0, s, cl->as<uint32_t>(),
+                         bx->as<float4>(), nn.as<uint32_t>(), merge.as<uint32_t>(), mergePos.as<uint32_t>(),
+                         keep.as<uint32_t>(), keepPos.as<uint32_t>(), st, ni, nodes.as<BinNode>(),
+                         leafParent.as<uint32_t>(), count.as<uint32_t>(), clOut->as<uint32_t>(),
+                         bxOut->as<float4>());
+      hipLaunchKernelGGL(k_ploc_totals, dim3(1), dim3(64), 0, s, merge.as<uint32_t>(), mergePos.as<uint32_t>(),
+                         keep.as<uint32_t>(), keepPos.as<uint32_t>(), mUpper, ni, st);
+      HIP_OK(hipGetLastError());
+      std::swap(cl, clOut);
+      std::swap(bx, bxOut);
+    }
+    HIP_OK(hipMemcpyAsync(&h, state.p, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (t[0] == 0 || created + t[0] > (uint32_t)ni || t[1] != (uint32_t)m - t[0])
-      throw std::runtime_error("ploc: inconsistent round (merges " + std::to_string(t[0]) + ", clusters " +
-                               std::to_string(t[1]) + " of " + std::to_string(m) + ")");
-    created += t[0];
-    m = (int)t[1];
-    std::swap(cl, clOut);
-    std::swap(bx, bxOut);
+    if (h.error) throw std::runtime_error("ploc: inconsistent round");
+    if (h.m <= 1) break;
+    mUpper = (int)h.m;
   }
+  const uint32_t created = h.created;
   if (created != (uint32_t)ni) throw std::runtime_error("ploc: incomplete tree");
   dfsPos.alloc((size_t)n * 4);
   keysDfs.alloc((size_t)n * 8);
@@ -841,6 +914,8 @@ int collapseWide(const BinNode* nodes, const float4* boxes, const uint64_t* keys
                  uint32_t nodeBase, uint32_t primBase, DevBuf& out, int& stackBound, hipStream_t s) {
   const int ni = n - 1;
   DevBuf fA, fB, aA, aB, innerCount, innerPos, total, bound;
+  ScanScratch ws;
+  ws.reserve(ni);
   fA.alloc((size_t)ni * 4);
   fB.alloc((size_t)ni * 4);
   aA.alloc((size_t)ni * 4);
@@ -859,11 +934,10 @@ int collapseWide(const BinNode* nodes, const float4* boxes, const uint64_t* keys
   while (nF > 0) {
     if (levelBase + (uint32_t)nF > (uint32_t)ni)
       throw std::runtime_error("bvh collapse: more 4-wide nodes than binary nodes");
-    std::vector<DevBuf> pool;
     hipLaunchKernelGGL(k_wide_count, dim3(grid(nF)), dim3(kLbvhBlock), 0, s, nodes, fr->as<uint32_t>(), nF, maxLeaf,
                        innerCount.as<uint32_t>());
     HIP_OK(hipGetLastError());
-    exclusiveScan(innerCount.as<uint32_t>(), innerPos.as<uint32_t>(), nF, pool, s);
+    exclusiveScan(innerCount.as<uint32_t>(), innerPos.as<uint32_t>(), nF, ws, s);
     hipLaunchKernelGGL(k_wide_total, dim3(1), dim3(64), 0, s, innerCount.as<uint32_t>(), innerPos.as<uint32_t>(), nF,
                        total.as<uint32_t>());
     hipLaunchKernelGGL(k_wide_emit, dim3(grid(nF)), dim3(kLbvhBlock), 0, s, nodes, boxes, keys, fr->as<uint32_t>(),

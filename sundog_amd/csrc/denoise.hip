@@ -390,28 +390,35 @@ static void launchHalo(const HaloArgs& A, hipStream_t s) {
 // distinct banks. Wave w = 3 * coTile + tapRow forms the 32x32 tiles of its
 // co tile for the three taps of its tap row (dy = tapRow - 1): per 16-pixel
 // K step one A read and three B reads feed three MFMAs. The next tile's
-// global loads are issued into registers before the current tile's MFMAs.
-// The block's threads hold fixed co channels while staging dy (the block has
-// 6 * Cout threads), so each sums its own dy values in FP32 for the bias in
-// a fixed order, and the six per-channel sums are added in thread order.
+// global loads (16-byte loads of four dy channels or eight input channels
+// of one pixel, consecutive lanes at consecutive pixels) are issued into
+// registers before the current tile's MFMAs and transposed into the LDS
+// layouts with 2-byte stores. Each staging slot of a thread always holds the
+// same pixel of the tile and the same four dy channels, so it sums them in
+// FP32 for the bias in tile order; a fixed butterfly over a wave's 64 pixel
+// slots then gives each channel's sum.
 // ---------------------------------------------------------------------------
 template <int kTC, int kNco>
 struct DwTile {
   static constexpr int kRows = 64 / kTC;            // output rows per tile
   static constexpr int kHalo = kRows + 2;           // input rows per tile
-  static constexpr int kSegs = kTC / 8;             // 8-pixel segments per row
   static constexpr int kXStride = kHalo * kTC + 8;  // BF16 per (dx, ci) plane
   static constexpr int kDyStride = 64 + 8;          // BF16 per co row
   static constexpr int kCout = 32 * kNco;
   static constexpr int kThreads = 192 * kNco;       // 3 waves per co tile = 6 * kCout
-  static constexpr int kDyTasks = (kCout * 8 + kThreads - 1) / kThreads;
-  static constexpr int kXItems = 32 * kHalo * kSegs;
-  static constexpr int kXTasks = (kXItems + kThreads - 1) / kThreads;
+  // staging: 16-byte loads, pixel-fastest across lanes
+  static constexpr int kCo4 = kCout / 4;            // float4 groups of a pixel's dy
+  static constexpr int kDyVec = 64 * kCo4;          // dy float4 loads per tile
+  static constexpr int kDyTasks = (kDyVec + kThreads - 1) / kThreads;
+  static constexpr int kXCols = kTC + 2;            // input columns x0 - 1 .. x0 + kTC
+  static constexpr int kXVec = kHalo * kXCols * 4;  // x loads of 8 channels per tile
+  static constexpr int kXTasks = (kXVec + kThreads - 1) / kThreads;
+  static_assert(kThreads % 64 == 0, "a wave's slots each hold one channel group");
 };
 
 struct DwArgs {
   const float* dy;     // [N][H][W][Cout] FP32
-  const uint16_t* x;   // [N][H][W][Cin] BF16
+  const uint16_t* x;   // [N][H][W][Cin] BF16, Cin a multiple of 8
   float* partial;      // [chunks][Cout * 9 * Cin + Cout]
   int N, H, W, Cin, Cout;
   int tilesX, tilesY, tiles, tilesPerChunk;
@@ -420,10 +427,15 @@ struct DwArgs {
 template <int kTC, int kNco>
 struct DwRegs {
   using T = DwTile<kTC, kNco>;
-  float dy[T::kDyTasks][8];
-  uint16_t x[T::kXTasks][10];  // input columns c0 - 1 .. c0 + 8 of one segment
+  float4 dy[T::kDyTasks];  // four channels of one pixel
+  uint4 x[T::kXTasks];     // eight BF16 channels of one input pixel
 };
 
+// Loads one tile's operands into registers with 16-byte loads. dy task k
+// (of kDyVec): pixel k % 64 of the tile, channels 4 (k / 64) .. +3, so a
+// wave's lanes read 64 pixels of one channel group; x task k (of kXVec):
+// column k % kXCols of halo row k / (4 kXCols), channels ci0 + 8 ((k /
+// kXCols) % 4) .. +7. Out-of-image pixels and channels past Cin load zeros.
 template <int kTC, int kNco>
 __device__ __forceinline__ void dwLoad(const DwArgs& A, int tile, int ci0, DwRegs<kTC, kNco>& R) {
   using T = DwTile<kTC, kNco>;
@@ -434,58 +446,60 @@ __device__ __forceinline__ void dwLoad(const DwArgs& A, int tile, int ci0, DwReg
 #pragma unroll
   for (int i = 0; i < T::kDyTasks; i++) {
     const int task = tid + i * T::kThreads;
-    const int co = task % T::kCout, seg = task / T::kCout;
-    const int y = y0 + seg / T::kSegs, xc = x0 + 8 * (seg % T::kSegs);
-    const bool rowOk = task < T::kCout * 8 && y < A.H;
-    const float* src = A.dy + ((size_t)(n * A.H + (rowOk ? y : 0)) * A.W) * T::kCout + co;
-#pragma unroll
-    for (int j = 0; j < 8; j++) R.dy[i][j] = rowOk && xc + j < A.W ? src[(size_t)(xc + j) * T::kCout] : 0.0f;
+    const int px = task & 63, co4 = task >> 6;
+    const int y = y0 + px / kTC, c = x0 + px % kTC;
+    const bool ok = task < T::kDyVec && y < A.H && c < A.W;
+    R.dy[i] = ok ? *reinterpret_cast<const float4*>(A.dy + ((size_t)(n * A.H + y) * A.W + c) * T::kCout + 4 * co4)
+                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 #pragma unroll
   for (int i = 0; i < T::kXTasks; i++) {
     const int task = tid + i * T::kThreads;
-    const int cil = task % 32, rest = task / 32;
-    const int hr = rest % T::kHalo, cs = rest / T::kHalo;
-    const int ci = ci0 + cil, y = y0 - 1 + hr, xc = x0 + 8 * cs - 1;
-    const bool rowOk = task < T::kXItems && ci < A.Cin && y >= 0 && y < A.H;
-    const uint16_t* src = A.x + ((size_t)(n * A.H + (rowOk ? y : 0)) * A.W) * A.Cin + (rowOk ? ci : 0);
-#pragma unroll
-    for (int j = 0; j < 10; j++) {
-      const int c = xc + j;
-      R.x[i][j] = rowOk && c >= 0 && c < A.W ? src[(size_t)c * A.Cin] : (uint16_t)0;
-    }
+    const int col = task % T::kXCols, rest = task / T::kXCols;
+    const int cg = rest & 3, hr = rest >> 2;
+    const int y = y0 - 1 + hr, c = x0 - 1 + col, ci = ci0 + 8 * cg;
+    const bool ok = task < T::kXVec && ci < A.Cin && y >= 0 && y < A.H && c >= 0 && c < A.W;
+    R.x[i] = ok ? *reinterpret_cast<const uint4*>(A.x + ((size_t)(n * A.H + y) * A.W + c) * A.Cin + ci)
+                : make_uint4(0u, 0u, 0u, 0u);
   }
 }
 
+// Writes the registers into the LDS layouts the MFMAs read (dys[co][px] in
+// BF16; xs[dx][ci][halo row][col], the column shifted by dx = -1, 0, +1 in
+// the three copies) with 2-byte stores, consecutive lanes at consecutive
+// pixels, and adds each dy value to its slot's FP32 bias sum (a slot always
+// holds the same pixel and channels).
 template <int kTC, int kNco>
-__device__ __forceinline__ void dwStore(const DwRegs<kTC, kNco>& R, uint16_t* dys, uint16_t* xs, float& bias) {
+__device__ __forceinline__ void dwStore(const DwRegs<kTC, kNco>& R, uint16_t* dys, uint16_t* xs,
+                                        float (&bias)[DwTile<kTC, kNco>::kDyTasks][4]) {
   using T = DwTile<kTC, kNco>;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < T::kDyTasks; i++) {
     const int task = tid + i * T::kThreads;
-    if (task >= T::kCout * 8) continue;
-    const int co = task % T::kCout, seg = task / T::kCout;
-    u16x8 v;
+    if (task >= T::kDyVec) continue;
+    const int px = task & 63, co4 = task >> 6;
+    const float v[4] = {R.dy[i].x, R.dy[i].y, R.dy[i].z, R.dy[i].w};
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      v[j] = f2bf(R.dy[i][j]);
-      bias += R.dy[i][j];
+    for (int k = 0; k < 4; k++) {
+      dys[(4 * co4 + k) * T::kDyStride + px] = f2bf(v[k]);
+      bias[i][k] += v[k];
     }
-    *(u16x8*)(dys + co * T::kDyStride + seg * 8) = v;
   }
 #pragma unroll
   for (int i = 0; i < T::kXTasks; i++) {
     const int task = tid + i * T::kThreads;
-    if (task >= T::kXItems) continue;
-    const int cil = task % 32, rest = task / 32;
-    const int hr = rest % T::kHalo, cs = rest / T::kHalo;
+    if (task >= T::kXVec) continue;
+    const int col = task % T::kXCols, rest = task / T::kXCols;
+    const int cg = rest & 3, hr = rest >> 2;
+    const uint32_t w[4] = {R.x[i].x, R.x[i].y, R.x[i].z, R.x[i].w};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      u16x8 v;
+      const int cOut = col - d;  // copy d holds input column x0 - 1 + cOut + d at column cOut
+      if (cOut < 0 || cOut >= kTC) continue;
+      uint16_t* dst = xs + (d * 32 + 8 * cg) * T::kXStride + hr * kTC + cOut;
 #pragma unroll
-      for (int j = 0; j < 8; j++) v[j] = R.x[i][j + d];
-      *(u16x8*)(xs + (d * 32 + cil) * T::kXStride + hr * kTC + 8 * cs) = v;
+      for (int k = 0; k < 8; k++) dst[k * T::kXStride] = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
     }
   }
 }
@@ -495,14 +509,13 @@ __global__ __launch_bounds__(192 * kNco) void k_conv_dw(DwArgs A) {
   using T = DwTile<kTC, kNco>;
   __shared__ __align__(16) uint16_t dys[T::kCout * T::kDyStride];
   __shared__ __align__(16) uint16_t xs[3 * 32 * T::kXStride];
-  __shared__ float biasPart[T::kThreads];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row = lane & 31, half = lane >> 5;
   const int coTile = wave / 3, tapRow = wave % 3;
   const int chunk = blockIdx.x, ci0 = blockIdx.y * 32;
   const int t0 = chunk * A.tilesPerChunk, t1 = min(A.tiles, t0 + A.tilesPerChunk);
   f32x16 acc[3] = {};
-  float bias = 0.0f;
+  float bias[T::kDyTasks][4] = {};
   DwRegs<kTC, kNco> R;
   dwLoad<kTC, kNco>(A, t0, ci0, R);
   for (int tile = t0; tile < t1; tile++) {
@@ -513,7 +526,7 @@ __global__ __launch_bounds__(192 * kNco) void k_conv_dw(DwArgs A) {
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
       const int s = 2 * ks + half;  // this half-wave's 8-pixel segment
-      const int r = s / T::kSegs, cs = s % T::kSegs;
+      const int r = s / (kTC / 8), cs = s % (kTC / 8);
       const u16x8 a = *(const u16x8*)(dys + (coTile * 32 + row) * T::kDyStride + s * 8);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
@@ -537,14 +550,21 @@ __global__ __launch_bounds__(192 * kNco) void k_conv_dw(DwArgs A) {
       }
     }
   }
-  if (blockIdx.y == 0) {  // thread tid staged channel tid % kCout
-    biasPart[tid] = bias;
-    __syncthreads();
-    if (tid < T::kCout) {
-      float s = biasPart[tid];
+  if (blockIdx.y == 0) {
+    // A wave's lanes hold the 64 pixels of one channel group in each slot
+    // (slot i of wave w: group w + i * kThreads / 64): a fixed-order
+    // butterfly over the lanes gives the group's four sums.
 #pragma unroll
-      for (int g = 1; g < 6; g++) s += biasPart[g * T::kCout + tid];
-      dst[nW + tid] = s;
+    for (int i = 0; i < T::kDyTasks; i++) {
+      const int co4 = wave + i * (T::kThreads / 64);
+      if (co4 >= T::kCo4) continue;  // uniform per wave
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float v = bias[i][k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) dst[nW + 4 * co4 + k] = v;
+      }
     }
   }
 }
@@ -1060,6 +1080,7 @@ struct sundog_denoiser {
     A.tilesX = (w + tc - 1) / tc;
     A.tilesY = (h + 64 / tc - 1) / (64 / tc);
     A.tiles = N * A.tilesX * A.tilesY;
+    if (c.cin % 8) throw std::runtime_error("denoiser: backward-weights needs Cin a multiple of 8");
     const int ciTiles = (c.cin + 31) / 32;
     // about kDwBlocks blocks, each over at least one tile, with at most
     // kDwPartialBytes of partial sums

@@ -156,7 +156,9 @@ struct LaunchParams {
 // Persistent tile-queue work item: one tile of one request for one pass.
 struct WorkItem {
   int32_t request;
-  int32_t tile;  // tile index in row-major order over the request's frame
+  int32_t tile;   // tile index in row-major order over the request's frame
+  int32_t group;  // sample group of the pass (0 when the pass is not split)
+  int32_t pad;
 };
 
 // The persistent kernel takes the resident scenes' parameters by value (a
@@ -177,6 +179,12 @@ struct SceneTable {
 struct RequestDev {
   int32_t scene;  // index into the launch's SceneTable
   int32_t width, height, sppTotal, sampleOffset, sppThisLaunch, passSpp;
+  // Sample groups of a pass: 1 folds each tile's samples in the kernel;
+  // more split a pass's samples over that many work items per tile, which
+  // write per-sample results to the pool regions of the batch's group
+  // buffer starting at regionBase (tiles x pools per tile of them), folded
+  // afterwards by k_fold_groups.
+  int32_t groups, regionBase;
   uint32_t seed;
   float4* accum;
   float4* albedo;

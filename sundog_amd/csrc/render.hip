@@ -1057,7 +1057,7 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
                                                                const WorkItem* __restrict__ items, int numItems,
                                                                QueueState* q, int tileSize, uint32_t* spill,
                                                                int spillStride, int stackLds, float* sampleBuf,
-                                                               unsigned long long* rayCounter) {
+                                                               float* groupBuf, unsigned long long* rayCounter) {
   extern __shared__ uint64_t s_stack[];  // stackLds * kBlock entries
   __shared__ uint32_t s_poolNext[kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1078,8 +1078,10 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
     if (item >= (uint32_t)numItems) break;
     const int req = __builtin_amdgcn_readfirstlane(items[item].request);
     const int tile = __builtin_amdgcn_readfirstlane(items[item].tile);
+    const int group = __builtin_amdgcn_readfirstlane(items[item].group);
     const RequestDev& R = reqs[req];
     if (R.sppThisLaunch <= 0) continue;  // request already complete
+    const int groups = __builtin_amdgcn_readfirstlane(R.groups);
     LaunchParams P = S.scene[__builtin_amdgcn_readfirstlane(R.scene)];
     P.width = R.width;
     P.height = R.height;
@@ -1093,6 +1095,14 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
     P.cam = R.cam;
     const int tilesX = (P.width + tileSize - 1) / tileSize;
     const int tx = (tile % tilesX) * tileSize, ty = (tile / tilesX) * tileSize;
+    // this item's share of the pass's samples (all of them unless split)
+    int gBegin = 0, gEnd = P.sppThisLaunch;
+    if (groups > 1) {
+      const int per = (P.sppThisLaunch + groups - 1) / groups;
+      gBegin = min(P.sppThisLaunch, group * per);
+      gEnd = min(P.sppThisLaunch, gBegin + per);
+      if (gBegin >= gEnd) continue;
+    }
     // each 64-pixel group of the tile drains one shared sample pool
     for (int base = 0; base < pxPerTile; base += 64) {
       auto slotPixel = [&](int j, int& px, int& py) -> bool {
@@ -1101,10 +1111,53 @@ __global__ __launch_bounds__(kBlock, kWaves) void k_persistent(SceneTable S, con
         py = ty + (k / tileSize);
         return k < pxPerTile && px < P.width && py < P.height;
       };
-      renderWavePooled(P, slotPixel, &s_poolNext[wave], waveBuf, st, wc, rays, 0, P.sppThisLaunch, true);
+      if (groups > 1) {
+        // samples [gBegin, gEnd) into the pool's region; k_fold_groups folds
+        const size_t region = (size_t)R.regionBase + (size_t)tile * (pxPerTile / 64) + base / 64;
+        renderWavePooled(P, slotPixel, &s_poolNext[wave], groupBuf + region * (kPoolPlanes * kPoolSamples * 64),
+                         st, wc, rays, gBegin, gEnd, false);
+      } else {
+        renderWavePooled(P, slotPixel, &s_poolNext[wave], waveBuf, st, wc, rays, 0, P.sppThisLaunch, true);
+      }
     }
   }
   flushCounters<false>(rayCounter != nullptr, rayCounter, nullptr, rays, wc);
+}
+
+// The fold of the passes that k_persistent split into sample groups: block
+// (item, pool) folds the 64 pixels of one pool of a group-0 item's tile, each
+// pixel's samples in sample order into its running means (the same
+// operations as the in-kernel fold, so films stay bit-identical).
+__global__ __launch_bounds__(64) void k_fold_groups(const RequestDev* __restrict__ reqs,
+                                                    const WorkItem* __restrict__ items, int tileSize,
+                                                    const float* __restrict__ groupBuf) {
+  const int pools = tileSize * tileSize / 64;
+  const int item = blockIdx.x / pools, pool = blockIdx.x % pools;
+  const WorkItem& wi = items[item];
+  if (wi.group != 0) return;
+  const RequestDev& R = reqs[wi.request];
+  if (R.groups <= 1 || R.sppThisLaunch <= 0) return;
+  const int tilesX = (R.width + tileSize - 1) / tileSize;
+  const int tx = (wi.tile % tilesX) * tileSize, ty = (wi.tile / tilesX) * tileSize;
+  const int j = threadIdx.x, k = pool * 64 + j;
+  const int px = tx + k % tileSize, py = ty + k / tileSize;
+  if (px >= R.width || py >= R.height) return;
+  const size_t region = (size_t)R.regionBase + (size_t)wi.tile * pools + pool;
+  const float* buf = groupBuf + region * (kPoolPlanes * kPoolSamples * 64);
+  const int pixel = py * R.width + px;
+  float4 fAcc = R.accum[pixel], fAlb = R.albedo[pixel], fNrm = R.normal[pixel];
+  for (int si = 0; si < R.sppThisLaunch; si++) {
+    float x[kPoolPlanes];
+#pragma unroll
+    for (int c = 0; c < kPoolPlanes; c++) x[c] = buf[c * (kPoolSamples * 64) + si * 64 + j];
+    const int s = R.sampleOffset + si;
+    fAcc = runningMean(fAcc, vec3{x[0], x[1], x[2]}, s);
+    fAlb = runningMean(fAlb, vec3{x[3], x[4], x[5]}, s);
+    fNrm = runningMean(fNrm, vec3{x[6], x[7], x[8]}, s);
+  }
+  R.accum[pixel] = fAcc;
+  R.albedo[pixel] = fAlb;
+  R.normal[pixel] = fNrm;
 }
 
 // ---------------------------------------------------------------------------
@@ -1723,23 +1776,32 @@ hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* first
 
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, unsigned long long* rayCounter,
-                            hipStream_t s) {
+                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, float* groupBuf,
+                            unsigned long long* rayCounter, hipStream_t s) {
   const size_t lds = (size_t)stackLds * kBlock * sizeof(uint64_t);
   // every wave of the grid owns one per-sample result region
   if (!sampleBuf || sampleBufWaves < (int64_t)blocks * (kBlock / 64)) return hipErrorInvalidValue;
   if (waves >= 5)
     hipLaunchKernelGGL((k_persistent<5>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, groupBuf, rayCounter);
   else if (waves == 4)
     hipLaunchKernelGGL((k_persistent<4>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, groupBuf, rayCounter);
   else if (waves == 3)
     hipLaunchKernelGGL((k_persistent<3>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, groupBuf, rayCounter);
   else
     hipLaunchKernelGGL((k_persistent<2>), dim3(blocks), dim3(kBlock), lds, s, S, reqs, items, numItems, q,
-                       tileSize, spill, spillStride, stackLds, sampleBuf, rayCounter);
+                       tileSize, spill, spillStride, stackLds, sampleBuf, groupBuf, rayCounter);
+  return hipGetLastError();
+}
+
+hipError_t launchFoldGroups(const RequestDev* reqs, const WorkItem* items, int numItems, int tileSize,
+                            const float* groupBuf, hipStream_t s) {
+  if (numItems <= 0) return hipSuccess;
+  if (tileSize != 8 && tileSize != 16 && tileSize != 32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fold_groups, dim3((unsigned)numItems * (tileSize * tileSize / 64)), dim3(64), 0, s, reqs,
+                     items, tileSize, groupBuf);
   return hipGetLastError();
 }
 

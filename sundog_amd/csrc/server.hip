@@ -60,6 +60,11 @@ struct ServerRequest {
   LaunchParams P{};
 };
 
+// Largest group buffer of a batch, in 64-pixel pool regions (36 KiB each:
+// 9 planes x 16 samples x 64 pixels of FP32): batches whose tiles hold more
+// pools are not split into sample groups.
+constexpr int64_t kMaxGroupPools = 16384;
+
 struct sundog_server {
   std::vector<std::unique_ptr<sundog_session>> scenes;
   size_t budget = 0;
@@ -70,12 +75,17 @@ struct sundog_server {
   int stackLds = 16;     // LDS traversal-stack entries per lane
   int blocksReq = 0;     // queue depth: resident persistent blocks per CU (0 = as many as fit)
   int graphPasses = 4;   // progressive passes captured into one graph launch (1 = no graph)
+  // sample groups per pass: 0 splits each tile's pass over enough work items
+  // to reach the resident waves when a batch's tiles alone cannot (at most
+  // pass_spp groups, and only with pass_spp <= 16); 1 never splits; 2..16
+  // always split that many ways (capped at pass_spp)
+  int groupsReq = 0;
   int serialWaves = 0;   // occupancy variant of the per-request megakernel baseline (0: picked per request)
   int blocksPerCU = 1;
   int numCUs = 1;
   int maxStackNeed = 0;  // deepest traversal stack over the resident scenes
   hipStream_t stream = nullptr;
-  DevBuf dReqs, dItems, dQueue, dFirst, spill, sampleBuf, rayCounter;
+  DevBuf dReqs, dItems, dQueue, dFirst, spill, sampleBuf, rayCounter, groupBuf;
   int64_t sampleBufWaves = 0;  // per-sample result regions (one per wave of the persistent grid)
   size_t reqCap = 0, itemCap = 0, queueCap = 0;
   Clock::time_point t0;
@@ -214,6 +224,23 @@ struct sundog_server {
     std::vector<int32_t> first;
     std::vector<WorkItem> work;
     int lastScene = -1;
+    // sample groups of this batch's passes (the same for every request)
+    const int poolsPerTile = tileSize * tileSize / 64;
+    const int residentWaves = numCUs * blocksPerCU * (renderBlockThreads() / 64);
+    int64_t totalTiles = 0;
+    for (const ServerRequest* rp : order)
+      totalTiles += (int64_t)((rp->width + tileSize - 1) / tileSize) * ((rp->height + tileSize - 1) / tileSize);
+    int sampleGroups = 1;
+    if (groupsReq != 1 && passSpp >= 2 && passSpp <= kPoolSamples && totalTiles * poolsPerTile <= kMaxGroupPools) {
+      if (groupsReq >= 2) sampleGroups = std::min(groupsReq, passSpp);
+      else if (totalTiles < residentWaves)
+        sampleGroups = (int)std::min<int64_t>(passSpp, (residentWaves + totalTiles - 1) / totalTiles);
+    }
+    if (sampleGroups > 1) {
+      const size_t need = (size_t)(totalTiles * poolsPerTile) * kPoolPlanes * kPoolSamples * 64 * sizeof(float);
+      if (groupBuf.bytes < need) groupBuf.alloc(need);
+    }
+    int64_t regionNext = 0;
     for (ServerRequest* rp : order) {
       ServerRequest& r = *rp;
       if (groups.empty() || (r.scene != lastScene && groups.back().numScenes == kMaxScenesPerLaunch)) {
@@ -232,10 +259,15 @@ struct sundog_server {
         lastScene = r.scene;
       }
       int local = (int)(reqs.size() - g.reqBegin);
-      reqs.push_back(requestRecord(r, g.numScenes - 1));
+      const int tx = (r.width + tileSize - 1) / tileSize, ty = (r.height + tileSize - 1) / tileSize;
+      RequestDev rd = requestRecord(r, g.numScenes - 1);
+      rd.groups = sampleGroups;
+      rd.regionBase = (int32_t)regionNext;
+      if (sampleGroups > 1) regionNext += (int64_t)tx * ty * poolsPerTile;
+      reqs.push_back(rd);
       first.push_back(r.done);
-      int tx = (r.width + tileSize - 1) / tileSize, ty = (r.height + tileSize - 1) / tileSize;
-      for (int t = 0; t < tx * ty; t++) work.push_back(WorkItem{(int32_t)local, t});
+      for (int t = 0; t < tx * ty; t++)
+        for (int k = 0; k < sampleGroups; k++) work.push_back(WorkItem{(int32_t)local, t, k, 0});
       g.reqEnd = reqs.size();
       g.itemEnd = work.size();
     }
@@ -265,10 +297,12 @@ struct sundog_server {
           QueueState* q = dQueue.as<QueueState>() + gi;
           RequestDev* rq = dReqs.as<RequestDev>() + g.reqBegin;
           HIP_OK(launchQueuePrep(q, rq, dFirst.as<int32_t>() + g.reqBegin, (int)(g.reqEnd - g.reqBegin), stream));
-          HIP_OK(launchPersistent(g.table, rq, dItems.as<WorkItem>() + g.itemBegin, (int)(g.itemEnd - g.itemBegin),
-                                  q, tileSize, waves, blocks, spill.as<uint32_t>(), (int)threads, stackLds,
-                                  sampleBuf.as<float>(), sampleBufWaves, rayCounter.as<unsigned long long>(),
-                                  stream));
+          const WorkItem* gi0 = dItems.as<WorkItem>() + g.itemBegin;
+          const int nItems = (int)(g.itemEnd - g.itemBegin);
+          HIP_OK(launchPersistent(g.table, rq, gi0, nItems, q, tileSize, waves, blocks, spill.as<uint32_t>(),
+                                  (int)threads, stackLds, sampleBuf.as<float>(), sampleBufWaves,
+                                  groupBuf.as<float>(), rayCounter.as<unsigned long long>(), stream));
+          if (sampleGroups > 1) HIP_OK(launchFoldGroups(rq, gi0, nItems, tileSize, groupBuf.as<float>(), stream));
         }
       }
     };
@@ -508,6 +542,11 @@ extern "C" int sundog_server_set_tuning(sundog_server* s, const char* key, int v
       if (value != 0 && (value < 2 || value > 5))
         throw std::runtime_error("tuning: serial_waves must be 0 (auto) or 2..5");
       s->serialWaves = value;
+    } else if (k == "groups") {
+      if (value < 0 || value > kPoolSamples)
+        throw std::runtime_error("tuning: groups must be 0 (auto), 1 (never split) or 2.." + std::to_string(kPoolSamples));
+      s->requireIdle("groups");
+      s->groupsReq = value;
     } else if (k == "pass_spp") {
       if (value <= 0) throw std::runtime_error("tuning: pass_spp must be positive");
       s->requireIdle("pass_spp");

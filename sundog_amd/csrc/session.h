@@ -40,8 +40,10 @@ hipError_t launchQueuePrep(QueueState* q, RequestDev* reqs, const int32_t* first
                            hipStream_t s);
 hipError_t launchPersistent(const SceneTable& S, const RequestDev* reqs, const WorkItem* items, int numItems,
                             QueueState* q, int tileSize, int waves, int blocks, uint32_t* spill, int spillStride,
-                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, unsigned long long* rayCounter,
-                            hipStream_t s);
+                            int stackLds, float* sampleBuf, int64_t sampleBufWaves, float* groupBuf,
+                            unsigned long long* rayCounter, hipStream_t s);
+hipError_t launchFoldGroups(const RequestDev* reqs, const WorkItem* items, int numItems, int tileSize,
+                            const float* groupBuf, hipStream_t s);
 int persistentBlocksPerCU(int waves, int stackLds);
 hipError_t launchWfShade(const LaunchParams& P, const WfParams& W, int blocks, hipStream_t s);
 hipError_t launchWfClosest(const LaunchParams& P, const WfParams& W, int blocks, int waves, hipStream_t s);

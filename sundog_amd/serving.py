@@ -175,6 +175,8 @@ def main(argv=None):
                     help="occupancy variant of the per-request megakernel baseline (0: picked per "
                          "request by timing its first passes, as a session does)")
     ap.add_argument("--pass-spp", type=int, default=16)
+    ap.add_argument("--groups", type=int, default=0,
+                    help="sample groups per pass: 0 splits small batches to fill the resident waves, 1 never")
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--spp-max", type=int, default=1024)
     ap.add_argument("--small-scenes", action="store_true")
@@ -190,6 +192,7 @@ def main(argv=None):
     srv.tune("stack", args.stack)
     srv.tune("graph_passes", args.graph_passes)
     srv.tune("serial_waves", args.serial_waves)
+    srv.tune("groups", args.groups)
     cw, ch, cs = (int(v) for v in args.autotune_size.split("x"))
     tuned = srv.autotune(sids[min(1, len(sids) - 1)], cw, ch, cs) if args.autotune else None
     reqs = make_requests(args.requests, args.qps, seed=args.seed, spp_hi=args.spp_max)

@@ -87,3 +87,26 @@ def test_autotune_then_serve(asset_dir):
     out = srv.fetch(rid, 40, 24)
     _, alone = srv.render_serial(sids[0], 40, 24, 20, 3, want_image=True)
     assert np.array_equal(out.view(np.uint32), alone.view(np.uint32))
+
+
+@pytest.mark.parametrize("groups,tile", [(1, 8), (3, 8), (8, 16), (0, 32)])
+def test_sample_groups_keep_films(asset_dir, groups, tile):
+    """A pass split into sample groups (several work items per tile, each
+    rendering a share of the pass's samples into per-pool regions that a
+    fold kernel folds in sample order), never split (1), or split
+    automatically (0: small batches fill the resident waves): every film
+    equals the request rendered alone."""
+    hs = _scenes(asset_dir)
+    srv = Server(budget_bytes=1 << 30, pass_spp=8, tile=tile)
+    srv.tune("groups", groups)
+    sids = [srv.add_scene(h) for h in hs]
+    reqs = [(sids[0], 48, 40, 24, 5), (sids[1], 64, 36, 13, 9), (sids[0], 33, 17, 40, 2)]
+    ids = [srv.submit(*r) for r in reqs]
+    srv.wait_idle(120.0)
+    done = {d["id"]: d for d in srv.poll()}
+    assert len(done) == len(reqs) and all(d["status"] == 2 for d in done.values())
+    for rid, (sc, w, h, spp, seed) in zip(ids, reqs):
+        mix = srv.fetch(rid, w, h)
+        _, alone = srv.render_serial(sc, w, h, spp, seed, want_image=True)
+        assert np.array_equal(mix.view(np.uint32), alone.view(np.uint32)), (groups, rid, w, h, spp)
+    assert srv.stats()["paths"] == sum(w * h * spp for _, w, h, spp, _ in reqs)

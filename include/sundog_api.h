@@ -401,7 +401,12 @@ SUNDOG_API int sundog_server_add_scene(sundog_server*, sundog_scene*, const sund
  * persistent blocks per CU, 0 = as many as fit), "graph_passes" (1..64:
  * progressive passes captured into one graph launch), "pass_spp" (> 0),
  * "serial_waves" (0: picked per request by timing its first passes, the
- * default; or 2..5: occupancy variant of the per-request baseline). */
+ * default; or 2..5: occupancy variant of the per-request baseline), and
+ * "groups" (sample groups per pass: 0, the default, splits each tile's pass
+ * over several work items when a batch's tiles cannot fill the resident
+ * waves; 1 never splits; 2..16 always splits that many ways, at most
+ * pass_spp; passes of more than 16 samples are never split; films are
+ * bit-identical either way). */
 SUNDOG_API int sundog_server_set_tuning(sundog_server*, const char* key, int value);
 /* Before the first submit: renders a calibration request of scene_id
  * (width x height x spp) under candidate launch shapes by coordinate descent

@@ -559,6 +559,10 @@ struct PlocState {
   uint32_t rounds;   // rounds that merged
 };
 
+__global__ void k_ploc_state_init(PlocState* st, uint32_t n) {
+  if (threadIdx.x == 0) *st = PlocState{n, 0u, 0u, 0u};
+}
+
 __global__ __launch_bounds__(kLbvhBlock) void k_ploc_nn(const float4* __restrict__ cbox,
                                                         const PlocState* __restrict__ st, int radius,
                                                         uint32_t* __restrict__ nn) {
@@ -745,8 +749,7 @@ void plocTree(const float4* boxes, DevBuf& keys, int n, int radius, DevBuf& node
   nodes.alloc((size_t)ni * sizeof(BinNode));
   ScanScratch ws;
   ws.reserve(n);
-  const PlocState init{(uint32_t)n, 0u, 0u, 0u};
-  HIP_OK(hipMemcpyAsync(state.p, &init, sizeof init, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ploc_state_init, dim3(1), dim3(64), 0, s, state.as<PlocState>(), (uint32_t)n);
   hipLaunchKernelGGL(k_ploc_init, dim3(grid(n)), dim3(kLbvhBlock), 0, s, boxes, keys.as<uint64_t>(), n,
                      clA.as<uint32_t>(), bxA.as<float4>());
   HIP_OK(hipGetLastError());

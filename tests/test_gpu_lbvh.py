@@ -148,3 +148,24 @@ def test_coincident_instances_closest_hit_equals_brute_force(mode, asset_dir):
     assert set(np.unique(i[i >= 0])) <= {0, 5, 7, 8}   # ties go to the first of the coincident instances
     assert np.array_equal(i, ib) and np.array_equal(p, pb)
     assert np.array_equal(t.view(np.uint32), tb.view(np.uint32))
+
+
+def test_bvh_export_checks_buffer_sizes(asset_dir):
+    """sundog_session_bvh fills the layout with NULL buffers and refuses a
+    node or triangle buffer smaller than the layout asks for."""
+    import ctypes
+    from sundog_amd import _lib
+    sc = configs.instanced(asset_dir, instances=8, subdiv=2, width=16, height=16, spp=1).build()
+    sess = sc.session()
+    lay = _lib.BvhLayout()
+    lib = sess.lib
+    assert lib.sundog_session_bvh(sess.ptr, ctypes.byref(lay), None, 0, None, 0, None, 0) == 0
+    assert lay.num_nodes >= 2 and lay.tlas_nodes >= 1 and lay.num_tris == 320 and lay.num_instances == 9
+    small = np.zeros(lay.num_nodes * 32 - 1, np.float32)
+    assert lib.sundog_session_bvh(sess.ptr, ctypes.byref(lay), small.ctypes.data_as(ctypes.c_void_p),
+                                  small.nbytes, None, 0, None, 0) != 0
+    assert "node buffer" in _lib.last_error(lib)
+    tris = np.zeros((lay.num_tris - 1, 12), np.float32)
+    assert lib.sundog_session_bvh(sess.ptr, ctypes.byref(lay), None, 0,
+                                  tris.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), tris.nbytes, None, 0) != 0
+    assert "triangle buffer" in _lib.last_error(lib)

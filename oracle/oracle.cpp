@@ -3,6 +3,14 @@
 // source it restates. Compiled with -ffp-contract=off so that the
 // geometry functions reproduce the GPU's bits (the GPU disables contraction
 // in the same functions).
+//
+// Parity unpinned (no check outside the two restatements fixes them):
+// fbm's per-octave coordinate offset (11.7, 5.3, 7.9) and seed stride 101,
+// the flame noise coordinates' y compression 0.32, and the flame mask's
+// smoothstep edges (0.62 in d', 0.12 in h). The flame profile, heat and
+// colour ramp constants are pinned against the reference report's closed
+// forms, the hash, value noise and fbm gain/lacunarity against published
+// definitions.
 #include "oracle.h"
 
 #include <omp.h>

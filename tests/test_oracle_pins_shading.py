@@ -289,6 +289,55 @@ def test_flame_emission_and_density_are_linear_in_their_scales(orc):
     assert np.allclose(s3, 2 * s1, rtol=1e-6, atol=0) and np.array_equal(e1, e3)
 
 
+def test_flame_profile_and_ramp_follow_the_published_closed_forms(orc):
+    """The reference's report states the field in closed form
+    (docs/report/12-volumes.md:62-76): radius r(h) = 3.2 sqrt(h) (1 - h)^1.55,
+    widest at h* = 1 / (1 + 2 x 1.55) with r(h*) ~ 1; normalised distance
+    d' = d / r(h) + (n - 0.45); heat = (0.88 - d')(1.25 - h) clamped to [0, 1];
+    colour (min(2.6 heat, 1), min(2.1 heat^2, 1), min(1.7 heat^3, 1)). With
+    noise_scale 0 every point samples the noise at the origin, so n is the
+    pinned fbm's value there and each formula can be checked on its own."""
+    H, R, SIG, INT, SEED, OCT = 2.0, 0.5, 4.0, 20.0, 3, 3
+    s = sd.Scene()
+    s.render(width=8, height=8, spp=1)
+    s.camera(lookfrom=[0, 0, 5], lookat=[0, 0, 0])
+    s.lambert("a", color=[0.5] * 3)
+    s.add("sphere", "a", scale(0.01), translate(0, -50, 0))
+    s.flame(base=[0, 0, 0], height=H, radius=R, intensity=INT, sigma=SIG, noise_scale=0.0, seed=SEED,
+            octaves=OCT)
+    flat = s.build().flatten()
+    n0 = float(orc.noise(np.zeros((1, 3), np.float32), SEED, octaves=OCT)[0])
+
+    hs = np.array([0.24390244])
+    r64 = 3.2 * np.sqrt(hs) * (1 - hs) ** 1.55
+    assert abs(r64[0] - 1.0) < 0.03          # the 3.2 normalises the peak radius to ~1 (1.025)
+
+    rng = np.random.default_rng(41)
+    h = rng.uniform(0.02, 0.98, 40000)
+    dn = rng.uniform(0.0, 1.6, 40000)          # radial distance in flame radii
+    a = rng.uniform(0, 2 * np.pi, 40000)
+    p = np.stack([dn * R * np.cos(a), h * H, dn * R * np.sin(a)], axis=1).astype(np.float32)
+    sg, em = orc.flame_field(flat, 0, p)
+    # evaluate the closed forms in float64 at the float32 points' own h and d
+    hp = p[:, 1].astype(np.float64) / H
+    dp = np.hypot(p[:, 0].astype(np.float64), p[:, 2].astype(np.float64)) / R
+    r = 3.2 * np.sqrt(hp) * (1 - hp) ** 1.55
+    dd = dp / np.maximum(r, 1e-3) + (n0 - 0.45)
+    # support: density exactly where d' < 1 (away from float rounding at the edge)
+    clear = np.abs(dd - 1.0) > 1e-4
+    assert ((sg > 0) == (dd < 1.0))[clear].all()
+    inside = (sg > 0) & clear
+    assert inside.sum() > 5000 and (~inside).sum() > 5000
+    # colour = emission / (intensity x mask), mask = density / sigma
+    col = em[inside] / (INT * (sg[inside] / SIG))[:, None]
+    heat = np.clip((0.88 - dd[inside]) * (1.25 - hp[inside]), 0.0, 1.0)
+    want = np.stack([np.minimum(2.6 * heat, 1), np.minimum(2.1 * heat ** 2, 1),
+                     np.minimum(1.7 * heat ** 3, 1)], axis=1)
+    assert np.abs(col - want).max() < 2e-4
+    # the ramp is exercised below, through and above each channel's knee
+    assert (heat < 0.2).any() and ((heat > 0.5) & (heat < 0.6)).any() and (heat > 0.9).any()
+
+
 # ---------------------------------------------------------------------------
 # mesh-light sampling (src/capi_render.cpp:119-172, device/light_sample.cuh:94-154)
 # ---------------------------------------------------------------------------

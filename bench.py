@@ -17,7 +17,12 @@ cpu_baseline times the CPU oracle on a bounded sample of the same frame,
 on the host cores given to this job. A short run of the serving workload
 (four resident scenes, Poisson arrivals, continuous batching in the
 persistent kernel) adds the served and serial-megakernel rates, their
-ratio and the p50 / p99 request latencies under "serving".
+ratio and the p50 / p99 request latencies under "serving". The two
+reference gallery scenes that need no asset files (01-marble-run and
+02-cornell-lume, replayed from their recorded call programs) are rendered
+as the reference's own benchmark renders them (1920x1080, 64 spp, render
+loop wall time and Mrays/s from sundog_render's stats file) and reported
+under "reference_scenes" beside the figures the reference publishes for them.
 """
 
 import argparse
@@ -106,6 +111,8 @@ def main():
     ap.add_argument("--serving-qps-sustained", type=float, default=1.0,
                     help="arrival rate of a second run of the same requests below the served capacity, "
                          "whose p50 / p99 measure service rather than queue growth (0: skip it)")
+    ap.add_argument("--no-reference-scenes", action="store_true",
+                    help="skip the reference gallery scenes reported under 'reference_scenes'")
     ap.add_argument("--cpu-tile-stride", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=128)
     ap.add_argument("--asset-dir", default=os.path.join(tempfile.gettempdir(), "sundog_assets"))
@@ -305,6 +312,12 @@ def main():
         }
     if args.serving_requests > 0:
         out["serving"] = serving_run(args)
+    if not args.no_reference_scenes:
+        from sundog_amd import gallery
+        out["reference_scenes"] = []
+        for name in sorted(gallery.PUBLISHED):
+            print("reference scene %s: 1920x1080, 64 spp" % name, flush=True)
+            out["reference_scenes"].append(gallery.measure(gallery.program_path(name)))
     print(json.dumps(out), flush=True)
 
 

@@ -989,9 +989,32 @@ struct sundog_session {
     bvhGpuMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
+  // While the occupancy variant is being timed (auto mode), a pass over a
+  // large frame runs as trial sub-passes of a quarter of its samples, so the
+  // search costs about one pass's worth of samples instead of several whole
+  // passes spent partly on slow variants (a 64-spp render in 16-spp passes
+  // would otherwise end before it uses the variant it found); once a variant
+  // is chosen, the rest of the pass is one launch. Progressive passes give
+  // the film of a single pass bit for bit, so the split changes no result.
+  // Frames whose quarter pass is below kTrialMinPaths paths are not split:
+  // such launches are too short to time reliably.
+  static constexpr double kTrialMinPaths = 2.0e6;
   void renderPass(int offset, int count) {
     if (count <= 0) return;
     if (offset < 0) throw std::runtime_error("render: negative sample offset");
+    int done = 0;
+    const int sub = (count + 3) / 4;
+    if (kernel != SUNDOG_KERNEL_WAVEFRONT && !countWork && count >= 4 &&
+        (double)P.width * P.height * sub >= kTrialMinPaths)
+      while (done < count && waves == 0 && autoPick.chosen == 0) {
+        const int c = std::min(sub, count - done);
+        launchPass(offset + done, c);
+        done += c;
+      }
+    if (done < count) launchPass(offset + done, count - done);
+  }
+
+  void launchPass(int offset, int count) {
     if (kernel == SUNDOG_KERNEL_WAVEFRONT) {
       wavefrontPass(offset, count);
       return;

@@ -172,7 +172,7 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
     s->groupPick.exhaustive = true;
     for (int i = 0; i < 32 && ((tuneWaves && s->autoPick.chosen == 0) || (tuneGroups && s->groupPick.chosen == 0));
          i++)
-      s->renderPass(0, count);
+      s->launchPass(0, count);  // whole passes: the autotuner times full-size launches
     if (tuneWaves && tuneGroups && s->groupPick.chosen != 0) {
       // the occupancy variant again, at the chosen groups (the groups change
       // how many blocks are resident at once, so the best variant can move)
@@ -180,7 +180,7 @@ extern "C" int sundog_session_autotune(sundog_session* s, int count, int* waves_
       s->autoPick = OccupancyPicker{};
       s->autoPick.rounds = 2;
       s->autoPick.exhaustive = true;
-      for (int i = 0; i < 9 && s->autoPick.chosen == 0; i++) s->renderPass(0, count);
+      for (int i = 0; i < 9 && s->autoPick.chosen == 0; i++) s->launchPass(0, count);
       s->sampleGroups = 0;  // auto mode, with the picker's choice kept
     }
     s->countWork = counting;

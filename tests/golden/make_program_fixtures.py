@@ -19,7 +19,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_SCENES = "/root/reference/scenes"
-SCENES = ["smoke", "02-cornell-lume", "features", "06-spot-cascade", "09-ember-shore", "10-suncatcher", "14-toy-factory"]
+SCENES = ["smoke", "01-marble-run", "02-cornell-lume", "features", "06-spot-cascade", "09-ember-shore", "10-suncatcher", "14-toy-factory"]
 
 
 def main():

@@ -17,7 +17,7 @@ import sundog_amd as sd
 from sundog_amd import scenelib
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FIXTURES = ["smoke", "02-cornell-lume", "features", "06-spot-cascade", "09-ember-shore", "10-suncatcher", "14-toy-factory"]
+FIXTURES = ["smoke", "01-marble-run", "02-cornell-lume", "features", "06-spot-cascade", "09-ember-shore", "10-suncatcher", "14-toy-factory"]
 
 
 def _load(name):

@@ -82,6 +82,51 @@ def measure(path, width=1920, height=1080, spp=64, lib=None, keep_dir=None):
     return out
 
 
+def read_pq_ppm(path):
+    """The 12-bit PQ / BT.2020 code values `sundog_render` writes beside its
+    PFM (`<out>.pq.ppm`, 16-bit big-endian, maxval 4095), as (h, w, 3)."""
+    import numpy as np
+    with open(path, "rb") as f:
+        data = f.read()
+    head, pos = [], 0
+    while len(head) < 4:
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        end = pos
+        while not data[end:end + 1].isspace():
+            end += 1
+        head.append(data[pos:end])
+        pos = end
+    pos += 1
+    if head[0] != b"P6" or int(head[3]) != 4095:
+        raise ValueError("%s: not a 12-bit PQ PPM" % path)
+    w, h = int(head[1]), int(head[2])
+    return np.frombuffer(data, ">u2", w * h * 3, pos).reshape(h, w, 3).astype(np.uint16)
+
+
+def psnr_codes(a, b, peak=4095.0):
+    """PSNR between two code-value images, as the reference's img_compare
+    computes it (tests/tools/img_compare.cpp:86-93): mean over every RGB
+    sample of ((a - b) / peak)^2, 10 log10(1 / mse)."""
+    import numpy as np
+    d = (a.astype(np.float64) - b.astype(np.float64)) / peak
+    mse = float(np.mean(d * d))
+    return float("inf") if mse <= 0.0 else 10.0 * np.log10(1.0 / mse)
+
+
+def render_codes(path, width, height, spp, seed, workdir, lib=None):
+    """Renders the recorded program at `path` through `sundog_render`
+    (no denoising) and returns its 12-bit PQ code values."""
+    lib = lib or _lib.load()
+    h = SceneHandle(apply_program(load_program(path), lib), lib)
+    try:
+        out = os.path.join(workdir, "%s_%d.pfm" % (scene_name(path), spp))
+        h.render_file(out, width=width, height=height, spp=spp, seed=seed, denoise=0)
+    finally:
+        h.close()
+    return read_pq_ppm(out[:-4] + ".pq.ppm")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("programs", nargs="*", help="program_<scene>.json files (default: the recorded gallery scenes)")

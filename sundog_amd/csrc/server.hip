@@ -742,10 +742,9 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
     // variant, then sample groups, from timed passes)
     OccupancyPicker pick;
     GroupPicker gpick;
-    auto t = Clock::now();
-    for (int done = 0; done < spp; done += 16) {
-      P.sampleOffset = done;
-      P.sppThisLaunch = std::min(16, spp - done);
+    auto launchTimed = [&](int offset, int count) {
+      P.sampleOffset = offset;
+      P.sppThisLaunch = count;
       bool trial = false, gTrial = false;
       const int w = serialWaves != 0 ? serialWaves : pick.pick(trial);
       int groups = 1;
@@ -762,6 +761,21 @@ extern "C" int sundog_server_render_serial(sundog_server* s, int scene_id, int w
                                ((double)width * height * P.sppThisLaunch);
       if (trial) pick.record(w, msPerPath);
       if (gTrial) gpick.record(groups, msPerPath);
+    };
+    // 16-spp passes; while the occupancy variant is being timed, a large
+    // frame's pass runs as quarter-pass trial launches, as a session's does
+    auto t = Clock::now();
+    for (int done = 0; done < spp; done += 16) {
+      const int count = std::min(16, spp - done);
+      const int sub = (count + 3) / 4;
+      int d = 0;
+      if (serialWaves == 0 && count >= 4 && (double)width * height * sub >= sundog_session::kTrialMinPaths)
+        while (d < count && pick.chosen == 0) {
+          const int c = std::min(sub, count - d);
+          launchTimed(done + d, c);
+          d += c;
+        }
+      if (d < count) launchTimed(done + d, count - d);
     }
     double ms = std::chrono::duration<double, std::milli>(Clock::now() - t).count();
     if (ms_out) *ms_out = ms;
